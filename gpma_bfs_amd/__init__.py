@@ -1,0 +1,193 @@
+"""gpma_bfs_amd: an MI355X (gfx950) GPMA for dynamic graphs.
+
+Host-side mirror of the reference's C++ interface (gpma.cuh, gpma_bfs.cuh):
+
+    g = GPMA(row_num)                      # GPMA<GPU>(row_num), gpma.cuh:79-82
+    update_gpma(g, keys, values)           # gpma.cuh:968-969
+    levels, reachable = gpma_bfs(g, start)  # gpma_bfs.cuh:517-533 (+ demo's count)
+
+keys are uint64 ``src << 32 | dst``; values are uint8, 0 = delete. Arrays
+may be numpy arrays (host) or torch tensors on the GPU (passed by device
+pointer). Everything runs through libgpma.so's C ABI; there is no CPU path.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import (  # noqa: F401
+    COL_IDX_NONE,
+    GPMA_ERR_ARG,
+    GPMA_ERR_CAPACITY,
+    GPMA_ERR_KEY_RANGE,
+    GPMA_OK,
+    KEY_MAX,
+    KEY_NONE,
+    GpmaConfig,
+    GpmaCsr,
+    GpmaError,
+    GpmaStats,
+    check,
+    last_error,
+    lib,
+)
+
+
+def _ptr(a, dtype):
+    """(pointer, length, on_device) for a numpy array or a torch tensor."""
+    if isinstance(a, np.ndarray):
+        if a.dtype != dtype:
+            raise TypeError(f"expected {np.dtype(dtype)}, got {a.dtype}")
+        if not a.flags.c_contiguous:
+            raise ValueError("array must be C-contiguous")
+        return a.ctypes.data, a.shape[0], 0
+    data_ptr = getattr(a, "data_ptr", None)
+    if data_ptr is not None:  # torch tensor
+        if not a.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        return data_ptr(), a.shape[0], 1 if a.is_cuda else 0
+    raise TypeError(f"unsupported array type {type(a)}")
+
+
+class GPMA:
+    """One device-resident GPMA (reference: class GPMA, gpma.cuh:31-102)."""
+
+    def __init__(self, row_num, device=-1, bulk_threshold=0):
+        cfg = GpmaConfig(ctypes.sizeof(GpmaConfig), device, bulk_threshold)
+        h = ctypes.c_void_p()
+        check(lib.gpma_create(row_num, ctypes.byref(cfg), ctypes.byref(h)), "gpma_create")
+        self._h = h
+        self.row_num = row_num
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise GpmaError(GPMA_ERR_ARG, "GPMA already closed")
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            lib.gpma_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def stats(self):
+        s = GpmaStats()
+        check(lib.gpma_get_stats(self.handle, ctypes.byref(s)), "gpma_get_stats")
+        return {name: getattr(s, name) for name, _ in GpmaStats._fields_}
+
+    @property
+    def segment_length(self):
+        return self.stats()["segment_length"]
+
+    @property
+    def tree_height(self):
+        return self.stats()["tree_height"]
+
+    def get_size(self):
+        """Capacity in slots (reference get_size(), gpma.cuh:52)."""
+        return self.stats()["capacity"]
+
+    def csr(self):
+        """Device pointers and shape (reference public members, gpma.cuh:34-50)."""
+        c = GpmaCsr()
+        check(lib.gpma_get_csr(self.handle, ctypes.byref(c)), "gpma_get_csr")
+        return c
+
+    def copy_layout(self):
+        """Host copies of keys, values, row_offset (reference mirror(), gpma.cuh:90-101)."""
+        C = self.get_size()
+        k = np.empty(C, np.uint64)
+        v = np.empty(C, np.uint8)
+        r = np.empty(self.row_num + 1, np.uint32)
+        check(lib.gpma_copy_layout(self.handle, k.ctypes.data, v.ctypes.data, r.ctypes.data), "gpma_copy_layout")
+        return k, v, r
+
+    def export_live(self):
+        """Sorted live keys (guards and the two KEY_MAX sentinels included)."""
+        n = ctypes.c_uint64()
+        check(lib.gpma_export_live(self.handle, None, 0, ctypes.byref(n)), "gpma_export_live")
+        out = np.empty(n.value, np.uint64)
+        check(lib.gpma_export_live(self.handle, out.ctypes.data, n.value, ctypes.byref(n)), "gpma_export_live")
+        return out[: n.value]
+
+    def sync(self):
+        check(lib.gpma_sync(self.handle), "gpma_sync")
+
+
+def init_csr_gpma(row_num, **kw):
+    """Older initialiser name the reference's drivers log (gpma_bfs_demo.cu:63)."""
+    return GPMA(row_num, **kw)
+
+
+def update_gpma(gpma, keys, values, strict=True):
+    """Apply one update batch (reference update_gpma, gpma.cuh:968-1050).
+
+    Unlike the reference, the inputs are not sorted or compacted in place.
+    Returns the status code; with strict=True a non-zero status raises
+    (GPMA_ERR_KEY_RANGE still applies the valid part of the batch)."""
+    kp, n, kdev = _ptr(keys, np.uint64)
+    vp, nv, vdev = _ptr(values, np.uint8)
+    if n != nv:
+        raise GpmaError(GPMA_ERR_ARG, f"keys ({n}) and values ({nv}) differ in length")
+    if kdev != vdev:
+        raise GpmaError(GPMA_ERR_ARG, "keys and values must both be on the host or both on the device")
+    rc = lib.gpma_update(gpma.handle, kp, vp, n, kdev)
+    if strict:
+        check(rc, "gpma_update")
+    return rc
+
+
+def gpma_bfs(gpma, start_node, results=None):
+    """BFS levels from start_node (reference gpma_bfs.cuh:517-533).
+
+    Returns (levels, reachable): levels[v] = 1 + hop distance, 0 if
+    unreachable; reachable = number of non-zero levels (gpma_bfs_demo.cu:80).
+    results may be a preallocated uint32 numpy array or a torch device tensor."""
+    if results is None:
+        results = np.empty(gpma.row_num, np.uint32)
+    rp, n, dev = _ptr(results, np.uint32) if isinstance(results, np.ndarray) else _ptr(results, None)
+    if n < gpma.row_num:
+        raise GpmaError(GPMA_ERR_ARG, "results shorter than row_num")
+    reach = ctypes.c_uint32()
+    check(lib.gpma_bfs(gpma.handle, start_node, rp, dev, ctypes.byref(reach)), "gpma_bfs")
+    return results, reach.value
+
+
+def gpma_csr_bfs(keys_ptr, values_ptr, row_offsets_ptr, node_size, edge_size, start_node):
+    """Legacy overload over raw device pointers (reference gpma_bfs.cuh:511-515)."""
+    out = np.empty(node_size, np.uint32)
+    check(lib.gpma_csr_bfs(keys_ptr, values_ptr, row_offsets_ptr, node_size, edge_size, start_node, out.ctypes.data),
+          "gpma_csr_bfs")
+    return out
+
+
+def gen_rmat(scale, ndraws, a=0.57, b=0.19, c=0.19, seed=1):
+    """Seeded R-MAT edge list: unique, no self-loops, shuffled (src, dst uint32)."""
+    n = ctypes.c_uint64()
+    check(lib.gpma_gen_rmat(scale, ndraws, a, b, c, seed, None, None, 0, ctypes.byref(n)), "gpma_gen_rmat")
+    src = np.empty(n.value, np.uint32)
+    dst = np.empty(n.value, np.uint32)
+    check(lib.gpma_gen_rmat(scale, ndraws, a, b, c, seed, src.ctypes.data, dst.ctypes.data, n.value,
+                            ctypes.byref(n)), "gpma_gen_rmat")
+    return src, dst
+
+
+def gen_powerlaw(N, E, alpha=2.1, soft_cap=2800.0, p1=0.24, seed=0):
+    """Seeded power-law out-degree graph with exactly E unique edges."""
+    src = np.empty(E, np.uint32)
+    dst = np.empty(E, np.uint32)
+    check(lib.gpma_gen_powerlaw(N, E, alpha, soft_cap, p1, seed, src.ctypes.data, dst.ctypes.data),
+          "gpma_gen_powerlaw")
+    return src, dst
+
+
+def make_keys(src, dst):
+    return (src.astype(np.uint64) << np.uint64(32)) | dst.astype(np.uint64)

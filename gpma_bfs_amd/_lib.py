@@ -1,0 +1,122 @@
+"""ctypes binding of libgpma.so (the C ABI declared in include/gpma.h).
+
+The library is the only compute path: if it is missing or fails to load,
+importing this module raises, and nothing falls back to a CPU path.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgpma.so")
+
+GPMA_OK = 0
+GPMA_ERR_ARG = 1
+GPMA_ERR_HIP = 2
+GPMA_ERR_OOM = 3
+GPMA_ERR_CAPACITY = 4
+GPMA_ERR_KEY_RANGE = 5
+GPMA_ERR_NO_DEVICE = 6
+
+ERROR_NAMES = {
+    GPMA_ERR_ARG: "GPMA_ERR_ARG",
+    GPMA_ERR_HIP: "GPMA_ERR_HIP",
+    GPMA_ERR_OOM: "GPMA_ERR_OOM",
+    GPMA_ERR_CAPACITY: "GPMA_ERR_CAPACITY",
+    GPMA_ERR_KEY_RANGE: "GPMA_ERR_KEY_RANGE",
+    GPMA_ERR_NO_DEVICE: "GPMA_ERR_NO_DEVICE",
+}
+
+KEY_NONE = 0xFFFFFFFFFFFFFFFF
+KEY_MAX = 0xFFFFFFFFFFFFFFFE
+COL_IDX_NONE = 0xFFFFFFFF
+
+
+class GpmaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERROR_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class GpmaConfig(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32),
+        ("device", ctypes.c_int32),
+        ("bulk_threshold", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 5),
+    ]
+
+
+class GpmaCsr(ctypes.Structure):
+    _fields_ = [
+        ("keys", ctypes.c_void_p),
+        ("values", ctypes.c_void_p),
+        ("row_offset", ctypes.c_void_p),
+        ("capacity", ctypes.c_uint64),
+        ("segment_length", ctypes.c_uint32),
+        ("tree_height", ctypes.c_uint32),
+        ("row_num", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+class GpmaStats(ctypes.Structure):
+    _fields_ = [
+        ("capacity", ctypes.c_uint64),
+        ("segment_length", ctypes.c_uint32),
+        ("tree_height", ctypes.c_uint32),
+        ("row_num", ctypes.c_uint32),
+        ("last_levels", ctypes.c_uint32),
+        ("updates_applied", ctypes.c_uint64),
+        ("invalid_skipped", ctypes.c_uint64),
+        ("dropped", ctypes.c_uint64),
+        ("resizes", ctypes.c_uint64),
+        ("bulk_loads", ctypes.c_uint64),
+        ("device_bytes", ctypes.c_uint64),
+    ]
+
+
+# name -> (restype, argtypes); mirrors include/gpma.h one to one
+_P = ctypes.c_void_p
+_U32 = ctypes.c_uint32
+_U64 = ctypes.c_uint64
+_I = ctypes.c_int
+SIGNATURES = {
+    "gpma_create": (_I, [_U32, _P, _P]),
+    "gpma_destroy": (_I, [_P]),
+    "gpma_update": (_I, [_P, _P, _P, _U64, _I]),
+    "gpma_bfs": (_I, [_P, _U32, _P, _I, _P]),
+    "gpma_csr_bfs": (_I, [_P, _P, _P, _U32, _U32, _U32, _P]),
+    "gpma_get_csr": (_I, [_P, _P]),
+    "gpma_copy_layout": (_I, [_P, _P, _P, _P]),
+    "gpma_export_live": (_I, [_P, _P, _U64, _P]),
+    "gpma_get_stats": (_I, [_P, _P]),
+    "gpma_sync": (_I, [_P]),
+    "gpma_stream": (_P, [_P]),
+    "gpma_last_error": (ctypes.c_char_p, []),
+    "gpma_gen_rmat": (_I, [_U32, _U64, ctypes.c_double, ctypes.c_double, ctypes.c_double, _U64, _P, _P, _U64, _P]),
+    "gpma_gen_powerlaw": (_I, [_U32, _U64, ctypes.c_double, ctypes.c_double, ctypes.c_double, _U64, _P, _P]),
+}
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run gpma_bfs_amd/build.py (or __graft_entry__.build())")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def last_error():
+    return (lib.gpma_last_error() or b"").decode(errors="replace")
+
+
+def check(rc, what=""):
+    if rc != GPMA_OK:
+        raise GpmaError(rc, f"{what}: {last_error()}")
+    return rc

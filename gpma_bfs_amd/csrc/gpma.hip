@@ -1,0 +1,609 @@
+// C ABI and host orchestration of the gfx950 GPMA.
+//
+// Device state (structure of arrays in HBM, one allocation each):
+//   keys[C] u64, values[C] u8       the gapped packed array (C = S << h)
+//   seps[C/S] u64                   first key of every segment (== keys[j*S]),
+//                                   the search array of locate
+//   row_offset[N+1] u32             row_offset[v+1] = slot of guard v
+// The host keeps S, h and the per-level bounds (gpma.cuh:37-46).
+//
+// update_gpma (gpma.cuh:968-1050) becomes: stage + radix sort (sort.hip),
+// locate, compaction of the pending inserts, one leaf-level kernel, then
+// per higher level either one LDS window kernel or the device-wide window
+// passes, a compaction, and one host read of the pending count; a resize or
+// a bulk load rebuilds the array with the same device-wide passes. The
+// common case (every insert placed at level 0 or 1) reads two counters back
+// per batch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gpma.h"
+#include "common.h"
+#include "kernels.h"
+
+using namespace gpma;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                     \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess)                                                                             \
+            return fail(e_ == hipErrorOutOfMemory ? GPMA_ERR_OOM : GPMA_ERR_HIP,                          \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                               \
+    } while (0)
+
+#define TRY(expr)                 \
+    do {                          \
+        int r_ = (expr);          \
+        if (r_ != GPMA_OK) return r_; \
+    } while (0)
+
+// gpma.cuh:104-126
+uint32_t fls32(uint32_t x) { return x ? 32u - (uint32_t)__builtin_clz(x) : 0u; }
+
+// Device buffer that only grows.
+struct Buf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need, uint64_t *total) {
+        if (need <= bytes && p) return GPMA_OK;
+        size_t nb = std::max(need, bytes + bytes / 2);
+        if (nb < 256) nb = 256;
+        if (p) {
+            (void)hipFree(p);
+            *total -= bytes;
+            p = nullptr;
+            bytes = 0;
+        }
+        hipError_t e = hipMalloc(&p, nb);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(GPMA_ERR_OOM, "hipMalloc(" + std::to_string(nb) + "): " + hipGetErrorString(e));
+        }
+        bytes = nb;
+        *total += nb;
+        return GPMA_OK;
+    }
+    void release(uint64_t *total) {
+        if (p) {
+            (void)hipFree(p);
+            *total -= bytes;
+        }
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename T>
+    T *as() const {
+        return (T *)p;
+    }
+};
+
+enum Ctr : uint32_t {
+    C_INVALID = 0,
+    C_NP0 = 1,
+    C_NP1 = 2,
+    C_SORTERR = 3,
+    C_LIVE = 4,
+    C_NEXT = 5,
+    C_NRUNS = 6,
+    C_COUNT = 16
+};
+
+}  // namespace
+
+struct gpma_handle {
+    int device = 0;
+    hipStream_t st = nullptr;
+    uint32_t N = 0;
+    uint64_t C = 0;
+    uint32_t S = 0, h = 0;
+    std::vector<uint32_t> lower, upper;
+    uint32_t bulk_threshold = 5u * 1000u * 1000u;
+    uint64_t dev_bytes = 0;
+    // structure
+    Buf keys, values, seps, row_offset;
+    // update scratch
+    Buf in_k, in_v, sk0, sv0, sk1, sv1, win, blkcnt, blkoff, sortwork;
+    Buf pk0, pv0, pw0, pd0, pk1, pv1, pw1, pd1;
+    Buf tmpk, tmpv, chunkcnt, chunkoff;
+    // bfs scratch
+    Buf lev, fr0, fr1;
+    Buf ctr;
+    uint32_t *hctr = nullptr;  // pinned mirror of ctr
+    // stats
+    uint64_t applied = 0, invalid = 0, dropped = 0, resizes = 0, bulk = 0;
+    uint32_t last_levels = 0;
+};
+
+namespace {
+
+// gpma.cuh:147-168
+void recalc_density(gpma_handle *g) {
+    const double lower_leaf = 0.08, lower_root = 0.42, upper_leaf = 0.92, upper_root = 0.84;
+    g->lower.assign(g->h + 1, 0);
+    g->upper.assign(g->h + 1, 0);
+    uint64_t len = g->S;
+    for (uint32_t i = 0; i <= g->h; i++) {
+        double dl = lower_root + (lower_leaf - lower_root) * (g->h - i) / g->h;
+        double du = upper_root + (upper_leaf - upper_root) * (g->h - i) / g->h;
+        g->lower[i] = (uint32_t)std::ceil(dl * (double)len);
+        g->upper[i] = (uint32_t)std::floor(du * (double)len);
+        if (i > 0) {
+            g->lower[i] = std::max(g->lower[i], 2 * g->lower[i - 1]);
+            g->upper[i] = std::min(g->upper[i], 2 * g->upper[i - 1]);
+        }
+        len <<= 1;
+    }
+}
+
+int sync_read_ctr(gpma_handle *g) {
+    HIP_TRY(hipMemcpyAsync(g->hctr, g->ctr.p, C_COUNT * sizeof(uint32_t), hipMemcpyDeviceToHost, g->st));
+    HIP_TRY(hipStreamSynchronize(g->st));
+    HIP_TRY(hipGetLastError());
+    return GPMA_OK;
+}
+
+int ensure_tmp(gpma_handle *g, uint64_t n) {
+    TRY(g->tmpk.ensure(n * 8, &g->dev_bytes));
+    TRY(g->tmpv.ensure(n, &g->dev_bytes));
+    uint64_t nch = (n + CHUNK - 1) / CHUNK + 1;
+    TRY(g->chunkcnt.ensure(nch * 4, &g->dev_bytes));
+    TRY(g->chunkoff.ensure(nch * 4, &g->dev_bytes));
+    return GPMA_OK;
+}
+
+// Live count of window [w, w+W) (one host sync), leaving chunk offsets for
+// a following compact_live.
+int count_live(gpma_handle *g, uint64_t w, uint64_t W, uint64_t *c) {
+    TRY(ensure_tmp(g, std::max<uint64_t>(W, 1)));
+    uint64_t nch = (W + CHUNK - 1) / CHUNK;
+    launch_count_live(g->keys.as<gkey>(), g->values.as<gval>(), w, W, g->chunkcnt.as<uint32_t>(), g->st);
+    launch_scan_blocks(g->chunkcnt.as<uint32_t>(), nch, g->chunkoff.as<uint32_t>(), g->ctr.as<uint32_t>() + C_LIVE,
+                       g->st);
+    TRY(sync_read_ctr(g));
+    *c = g->hctr[C_LIVE];
+    return GPMA_OK;
+}
+
+// resize_gpma + root rebalance (gpma.cuh:906-934,1034-1044) when
+// check_bounds, or significant_insert (gpma.cuh:936-966) when not: compact
+// every live item, size the array to the smallest power of two >= 4 with
+// floor(0.84 * C') >= live + pending, and redistribute the merge of the live
+// items and the pending inserts over the whole new array.
+int rebuild(gpma_handle *g, const gkey *pk, const gval *pv, uint64_t np, bool check_bounds) {
+    uint64_t c = 0;
+    TRY(count_live(g, 0, g->C, &c));
+    TRY(ensure_tmp(g, g->C));
+    launch_compact_live(g->keys.as<gkey>(), g->values.as<gval>(), 0, g->C, g->chunkoff.as<uint32_t>(),
+                        g->tmpk.as<gkey>(), g->tmpv.as<gval>(), g->st);
+    uint64_t merge = c + np;
+    uint64_t T = 4;
+    while (std::floor(0.84 * (double)T) < (double)merge) T *= 2;
+    if (T > (1ull << 31)) return fail(GPMA_ERR_CAPACITY, "packed array would exceed 2^31 slots");
+    g->S = 1u << (fls32(fls32((uint32_t)T)) - 1);
+    g->h = fls32((uint32_t)(T / g->S)) - 1;
+    recalc_density(g);
+    if (T != g->C) {
+        // tmp already holds every live item, so the old arrays can go first
+        HIP_TRY(hipStreamSynchronize(g->st));
+        g->keys.release(&g->dev_bytes);
+        g->values.release(&g->dev_bytes);
+        g->seps.release(&g->dev_bytes);
+        TRY(g->keys.ensure(T * 8, &g->dev_bytes));
+        TRY(g->values.ensure(T, &g->dev_bytes));
+        TRY(g->seps.ensure((T / g->S) * 8, &g->dev_bytes));
+    }
+    g->C = T;
+    if (check_bounds) g->resizes++;
+    else g->bulk++;
+    const uint32_t lo = g->lower[g->h], hi = g->upper[g->h];
+    if (!check_bounds || (lo <= merge && merge <= hi)) {
+        launch_merge_redistribute(g->tmpk.as<gkey>(), g->tmpv.as<gval>(), c, pk, pv, np, g->keys.as<gkey>(),
+                                  g->values.as<gval>(), g->seps.as<gkey>(), g->row_offset.as<uint32_t>(), 0, T,
+                                  g->S, g->st);
+    } else {
+        // The reference leaves the compacted array and drops the inserts.
+        launch_write_compacted(g->tmpk.as<gkey>(), g->tmpv.as<gval>(), c, g->keys.as<gkey>(),
+                               g->values.as<gval>(), T, g->seps.as<gkey>(), g->row_offset.as<uint32_t>(), g->S,
+                               g->st);
+        g->dropped += np;
+    }
+    return GPMA_OK;
+}
+
+int init_structure(gpma_handle *g) {
+    // gpma.cuh:54-69: keys [MAX, NONE, MAX, NONE], values [1, 0, 1, 0], S = 2, h = 1
+    g->C = 4;
+    g->S = 2;
+    g->h = 1;
+    recalc_density(g);
+    TRY(g->keys.ensure(4 * 8, &g->dev_bytes));
+    TRY(g->values.ensure(4, &g->dev_bytes));
+    TRY(g->seps.ensure(2 * 8, &g->dev_bytes));
+    TRY(g->row_offset.ensure(((uint64_t)g->N + 1) * 4, &g->dev_bytes));
+    gkey hk[4] = {KEY_MAX, KEY_NONE, KEY_MAX, KEY_NONE};
+    gval hv[4] = {1, 0, 1, 0};
+    gkey hs[2] = {KEY_MAX, KEY_MAX};
+    HIP_TRY(hipMemcpyAsync(g->keys.p, hk, sizeof(hk), hipMemcpyHostToDevice, g->st));
+    HIP_TRY(hipMemcpyAsync(g->values.p, hv, sizeof(hv), hipMemcpyHostToDevice, g->st));
+    HIP_TRY(hipMemcpyAsync(g->seps.p, hs, sizeof(hs), hipMemcpyHostToDevice, g->st));
+    fill_u32(g->row_offset.as<uint32_t>(), 0, (uint64_t)g->N + 1, g->st);
+    HIP_TRY(hipStreamSynchronize(g->st));
+    return GPMA_OK;
+}
+
+int ensure_update_buffers(gpma_handle *g, uint64_t n) {
+    uint64_t *t = &g->dev_bytes;
+    TRY(g->in_k.ensure(n * 8, t));
+    TRY(g->in_v.ensure(n, t));
+    TRY(g->sk0.ensure(n * 8, t));
+    TRY(g->sv0.ensure(n, t));
+    TRY(g->sk1.ensure(n * 8, t));
+    TRY(g->sv1.ensure(n, t));
+    TRY(g->win.ensure(n * 4, t));
+    uint64_t nb = (n + CMP_BLOCK - 1) / CMP_BLOCK + 1;
+    TRY(g->blkcnt.ensure(nb * 4, t));
+    TRY(g->blkoff.ensure(nb * 4, t));
+    TRY(g->sortwork.ensure((8 * 256 + 8 + 8 + sort_status_words(n, 8)) * 4, t));
+    TRY(g->pk0.ensure(n * 8, t));
+    TRY(g->pv0.ensure(n, t));
+    TRY(g->pw0.ensure(n * 4, t));
+    TRY(g->pd0.ensure(n, t));
+    TRY(g->pk1.ensure(n * 8, t));
+    TRY(g->pv1.ensure(n, t));
+    TRY(g->pw1.ensure(n * 4, t));
+    TRY(g->pd1.ensure(n, t));
+    return GPMA_OK;
+}
+
+int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint64_t n, int on_device,
+              bool guards_ok = false) {
+    if (n == 0) return GPMA_OK;
+    if (!keys || !values) return fail(GPMA_ERR_ARG, "gpma_update: null keys or values");
+    if (n >= (1ull << 30)) return fail(GPMA_ERR_CAPACITY, "gpma_update: batch larger than 2^30 records");
+    HIP_TRY(hipSetDevice(g->device));
+    TRY(ensure_update_buffers(g, n));
+    hipStream_t st = g->st;
+    uint32_t *ctr = g->ctr.as<uint32_t>();
+    fill_u32(ctr, 0, C_COUNT, st);
+    const gkey *src_k = keys;
+    const gval *src_v = values;
+    if (!on_device) {
+        HIP_TRY(hipMemcpyAsync(g->in_k.p, keys, n * 8, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(g->in_v.p, values, n, hipMemcpyHostToDevice, st));
+        src_k = g->in_k.as<gkey>();
+        src_v = g->in_v.as<gval>();
+    }
+    launch_stage(src_k, src_v, g->sk0.as<gkey>(), g->sv0.as<gval>(), n, g->N, guards_ok, ctr + C_INVALID, st);
+    // digits: bits of N-1 plus one (so KEY_NONE sorts after every valid key)
+    uint32_t nb = std::max<uint32_t>(1, fls32(g->N ? g->N - 1 : 0)) + 1;
+    uint32_t dpw = std::min<uint32_t>(4, (nb + 7) / 8);
+    SortShifts sh{};
+    uint32_t npass = 0;
+    for (uint32_t d = 0; d < dpw; d++) sh.s[npass++] = 8 * d;
+    for (uint32_t d = 0; d < dpw; d++) sh.s[npass++] = 32 + 8 * d;
+    int cur = radix_sort_pairs(g->sk0.as<gkey>(), g->sv0.as<gval>(), g->sk1.as<gkey>(), g->sv1.as<gval>(), n, sh,
+                               npass, g->sortwork.as<uint32_t>(), ctr + C_SORTERR, st);
+    const gkey *ks = cur ? g->sk1.as<gkey>() : g->sk0.as<gkey>();
+    const gval *vs = cur ? g->sv1.as<gval>() : g->sv0.as<gval>();
+    // locate + in-place delete / modify
+    launch_locate(ks, vs, n, g->keys.as<gkey>(), g->values.as<gval>(), g->seps.as<gkey>(), g->C / g->S, g->S,
+                  g->win.as<uint32_t>(), g->blkcnt.as<uint32_t>(), st);
+    uint64_t nblk = (n + CMP_BLOCK - 1) / CMP_BLOCK;
+    launch_scan_blocks(g->blkcnt.as<uint32_t>(), nblk, g->blkoff.as<uint32_t>(), ctr + C_NP0, st);
+    launch_compact(ks, vs, g->win.as<uint32_t>(), nullptr, n, nullptr, g->blkoff.as<uint32_t>(), g->pk0.as<gkey>(),
+                   g->pv0.as<gval>(), g->pw0.as<uint32_t>(), 0xFFFFFFFFu, st);
+    TRY(sync_read_ctr(g));
+    if (g->hctr[C_SORTERR]) return fail(GPMA_ERR_HIP, "radix sort look-back exceeded its spin limit");
+    const uint64_t inval = g->hctr[C_INVALID];
+    g->invalid += inval;
+    g->applied += n - inval;
+    uint64_t np = g->hctr[C_NP0];
+    g->last_levels = 0;
+    // pending list ping-pong: (pk, pv, pw, pd)[cur] with its count in ctr[np_slot]
+    Buf *PK[2] = {&g->pk0, &g->pk1}, *PV[2] = {&g->pv0, &g->pv1}, *PW[2] = {&g->pw0, &g->pw1},
+        *PD[2] = {&g->pd0, &g->pd1};
+    uint32_t npslot[2] = {C_NP0, C_NP1};
+    int pc = 0;
+    if (np >= g->bulk_threshold) {
+        TRY(rebuild(g, g->pk0.as<gkey>(), g->pv0.as<gval>(), np, false));
+        HIP_TRY(hipStreamSynchronize(g->st));
+        HIP_TRY(hipGetLastError());
+        return inval ? fail(GPMA_ERR_KEY_RANGE, std::to_string(inval) + " keys with src/dst >= row_num skipped")
+                     : GPMA_OK;
+    }
+    uint32_t level = 0;
+    while (np > 0 && level <= g->h) {
+        const uint64_t W = (uint64_t)g->S << level;
+        gkey *pk = PK[pc]->as<gkey>();
+        gval *pv = PV[pc]->as<gval>();
+        uint32_t *pw = PW[pc]->as<uint32_t>();
+        uint8_t *pd = PD[pc]->as<uint8_t>();
+        uint32_t *d_np = ctr + npslot[pc];
+        fill_u8(pd, 0, np, st);
+        g->last_levels = level + 1;
+        if (level == 0) {
+            launch_leaf(g->S, pk, pv, pw, pd, np, d_np, g->keys.as<gkey>(), g->values.as<gval>(),
+                        g->seps.as<gkey>(), g->row_offset.as<uint32_t>(), g->lower[0], g->upper[0], st);
+        } else if (W <= WIN_MAX) {
+            launch_window(g->S, (uint32_t)W, pk, pv, pw, pd, np, d_np, g->keys.as<gkey>(), g->values.as<gval>(),
+                          g->seps.as<gkey>(), g->row_offset.as<uint32_t>(), g->lower[level], g->upper[level], st);
+        } else {
+            // wide windows: runs found on the host from the window ids
+            std::vector<uint32_t> hw(np);
+            HIP_TRY(hipMemcpyAsync(hw.data(), pw, np * 4, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            uint64_t a = 0;
+            while (a < np) {
+                uint64_t b = a + 1;
+                while (b < np && hw[b] == hw[a]) b++;
+                const uint64_t r = b - a;
+                uint64_t c = 0;
+                TRY(count_live(g, hw[a], W, &c));
+                const uint64_t m = c + r;
+                if (g->lower[level] <= m && m <= g->upper[level]) {
+                    launch_compact_live(g->keys.as<gkey>(), g->values.as<gval>(), hw[a], W,
+                                        g->chunkoff.as<uint32_t>(), g->tmpk.as<gkey>(), g->tmpv.as<gval>(), st);
+                    launch_merge_redistribute(g->tmpk.as<gkey>(), g->tmpv.as<gval>(), c, pk + a, pv + a, r,
+                                              g->keys.as<gkey>(), g->values.as<gval>(), g->seps.as<gkey>(),
+                                              g->row_offset.as<uint32_t>(), hw[a], W, g->S, st);
+                    launch_mark_done(pd, a, b, st);
+                }
+                a = b;
+            }
+        }
+        // drop placed inserts, move the rest up one level
+        const uint32_t mask = ~(uint32_t)(((uint64_t)g->S << (level + 1)) - 1);
+        const uint64_t nbp = (np + CMP_BLOCK - 1) / CMP_BLOCK;
+        launch_count_pending(pd, np, d_np, g->blkcnt.as<uint32_t>(), st);
+        launch_scan_blocks(g->blkcnt.as<uint32_t>(), nbp, g->blkoff.as<uint32_t>(), ctr + npslot[pc ^ 1], st);
+        launch_compact(pk, pv, pw, pd, np, d_np, g->blkoff.as<uint32_t>(), PK[pc ^ 1]->as<gkey>(),
+                       PV[pc ^ 1]->as<gval>(), PW[pc ^ 1]->as<uint32_t>(), mask, st);
+        TRY(sync_read_ctr(g));
+        pc ^= 1;
+        np = g->hctr[npslot[pc]];
+        level++;
+    }
+    if (np > 0) {
+        // gpma.cuh:1034-1044: grow (or shrink) and rebalance the root
+        TRY(rebuild(g, PK[pc]->as<gkey>(), PV[pc]->as<gval>(), np, true));
+    }
+    HIP_TRY(hipStreamSynchronize(g->st));
+    HIP_TRY(hipGetLastError());
+    if (inval) return fail(GPMA_ERR_KEY_RANGE, std::to_string(inval) + " keys with src/dst >= row_num skipped");
+    return GPMA_OK;
+}
+
+int ensure_bfs(gpma_handle *g) {
+    uint64_t n = (uint64_t)g->N + 1;
+    TRY(g->lev.ensure(n * 4, &g->dev_bytes));
+    TRY(g->fr0.ensure(n * 4, &g->dev_bytes));
+    TRY(g->fr1.ensure(n * 4, &g->dev_bytes));
+    return GPMA_OK;
+}
+
+// Level loop of gpma_bfs_multidev_single (gpma_bfs.cuh:446-509) on one device.
+int run_bfs(const gkey *keys, const gval *values, const uint32_t *row_offset, uint32_t N, uint32_t start,
+            uint32_t *lev, uint32_t *fa, uint32_t *fb, uint32_t *d_cnt, uint32_t *h_cnt, hipStream_t st,
+            uint32_t *reach_out) {
+    launch_bfs_init(lev, N, start, fa, st);
+    uint32_t nf = 1, reach = 1, level = 1;
+    while (nf) {
+        fill_u32(d_cnt, 0, 1, st);
+        launch_bfs_expand(keys, values, row_offset, fa, nf, fb, d_cnt, lev, level, st);
+        HIP_TRY(hipMemcpyAsync(h_cnt, d_cnt, 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        nf = *h_cnt;
+        reach += nf;
+        std::swap(fa, fb);
+        level++;
+    }
+    HIP_TRY(hipGetLastError());
+    if (reach_out) *reach_out = reach;
+    return GPMA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *gpma_last_error(void) { return g_last_error.c_str(); }
+
+int gpma_create(uint32_t row_num, const gpma_config *cfg, gpma_t *out) {
+    if (!out) return fail(GPMA_ERR_ARG, "gpma_create: null out");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(GPMA_ERR_NO_DEVICE, "no HIP device");
+    if (row_num >= 0xFFFFFFFFu) return fail(GPMA_ERR_ARG, "gpma_create: row_num must be < 2^32-1");
+    gpma_handle *g = new gpma_handle();
+    g->N = row_num;
+    if (cfg && cfg->struct_size >= sizeof(gpma_config)) {
+        if (cfg->device >= 0) g->device = cfg->device;
+        else hipGetDevice(&g->device);
+        if (cfg->bulk_threshold) g->bulk_threshold = cfg->bulk_threshold;
+    } else {
+        hipGetDevice(&g->device);
+    }
+    int rc = GPMA_OK;
+    do {
+        if (hipSetDevice(g->device) != hipSuccess) {
+            rc = fail(GPMA_ERR_ARG, "gpma_create: bad device");
+            break;
+        }
+        if (hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking) != hipSuccess) {
+            rc = fail(GPMA_ERR_HIP, "hipStreamCreate failed");
+            break;
+        }
+        if ((rc = g->ctr.ensure(C_COUNT * 4, &g->dev_bytes)) != GPMA_OK) break;
+        if (hipHostMalloc((void **)&g->hctr, C_COUNT * 4, hipHostMallocDefault) != hipSuccess) {
+            rc = fail(GPMA_ERR_OOM, "hipHostMalloc failed");
+            break;
+        }
+        if ((rc = init_structure(g)) != GPMA_OK) break;
+        // gpma.cuh:70-78: insert every row guard (v << 32 | 0xFFFFFFFF) with value 1
+        if (row_num) {
+            std::vector<uint64_t> guards(row_num);
+            std::vector<uint8_t> ones(row_num, 1);
+            for (uint32_t v = 0; v < row_num; v++) guards[v] = ((uint64_t)v << 32) | 0xFFFFFFFFull;
+            rc = do_update(g, guards.data(), ones.data(), row_num, 0, /*guards_ok=*/true);
+            g->invalid = 0;
+            g->applied = 0;
+        }
+    } while (0);
+    if (rc != GPMA_OK) {
+        gpma_destroy(g);
+        return rc;
+    }
+    *out = g;
+    return GPMA_OK;
+}
+
+int gpma_destroy(gpma_t g) {
+    if (!g) return GPMA_OK;
+    hipSetDevice(g->device);
+    if (g->st) hipStreamSynchronize(g->st);
+    Buf *all[] = {&g->keys, &g->values, &g->seps, &g->row_offset, &g->in_k, &g->in_v, &g->sk0, &g->sv0,
+                  &g->sk1, &g->sv1, &g->win, &g->blkcnt, &g->blkoff, &g->sortwork, &g->pk0, &g->pv0,
+                  &g->pw0, &g->pd0, &g->pk1, &g->pv1, &g->pw1, &g->pd1, &g->tmpk, &g->tmpv, &g->chunkcnt,
+                  &g->chunkoff, &g->lev, &g->fr0, &g->fr1, &g->ctr};
+    for (Buf *b : all) b->release(&g->dev_bytes);
+    if (g->hctr) hipHostFree(g->hctr);
+    if (g->st) hipStreamDestroy(g->st);
+    delete g;
+    return GPMA_OK;
+}
+
+int gpma_update(gpma_t g, const uint64_t *keys, const uint8_t *values, uint64_t n, int on_device) {
+    if (!g) return fail(GPMA_ERR_ARG, "gpma_update: null handle");
+    return do_update(g, keys, values, n, on_device);
+}
+
+int gpma_bfs(gpma_t g, uint32_t start, uint32_t *levels, int on_device, uint32_t *reachable) {
+    if (!g) return fail(GPMA_ERR_ARG, "gpma_bfs: null handle");
+    if (start >= g->N) return fail(GPMA_ERR_ARG, "gpma_bfs: start node >= row_num");
+    HIP_TRY(hipSetDevice(g->device));
+    TRY(ensure_bfs(g));
+    uint32_t reach = 0;
+    TRY(run_bfs(g->keys.as<gkey>(), g->values.as<gval>(), g->row_offset.as<uint32_t>(), g->N, start,
+                g->lev.as<uint32_t>(), g->fr0.as<uint32_t>(), g->fr1.as<uint32_t>(), g->ctr.as<uint32_t>() + C_NEXT,
+                g->hctr + C_NEXT, g->st, &reach));
+    if (levels) {
+        HIP_TRY(hipMemcpyAsync(levels, g->lev.p, (uint64_t)g->N * 4,
+                               on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, g->st));
+        HIP_TRY(hipStreamSynchronize(g->st));
+    }
+    if (reachable) *reachable = reach;
+    return GPMA_OK;
+}
+
+int gpma_csr_bfs(const uint64_t *keys, const uint8_t *values, const uint32_t *row_offsets, uint32_t node_size,
+                 uint32_t edge_size, uint32_t start_node, uint32_t *results) {
+    (void)edge_size;
+    if (!keys || !values || !row_offsets || !results) return fail(GPMA_ERR_ARG, "gpma_csr_bfs: null pointer");
+    if (start_node >= node_size) return fail(GPMA_ERR_ARG, "gpma_csr_bfs: start_node >= node_size");
+    uint32_t *buf = nullptr, *hc = nullptr;
+    hipStream_t st = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    uint64_t n = (uint64_t)node_size + 1;
+    hipError_t e = hipMalloc((void **)&buf, (3 * n + 1) * 4);
+    if (e != hipSuccess) {
+        hipStreamDestroy(st);
+        return fail(GPMA_ERR_OOM, "gpma_csr_bfs: hipMalloc failed");
+    }
+    e = hipHostMalloc((void **)&hc, 4, hipHostMallocDefault);
+    int rc = GPMA_OK;
+    if (e != hipSuccess) rc = fail(GPMA_ERR_OOM, "gpma_csr_bfs: hipHostMalloc failed");
+    if (rc == GPMA_OK)
+        rc = run_bfs(keys, values, row_offsets, node_size, start_node, buf, buf + n, buf + 2 * n, buf + 3 * n, hc,
+                     st, nullptr);
+    if (rc == GPMA_OK) {
+        if (hipMemcpyAsync(results, buf, (uint64_t)node_size * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            rc = fail(GPMA_ERR_HIP, "gpma_csr_bfs: copy-out failed");
+    }
+    hipFree(buf);
+    if (hc) hipHostFree(hc);
+    hipStreamDestroy(st);
+    return rc;
+}
+
+int gpma_get_csr(gpma_t g, gpma_csr *out) {
+    if (!g || !out) return fail(GPMA_ERR_ARG, "gpma_get_csr: null argument");
+    out->keys = g->keys.as<uint64_t>();
+    out->values = g->values.as<uint8_t>();
+    out->row_offset = g->row_offset.as<uint32_t>();
+    out->capacity = g->C;
+    out->segment_length = g->S;
+    out->tree_height = g->h;
+    out->row_num = g->N;
+    out->reserved = 0;
+    return GPMA_OK;
+}
+
+int gpma_copy_layout(gpma_t g, uint64_t *keys, uint8_t *values, uint32_t *row_offset) {
+    if (!g) return fail(GPMA_ERR_ARG, "gpma_copy_layout: null handle");
+    HIP_TRY(hipSetDevice(g->device));
+    if (keys) HIP_TRY(hipMemcpyAsync(keys, g->keys.p, g->C * 8, hipMemcpyDeviceToHost, g->st));
+    if (values) HIP_TRY(hipMemcpyAsync(values, g->values.p, g->C, hipMemcpyDeviceToHost, g->st));
+    if (row_offset)
+        HIP_TRY(hipMemcpyAsync(row_offset, g->row_offset.p, ((uint64_t)g->N + 1) * 4, hipMemcpyDeviceToHost, g->st));
+    HIP_TRY(hipStreamSynchronize(g->st));
+    return GPMA_OK;
+}
+
+int gpma_export_live(gpma_t g, uint64_t *out, uint64_t cap, uint64_t *n) {
+    if (!g || !n) return fail(GPMA_ERR_ARG, "gpma_export_live: null argument");
+    HIP_TRY(hipSetDevice(g->device));
+    uint64_t c = 0;
+    TRY(count_live(g, 0, g->C, &c));
+    *n = c;
+    if (!out) return GPMA_OK;
+    if (cap < c) return fail(GPMA_ERR_CAPACITY, "gpma_export_live: buffer holds " + std::to_string(cap) +
+                                                     " keys, " + std::to_string(c) + " live");
+    launch_compact_live(g->keys.as<gkey>(), g->values.as<gval>(), 0, g->C, g->chunkoff.as<uint32_t>(),
+                        g->tmpk.as<gkey>(), g->tmpv.as<gval>(), g->st);
+    HIP_TRY(hipMemcpyAsync(out, g->tmpk.p, c * 8, hipMemcpyDeviceToHost, g->st));
+    HIP_TRY(hipStreamSynchronize(g->st));
+    HIP_TRY(hipGetLastError());
+    return GPMA_OK;
+}
+
+int gpma_get_stats(gpma_t g, gpma_stats *out) {
+    if (!g || !out) return fail(GPMA_ERR_ARG, "gpma_get_stats: null argument");
+    out->capacity = g->C;
+    out->segment_length = g->S;
+    out->tree_height = g->h;
+    out->row_num = g->N;
+    out->last_levels = g->last_levels;
+    out->updates_applied = g->applied;
+    out->invalid_skipped = g->invalid;
+    out->dropped = g->dropped;
+    out->resizes = g->resizes;
+    out->bulk_loads = g->bulk;
+    out->device_bytes = g->dev_bytes;
+    return GPMA_OK;
+}
+
+int gpma_sync(gpma_t g) {
+    if (!g) return fail(GPMA_ERR_ARG, "gpma_sync: null handle");
+    HIP_TRY(hipStreamSynchronize(g->st));
+    return GPMA_OK;
+}
+
+void *gpma_stream(gpma_t g) { return g ? (void *)g->st : nullptr; }
+
+}  // extern "C"

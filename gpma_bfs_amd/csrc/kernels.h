@@ -1,0 +1,85 @@
+// Host-side launchers for the gfx950 kernels (all asynchronous on `st`).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+#include "common.h"
+
+namespace gpma {
+
+// Digit shifts (bit positions of 8-bit digits, LSD first) for one sort.
+struct SortShifts {
+    uint32_t s[8];
+};
+
+void fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st);
+void fill_u8(uint8_t *p, uint8_t v, uint64_t n, hipStream_t st);
+void fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t st);
+
+// ---- sort.hip
+uint64_t sort_status_words(uint64_t n, uint32_t npass);
+int radix_sort_pairs(gkey *k0, gval *v0, gkey *k1, gval *v1, uint64_t n, const SortShifts &shifts,
+                     uint32_t npass, uint32_t *work, uint32_t *d_err, hipStream_t st);
+
+// ---- update.hip
+constexpr uint32_t CMP_BLOCK = 256;  // items per compaction block
+constexpr uint32_t WIN_MAX = 4096;   // widest window rebalanced by one workgroup in LDS
+constexpr uint32_t CHUNK = 4096;     // slots per workgroup in device-wide window passes
+
+// Copy a batch into sort buffers; keys with src/dst >= N become KEY_NONE
+// (sorted last, skipped by locate) and are counted in *invalid. With
+// guards_ok, dst == 0xFFFFFFFF (a row guard) is accepted.
+void launch_stage(const gkey *ik, const gval *iv, gkey *ok, gval *ov, uint64_t n, uint32_t N, bool guards_ok,
+                  uint32_t *invalid, hipStream_t st);
+// Locate + in-place delete/modify. win[i] = leaf base or SIZE_NONE; per-block
+// pending counts into blkcnt[ceil(n/CMP_BLOCK)].
+void launch_locate(const gkey *ks, const gval *vs, uint64_t n, gkey *keys, gval *values, const gkey *seps,
+                   uint64_t nseg, uint32_t S, uint32_t *win, uint32_t *blkcnt, hipStream_t st);
+// Exclusive scan of nb block counts (one workgroup); *total gets the sum.
+void launch_scan_blocks(const uint32_t *cnt, uint64_t nb, uint32_t *off, uint32_t *total, hipStream_t st);
+// Stable compaction of entries whose win != SIZE_NONE (done == nullptr) or
+// whose done flag is 0, writing win & mask. Entries at or beyond *d_n (or
+// n_ub when d_n is null) are ignored. Uses off[] from the block scan.
+void launch_compact(const gkey *ik, const gval *iv, const uint32_t *iw, const uint8_t *idone, uint64_t n_ub,
+                    const uint32_t *d_n, const uint32_t *off, gkey *ok, gval *ov, uint32_t *ow, uint32_t mask,
+                    hipStream_t st);
+// Per-block counts of entries with done == 0 among the first *d_n.
+void launch_count_pending(const uint8_t *done, uint64_t n_ub, const uint32_t *d_n, uint32_t *blkcnt,
+                          hipStream_t st);
+// Level-0 rebalance: one thread per leaf run.
+void launch_leaf(uint32_t S, const gkey *pk, const gval *pv, const uint32_t *pw, uint8_t *pd, uint64_t n_ub,
+                 const uint32_t *d_np, gkey *keys, gval *values, gkey *seps, uint32_t *row_offset, uint32_t lower,
+                 uint32_t upper, hipStream_t st);
+// Levels whose window fits in LDS: one workgroup per window run.
+void launch_window(uint32_t S, uint32_t W, const gkey *pk, const gval *pv, const uint32_t *pw, uint8_t *pd,
+                   uint64_t n_ub, const uint32_t *d_np, gkey *keys, gval *values, gkey *seps,
+                   uint32_t *row_offset, uint32_t lower, uint32_t upper, hipStream_t st);
+// Run heads of the pending list: heads[j] = index of the j-th run start.
+void launch_run_heads(const uint32_t *pw, uint64_t n_ub, const uint32_t *d_np, uint32_t *flags_cnt,
+                      uint32_t *off, uint32_t *heads, uint32_t *d_nruns, hipStream_t st);
+// Device-wide window passes (window [w, w+W) of the array).
+void launch_count_live(const gkey *keys, const gval *values, uint64_t w, uint64_t W, uint32_t *chunkcnt,
+                       hipStream_t st);
+void launch_compact_live(const gkey *keys, const gval *values, uint64_t w, uint64_t W, const uint32_t *chunkoff,
+                         gkey *tk, gval *tv, hipStream_t st);
+// Merge c compacted items with r sorted inserts and redistribute them into
+// window [w, w+W) of (keys, values), writing gaps, row_offset and seps.
+void launch_merge_redistribute(const gkey *tk, const gval *tv, uint64_t c, const gkey *ik, const gval *iv,
+                               uint64_t r, gkey *keys, gval *values, gkey *seps, uint32_t *row_offset,
+                               uint64_t w, uint64_t W, uint32_t S, hipStream_t st);
+void launch_mark_done(uint8_t *pd, uint64_t a, uint64_t b, hipStream_t st);
+// Write c compacted items to keys[0..c), KEY_NONE after; rebuild seps and
+// point row_offset at the compacted guards.
+void launch_write_compacted(const gkey *tk, const gval *tv, uint64_t c, gkey *keys, gval *values, uint64_t C,
+                            gkey *seps, uint32_t *row_offset, uint32_t S, hipStream_t st);
+
+// ---- bfs.hip
+void launch_bfs_init(uint32_t *levels, uint32_t N, uint32_t start, uint32_t *frontier, hipStream_t st);
+void launch_bfs_expand(const gkey *keys, const gval *values, const uint32_t *row_offset, const uint32_t *front,
+                       uint32_t nf, uint32_t *next, uint32_t *next_cnt, uint32_t *levels, uint32_t level,
+                       hipStream_t st);
+
+}  // namespace gpma

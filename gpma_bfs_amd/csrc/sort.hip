@@ -1,0 +1,196 @@
+// Stable LSD radix sort of (u64 key, u8 value) pairs for update batches.
+//
+// Reference counterpart: thrust::sort_by_key(update_keys, update_values)
+// (gpma.cuh:977), which sorts the batch by the full 64-bit key. Here only
+// the 8-bit digits that can differ between valid keys are sorted: a valid
+// key has src, dst < row_num, so bits above the row-number width are zero in
+// both 32-bit halves; the caller passes that digit list.
+//
+// One launch per digit pass ("onesweep", Merrill & Garland / Adinets &
+// Merrill 2022): each 2048-item tile ranks its items per digit with wave64
+// ballots, publishes its per-digit counts, looks back over earlier tiles'
+// published counts to find its global offsets, and scatters. Tiles take
+// tickets in launch order, and a tile publishes its own counts before it
+// waits on anyone, so every wait is on a tile that is already running and
+// will publish without waiting; the wait is also bounded by a spin limit
+// that raises an error flag instead of hanging.
+#include "common.h"
+#include "kernels.h"
+
+namespace gpma {
+
+namespace {
+
+constexpr uint32_t SORT_THREADS = 256;
+constexpr uint32_t SORT_WAVES = SORT_THREADS / WAVE;
+constexpr uint32_t SORT_IPT = 8;                       // items per lane
+constexpr uint32_t WAVE_ITEMS = WAVE * SORT_IPT;       // 512
+constexpr uint32_t TILE = SORT_THREADS * SORT_IPT;     // 2048
+constexpr uint32_t FLAG_AGG = 1u << 30;
+constexpr uint32_t FLAG_INC = 2u << 30;
+constexpr uint32_t VAL_MASK = (1u << 30) - 1;
+constexpr uint32_t SPIN_LIMIT = 1u << 22;
+
+// Global digit histograms for every pass at once; also clears the
+// look-back status words and tile tickets for the passes that follow.
+__global__ __launch_bounds__(SORT_THREADS) void k_sort_hist(const gkey *__restrict__ keys, uint64_t n,
+                                                            SortShifts shifts, uint32_t npass,
+                                                            uint32_t *__restrict__ hist, uint32_t *__restrict__ status,
+                                                            uint64_t nstatus) {
+    __shared__ uint32_t h[8][256];
+    for (uint32_t i = threadIdx.x; i < 8 * 256; i += SORT_THREADS) (&h[0][0])[i] = 0;
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * SORT_THREADS;
+    for (uint64_t i = (uint64_t)blockIdx.x * SORT_THREADS + threadIdx.x; i < n; i += stride) {
+        gkey k = keys[i];
+        for (uint32_t p = 0; p < npass; p++) atomicAdd(&h[p][(uint32_t)(k >> shifts.s[p]) & 0xFF], 1u);
+    }
+    for (uint64_t i = (uint64_t)blockIdx.x * SORT_THREADS + threadIdx.x; i < nstatus; i += stride) status[i] = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < npass * 256; i += SORT_THREADS) {
+        uint32_t c = (&h[0][0])[i];
+        if (c) atomicAdd(&hist[i], c);
+    }
+}
+
+__global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(const gkey *__restrict__ kin, const gval *__restrict__ vin,
+                                                            gkey *__restrict__ kout, gval *__restrict__ vout,
+                                                            uint64_t n, uint32_t shift,
+                                                            const uint32_t *__restrict__ hist,  // [256] of this pass
+                                                            uint32_t *__restrict__ status,      // [ntiles][256]
+                                                            uint32_t *__restrict__ ticket, uint32_t *__restrict__ err) {
+    __shared__ uint32_t cnt[SORT_WAVES][256];
+    __shared__ uint32_t gofs[256];
+    __shared__ uint32_t scan_tmp[SORT_WAVES + 1];
+    __shared__ uint32_t s_tile;
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t wid = tid / WAVE;
+    const uint32_t lane = lane_id();
+    for (uint32_t i = tid; i < SORT_WAVES * 256; i += SORT_THREADS) (&cnt[0][0])[i] = 0;
+    if (tid == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint64_t wbase = (uint64_t)tile * TILE + (uint64_t)wid * WAVE_ITEMS;
+
+    gkey k[SORT_IPT];
+    gval v[SORT_IPT];
+    uint32_t rank[SORT_IPT];
+    uint32_t dig[SORT_IPT];
+#pragma unroll
+    for (uint32_t j = 0; j < SORT_IPT; j++) {
+        uint64_t idx = wbase + (uint64_t)j * WAVE + lane;
+        bool ok = idx < n;
+        k[j] = ok ? kin[idx] : 0;
+        v[j] = ok ? vin[idx] : 0;
+        dig[j] = ok ? ((uint32_t)(k[j] >> shift) & 0xFF) : 256u;
+    }
+    // Stable per-wave ranking in striped order (item j of all lanes, then j+1).
+#pragma unroll
+    for (uint32_t j = 0; j < SORT_IPT; j++) {
+        const uint32_t d = dig[j];
+        const bool ok = d < 256;
+        uint64_t match = ballot(ok);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            uint64_t bb = ballot(ok && ((d >> b) & 1));
+            match &= ((d >> b) & 1) ? bb : ~bb;
+        }
+        uint32_t leader = ok ? (uint32_t)__ffsll((unsigned long long)match) - 1 : 0;
+        uint32_t base = 0;
+        if (ok && lane == leader) {
+            base = cnt[wid][d];
+            cnt[wid][d] = base + (uint32_t)__popcll(match);
+        }
+        base = __shfl(base, leader, WAVE);
+        rank[j] = base + (uint32_t)__popcll(match & lanemask_lt());
+    }
+    __syncthreads();
+    // Thread d: tile count for digit d, and per-wave prefixes.
+    const uint32_t d = tid;  // SORT_THREADS == 256
+    uint32_t tot = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < SORT_WAVES; w++) {
+        uint32_t c = cnt[w][d];
+        cnt[w][d] = tot;
+        tot += c;
+    }
+    // Publish this tile's aggregate, then look back.
+    uint32_t *my = status + (uint64_t)tile * 256 + d;
+    if (tile == 0) {
+        __hip_atomic_store(my, FLAG_INC | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        __hip_atomic_store(my, FLAG_AGG | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    uint32_t excl = 0;
+    if (tile > 0) {
+        int64_t t = (int64_t)tile - 1;
+        uint32_t spins = 0;
+        while (t >= 0) {
+            uint32_t s = __hip_atomic_load(status + (uint64_t)t * 256 + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t flag = s & ~VAL_MASK;
+            if (flag == 0) {
+                if (++spins > SPIN_LIMIT) {
+                    atomicOr(err, 1u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            excl += s & VAL_MASK;
+            if (flag == FLAG_INC) break;
+            t--;
+        }
+        __hip_atomic_store(my, FLAG_INC | (excl + tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // Global digit base: exclusive scan of the pass histogram.
+    uint32_t htot;
+    uint32_t hb = block_exclusive_scan(hist[d], scan_tmp, &htot);
+    gofs[d] = hb + excl;
+    __syncthreads();
+#pragma unroll
+    for (uint32_t j = 0; j < SORT_IPT; j++) {
+        if (dig[j] < 256) {
+            uint64_t pos = (uint64_t)gofs[dig[j]] + cnt[wid][dig[j]] + rank[j];
+            kout[pos] = k[j];
+            vout[pos] = v[j];
+        }
+    }
+}
+
+}  // namespace
+
+uint64_t sort_status_words(uint64_t n, uint32_t npass) {
+    uint64_t tiles = (n + TILE - 1) / TILE;
+    return tiles * 256 * npass;
+}
+
+// Sorts (k0, v0) by the byte digits at `shifts` (LSD first), ping-ponging
+// with (k1, v1). Returns 0 if the result is in (k0, v0), 1 if in (k1, v1).
+// work: hist [npass*256] + tickets [npass] + err [1] + status words.
+int radix_sort_pairs(gkey *k0, gval *v0, gkey *k1, gval *v1, uint64_t n, const SortShifts &shifts,
+                     uint32_t npass, uint32_t *work, uint32_t *d_err, hipStream_t st) {
+    if (n == 0 || npass == 0) return 0;
+    uint32_t *hist = work;
+    uint32_t *tickets = hist + npass * 256;
+    uint32_t *status = tickets + npass;
+    uint64_t tiles = (n + TILE - 1) / TILE;
+    uint64_t nstatus = tiles * 256 * npass;
+    fill_u32(hist, 0, npass * 256 + npass, st);
+    uint32_t hb = (uint32_t)std::min<uint64_t>(1024, (std::max<uint64_t>(n, nstatus) + SORT_THREADS - 1) / SORT_THREADS);
+    if (hb == 0) hb = 1;
+    hipLaunchKernelGGL(k_sort_hist, dim3(hb), dim3(SORT_THREADS), 0, st, k0, n, shifts, npass, hist, status, nstatus);
+    int cur = 0;
+    for (uint32_t p = 0; p < npass; p++) {
+        gkey *ki = cur ? k1 : k0;
+        gval *vi = cur ? v1 : v0;
+        gkey *ko = cur ? k0 : k1;
+        gval *vo = cur ? v0 : v1;
+        hipLaunchKernelGGL(k_sort_pass, dim3((uint32_t)tiles), dim3(SORT_THREADS), 0, st, ki, vi, ko, vo, n,
+                           shifts.s[p], hist + p * 256, status + (uint64_t)p * tiles * 256, tickets + p, d_err);
+        cur ^= 1;
+    }
+    return cur;
+}
+
+}  // namespace gpma

@@ -1,0 +1,583 @@
+// Update-path kernels: stage, locate, pending compaction, level-0 leaf
+// rebalance, LDS window rebalance, and the device-wide window passes used
+// for wide windows, resize and bulk load.
+//
+// Semantics follow the reference's update_gpma (gpma.cuh:968-1050):
+//   locate: binary search over segment first keys, then a scan of the leaf;
+//           an existing key (live or tombstoned) takes the update's value in
+//           place, a delete never inserts (gpma.cuh:188-240);
+//   rebalance at level L (window W = S << L): drop gaps and tombstones, and
+//           if lower[L] <= live + pending <= upper[L] merge the pending
+//           inserts in and redistribute evenly (gpma.cuh:294-540,636-663);
+//           a failing window is left as it was and its inserts move up.
+// The per-window merge replaces the reference's per-window radix sort: the
+// live items are already sorted and so are the pending inserts.
+#include "common.h"
+#include "kernels.h"
+
+namespace gpma {
+
+namespace {
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_fill_u8(uint8_t *p, uint8_t v, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_fill_u64(uint64_t *p, uint64_t v, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+uint32_t grid_for(uint64_t n, uint32_t threads, uint32_t cap = 4096) {
+    uint64_t b = (n + threads - 1) / threads;
+    if (b == 0) b = 1;
+    return (uint32_t)std::min<uint64_t>(b, cap);
+}
+
+// Wave-aggregated increment of a global counter by the number of lanes with p.
+__device__ __forceinline__ void wave_count(uint32_t *ctr, bool p) {
+    uint64_t m = ballot(p);
+    if (m && lane_id() == (uint32_t)__ffsll((unsigned long long)m) - 1) atomicAdd(ctr, (uint32_t)__popcll(m));
+}
+
+__global__ __launch_bounds__(256) void k_stage(const gkey *__restrict__ ik, const gval *__restrict__ iv,
+                                               gkey *__restrict__ ok, gval *__restrict__ ov, uint64_t n,
+                                               uint32_t N, uint32_t guards_ok, uint32_t *invalid) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i - threadIdx.x < n; i += (uint64_t)gridDim.x * 256) {
+        bool bad = false;
+        if (i < n) {
+            gkey k = ik[i];
+            const uint32_t dst = (uint32_t)k;
+            bad = (uint32_t)(k >> 32) >= N || (dst >= N && !(guards_ok && dst == COL_NONE));
+            ok[i] = bad ? KEY_NONE : k;
+            ov[i] = iv[i];
+        }
+        wave_count(invalid, bad);
+    }
+}
+
+// One update per thread (gpma.cuh:204-226). Sorted input, so the last of
+// equal keys is the one applied (earlier duplicates are skipped).
+__global__ __launch_bounds__(CMP_BLOCK) void k_locate(const gkey *__restrict__ ks, const gval *__restrict__ vs,
+                                                      uint64_t n, gkey *__restrict__ keys, gval *__restrict__ values,
+                                                      const gkey *__restrict__ seps, uint64_t nseg, uint32_t S,
+                                                      uint32_t *__restrict__ win, uint32_t *__restrict__ blkcnt) {
+    __shared__ uint32_t wcnt[CMP_BLOCK / WAVE];
+    const uint64_t i = (uint64_t)blockIdx.x * CMP_BLOCK + threadIdx.x;
+    bool pend = false;
+    if (i < n) {
+        const gkey k = ks[i];
+        const gval v = vs[i];
+        uint32_t res = SIZE_NONE;
+        const bool skip = (k == KEY_NONE) || (i + 1 < n && ks[i + 1] == k);
+        if (!skip) {
+            uint64_t p = 0;
+            for (uint64_t b = nseg >> 1; b > 0; b >>= 1)
+                if (seps[p | b] <= k) p |= b;
+            const uint64_t base = p * S;
+            bool found = false;
+            for (uint32_t j = 0; j < S; j++) {
+                if (keys[base + j] == k) {
+                    values[base + j] = v;
+                    found = true;
+                    break;
+                }
+            }
+            if (!found && v != 0) {
+                res = (uint32_t)base;
+                pend = true;
+            }
+        }
+        win[i] = res;
+    }
+    uint64_t m = ballot(pend);
+    if (lane_id() == 0) wcnt[threadIdx.x / WAVE] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (uint32_t w = 0; w < CMP_BLOCK / WAVE; w++) s += wcnt[w];
+        blkcnt[blockIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_scan_blocks(const uint32_t *__restrict__ cnt, uint64_t nb,
+                                                      uint32_t *__restrict__ off, uint32_t *__restrict__ total) {
+    __shared__ uint32_t scratch[1024 / WAVE + 1];
+    const uint64_t per = (nb + 1023) / 1024;
+    const uint64_t a = std::min<uint64_t>(nb, threadIdx.x * per), b = std::min<uint64_t>(nb, a + per);
+    uint32_t s = 0;
+    for (uint64_t j = a; j < b; j++) s += cnt[j];
+    uint32_t tot;
+    uint32_t pre = block_exclusive_scan(s, scratch, &tot);
+    for (uint64_t j = a; j < b; j++) {
+        uint32_t c = cnt[j];
+        off[j] = pre;
+        pre += c;
+    }
+    if (threadIdx.x == 0) *total = tot;
+}
+
+__global__ __launch_bounds__(CMP_BLOCK) void k_count_pending(const uint8_t *__restrict__ done, uint64_t n_ub,
+                                                             const uint32_t *__restrict__ d_n,
+                                                             uint32_t *__restrict__ blkcnt) {
+    __shared__ uint32_t wcnt[CMP_BLOCK / WAVE];
+    const uint64_t n = d_n ? *d_n : n_ub;
+    const uint64_t i = (uint64_t)blockIdx.x * CMP_BLOCK + threadIdx.x;
+    bool keep = i < n && done[i] == 0;
+    uint64_t m = ballot(keep);
+    if (lane_id() == 0) wcnt[threadIdx.x / WAVE] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (uint32_t w = 0; w < CMP_BLOCK / WAVE; w++) s += wcnt[w];
+        blkcnt[blockIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(CMP_BLOCK) void k_compact(const gkey *__restrict__ ik, const gval *__restrict__ iv,
+                                                       const uint32_t *__restrict__ iw,
+                                                       const uint8_t *__restrict__ idone, uint64_t n_ub,
+                                                       const uint32_t *__restrict__ d_n,
+                                                       const uint32_t *__restrict__ off, gkey *__restrict__ ok,
+                                                       gval *__restrict__ ov, uint32_t *__restrict__ ow,
+                                                       uint32_t mask) {
+    __shared__ uint32_t scratch[CMP_BLOCK / WAVE + 1];
+    const uint64_t n = d_n ? *d_n : n_ub;
+    const uint64_t i = (uint64_t)blockIdx.x * CMP_BLOCK + threadIdx.x;
+    bool keep = false;
+    if (i < n) keep = idone ? (idone[i] == 0) : (iw[i] != SIZE_NONE);
+    uint32_t tot;
+    uint32_t pre = block_exclusive_scan(keep ? 1u : 0u, scratch, &tot);
+    if (keep) {
+        uint64_t o = (uint64_t)off[blockIdx.x] + pre;
+        ok[o] = ik[i];
+        ov[o] = iv[i];
+        ow[o] = iw[i] & mask;
+    }
+}
+
+// Level 0 (window = one leaf of S slots), one thread per run of pending
+// inserts that share a leaf. With f = W/S = 1 the redistribution packs the
+// merged items at the front of the leaf (gpma.cuh:636-654 with f = 1).
+template <uint32_t S>
+__global__ __launch_bounds__(256) void k_leaf(const gkey *__restrict__ pk, const gval *__restrict__ pv,
+                                              const uint32_t *__restrict__ pw, uint8_t *__restrict__ pd,
+                                              const uint32_t *__restrict__ d_np, gkey *__restrict__ keys,
+                                              gval *__restrict__ values, gkey *__restrict__ seps,
+                                              uint32_t *__restrict__ row_offset, uint32_t lower, uint32_t upper) {
+    const uint64_t np = *d_np;
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= np) return;
+    const uint32_t w = pw[i];
+    if (i > 0 && pw[i - 1] == w) return;
+    uint32_t r = 1;
+    while (r <= upper && i + r < np && pw[i + r] == w) r++;
+    if (r > upper) return;
+    gkey k[S];
+    gval v[S];
+    bool live[S];
+    uint32_t shift[S];
+    uint32_t c = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < S; j++) {
+        k[j] = keys[(uint64_t)w + j];
+        v[j] = values[(uint64_t)w + j];
+        live[j] = is_live(k[j], v[j]);
+        c += live[j] ? 1u : 0u;
+        shift[j] = 0;
+    }
+    const uint32_t m = c + r;
+    if (m < lower || m > upper) return;
+    gkey first = KEY_NONE;
+    for (uint32_t u = 0; u < r; u++) {
+        const gkey ik = pk[i + u];
+        const gval iv = pv[i + u];
+        uint32_t lb = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < S; j++) {
+            if (live[j]) {
+                if (k[j] < ik) lb++;
+                else shift[j]++;
+            }
+        }
+        const uint32_t pos = u + lb;
+        keys[(uint64_t)w + pos] = ik;
+        values[(uint64_t)w + pos] = iv;
+        if (is_guard(ik)) row_offset[(ik >> 32) + 1] = w + pos;
+        if (pos == 0) first = ik;
+        pd[i + u] = 1;
+    }
+    uint32_t lr = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < S; j++) {
+        if (live[j]) {
+            const uint32_t pos = lr + shift[j];
+            keys[(uint64_t)w + pos] = k[j];
+            values[(uint64_t)w + pos] = v[j];
+            if (is_guard(k[j])) row_offset[(k[j] >> 32) + 1] = w + pos;
+            if (pos == 0) first = k[j];
+            lr++;
+        }
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < S; j++) {
+        if (j >= m) {
+            keys[(uint64_t)w + j] = KEY_NONE;
+            values[(uint64_t)w + j] = 0;
+        }
+    }
+    seps[w / S] = first;
+}
+
+// number of entries of a sorted array a[0..n) strictly below x
+__device__ __forceinline__ uint64_t lower_bound_g(const gkey *a, uint64_t n, gkey x) {
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (a[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// One workgroup per window run for S < W <= WIN_MAX: compact the window's
+// live items into LDS, check the level's bounds, and write merged items
+// straight to their redistributed slots.
+__global__ __launch_bounds__(256) void k_window(uint32_t S, uint32_t W, const gkey *__restrict__ pk,
+                                                const gval *__restrict__ pv, const uint32_t *__restrict__ pw,
+                                                uint8_t *__restrict__ pd, const uint32_t *__restrict__ d_np,
+                                                gkey *__restrict__ keys, gval *__restrict__ values,
+                                                gkey *__restrict__ seps, uint32_t *__restrict__ row_offset,
+                                                uint32_t lower, uint32_t upper) {
+    __shared__ gkey lk[WIN_MAX];
+    __shared__ gval lv[WIN_MAX];
+    __shared__ uint32_t scratch[256 / WAVE + 1];
+    __shared__ uint32_t s_r;
+    const uint64_t np = *d_np;
+    const uint64_t i = blockIdx.x;
+    if (i >= np) return;
+    const uint32_t w = pw[i];
+    if (i > 0 && pw[i - 1] == w) return;
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) s_r = upper + 1;
+    __syncthreads();
+    for (uint32_t t = tid; t <= upper; t += 256) {
+        bool in = (i + t < np) && pw[i + t] == w;
+        if (!in) atomicMin(&s_r, t);
+    }
+    __syncthreads();
+    const uint32_t r = s_r;
+    if (r > upper) return;
+    uint32_t c = 0;
+    for (uint32_t b = 0; b < W; b += 256) {
+        const uint32_t p = b + tid;
+        bool live = false;
+        gkey k = KEY_NONE;
+        gval v = 0;
+        if (p < W) {
+            k = keys[(uint64_t)w + p];
+            v = values[(uint64_t)w + p];
+            live = is_live(k, v);
+        }
+        uint32_t tot;
+        uint32_t pre = block_exclusive_scan(live ? 1u : 0u, scratch, &tot);
+        if (live) {
+            lk[c + pre] = k;
+            lv[c + pre] = v;
+        }
+        c += tot;
+    }
+    __syncthreads();
+    const uint32_t m = c + r;
+    if (m < lower || m > upper) return;
+    const uint64_t f = W / S;
+    for (uint32_t t = tid; t < c; t += 256) {
+        const gkey k = lk[t];
+        const uint64_t mi = t + lower_bound_g(pk + i, r, k);
+        const uint32_t slot = redistribute_slot(f, S, m, mi);
+        keys[(uint64_t)w + slot] = k;
+        values[(uint64_t)w + slot] = lv[t];
+        if (is_guard(k)) row_offset[(k >> 32) + 1] = w + slot;
+        if (slot % S == 0) seps[(w + slot) / S] = k;
+    }
+    for (uint32_t u = tid; u < r; u += 256) {
+        const gkey k = pk[i + u];
+        uint32_t lo = 0, hi = c;
+        while (lo < hi) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (lk[mid] < k) lo = mid + 1;
+            else hi = mid;
+        }
+        const uint64_t mi = u + lo;
+        const uint32_t slot = redistribute_slot(f, S, m, mi);
+        keys[(uint64_t)w + slot] = k;
+        values[(uint64_t)w + slot] = pv[i + u];
+        if (is_guard(k)) row_offset[(k >> 32) + 1] = w + slot;
+        if (slot % S == 0) seps[(w + slot) / S] = k;
+        pd[i + u] = 1;
+    }
+    for (uint32_t p = tid; p < W; p += 256) {
+        const uint64_t s = p / S, l = p % S;
+        const uint64_t a = seg_first_item(f, m, s);
+        const uint64_t cnt = seg_first_item(f, m, s + 1) - a;
+        if (l >= cnt) {
+            keys[(uint64_t)w + p] = KEY_NONE;
+            values[(uint64_t)w + p] = 0;
+            if (l == 0) seps[(w + p) / S] = KEY_NONE;
+        }
+    }
+}
+
+__global__ __launch_bounds__(CMP_BLOCK) void k_head_count(const uint32_t *__restrict__ pw, uint64_t n_ub,
+                                                          const uint32_t *__restrict__ d_np,
+                                                          uint32_t *__restrict__ blkcnt) {
+    __shared__ uint32_t wcnt[CMP_BLOCK / WAVE];
+    const uint64_t n = *d_np;
+    const uint64_t i = (uint64_t)blockIdx.x * CMP_BLOCK + threadIdx.x;
+    bool head = i < n && (i == 0 || pw[i] != pw[i - 1]);
+    uint64_t m = ballot(head);
+    if (lane_id() == 0) wcnt[threadIdx.x / WAVE] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (uint32_t w = 0; w < CMP_BLOCK / WAVE; w++) s += wcnt[w];
+        blkcnt[blockIdx.x] = s;
+    }
+    (void)n_ub;
+}
+
+__global__ __launch_bounds__(CMP_BLOCK) void k_head_scatter(const uint32_t *__restrict__ pw, uint64_t n_ub,
+                                                            const uint32_t *__restrict__ d_np,
+                                                            const uint32_t *__restrict__ off,
+                                                            uint32_t *__restrict__ heads) {
+    __shared__ uint32_t scratch[CMP_BLOCK / WAVE + 1];
+    const uint64_t n = *d_np;
+    const uint64_t i = (uint64_t)blockIdx.x * CMP_BLOCK + threadIdx.x;
+    bool head = i < n && (i == 0 || pw[i] != pw[i - 1]);
+    uint32_t tot;
+    uint32_t pre = block_exclusive_scan(head ? 1u : 0u, scratch, &tot);
+    if (head) heads[off[blockIdx.x] + pre] = (uint32_t)i;
+    (void)n_ub;
+}
+
+__global__ __launch_bounds__(256) void k_count_live(const gkey *__restrict__ keys, const gval *__restrict__ values,
+                                                    uint64_t w, uint64_t W, uint32_t *__restrict__ chunkcnt) {
+    __shared__ uint32_t wcnt[256 / WAVE];
+    const uint64_t base = (uint64_t)blockIdx.x * CHUNK;
+    uint32_t c = 0;
+    for (uint32_t j = threadIdx.x; j < CHUNK; j += 256) {
+        uint64_t p = base + j;
+        if (p < W) c += is_live(keys[w + p], values[w + p]) ? 1u : 0u;
+    }
+    // wave reduce
+    for (int d = WAVE / 2; d > 0; d >>= 1) c += __shfl_down(c, d, WAVE);
+    if (lane_id() == 0) wcnt[threadIdx.x / WAVE] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) chunkcnt[blockIdx.x] = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+}
+
+__global__ __launch_bounds__(256) void k_compact_live(const gkey *__restrict__ keys, const gval *__restrict__ values,
+                                                      uint64_t w, uint64_t W, const uint32_t *__restrict__ chunkoff,
+                                                      gkey *__restrict__ tk, gval *__restrict__ tv) {
+    __shared__ uint32_t scratch[256 / WAVE + 1];
+    const uint64_t base = (uint64_t)blockIdx.x * CHUNK;
+    uint64_t o = chunkoff[blockIdx.x];
+    for (uint32_t j = 0; j < CHUNK; j += 256) {
+        uint64_t p = base + j + threadIdx.x;
+        bool live = false;
+        gkey k = KEY_NONE;
+        gval v = 0;
+        if (p < W) {
+            k = keys[w + p];
+            v = values[w + p];
+            live = is_live(k, v);
+        }
+        uint32_t tot;
+        uint32_t pre = block_exclusive_scan(live ? 1u : 0u, scratch, &tot);
+        if (live) {
+            tk[o + pre] = k;
+            tv[o + pre] = v;
+        }
+        o += tot;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_merge_scatter(const gkey *__restrict__ tk, const gval *__restrict__ tv,
+                                                       uint64_t c, const gkey *__restrict__ ik,
+                                                       const gval *__restrict__ iv, uint64_t r,
+                                                       gkey *__restrict__ keys, gval *__restrict__ values,
+                                                       gkey *__restrict__ seps, uint32_t *__restrict__ row_offset,
+                                                       uint64_t w, uint64_t W, uint32_t S) {
+    const uint64_t m = c + r;
+    const uint64_t f = W / S;
+    for (uint64_t idx = (uint64_t)blockIdx.x * 256 + threadIdx.x; idx < m; idx += (uint64_t)gridDim.x * 256) {
+        gkey k;
+        gval v;
+        uint64_t mi;
+        if (idx < c) {
+            k = tk[idx];
+            v = tv[idx];
+            mi = idx + lower_bound_g(ik, r, k);
+        } else {
+            const uint64_t u = idx - c;
+            k = ik[u];
+            v = iv[u];
+            mi = u + lower_bound_g(tk, c, k);
+        }
+        const uint32_t slot = redistribute_slot(f, S, m, mi);
+        keys[w + slot] = k;
+        values[w + slot] = v;
+        if (is_guard(k)) row_offset[(k >> 32) + 1] = (uint32_t)(w + slot);
+        if (slot % S == 0) seps[(w + slot) / S] = k;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill_gaps(uint64_t m, gkey *__restrict__ keys, gval *__restrict__ values,
+                                                   gkey *__restrict__ seps, uint64_t w, uint64_t W, uint32_t S) {
+    const uint64_t f = W / S;
+    for (uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x; p < W; p += (uint64_t)gridDim.x * 256) {
+        const uint64_t s = p / S, l = p % S;
+        const uint64_t a = seg_first_item(f, m, s);
+        const uint64_t cnt = seg_first_item(f, m, s + 1) - a;
+        if (l >= cnt) {
+            keys[w + p] = KEY_NONE;
+            values[w + p] = 0;
+            if (l == 0) seps[(w + p) / S] = KEY_NONE;
+        }
+    }
+}
+
+// The reference keeps row_offset stale here (gpma.cuh:906-934 compacts
+// without touching it); pointing row_offset at the compacted guards instead
+// keeps every later BFS range inside the array.
+__global__ __launch_bounds__(256) void k_write_compacted(const gkey *__restrict__ tk, const gval *__restrict__ tv,
+                                                         uint64_t c, gkey *__restrict__ keys,
+                                                         gval *__restrict__ values, uint64_t C,
+                                                         gkey *__restrict__ seps, uint32_t *__restrict__ row_offset,
+                                                         uint32_t S) {
+    for (uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x; p < C; p += (uint64_t)gridDim.x * 256) {
+        gkey k = p < c ? tk[p] : KEY_NONE;
+        keys[p] = k;
+        values[p] = p < c ? tv[p] : 0;
+        if (p % S == 0) seps[p / S] = k;
+        if (p < c && is_guard(k)) row_offset[(k >> 32) + 1] = (uint32_t)p;
+    }
+}
+
+}  // namespace
+
+void fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 2048)), dim3(256), 0, st, p, v, n);
+}
+void fill_u8(uint8_t *p, uint8_t v, uint64_t n, hipStream_t st) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(n, 256, 2048)), dim3(256), 0, st, p, v, n);
+}
+void fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t st) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256, 2048)), dim3(256), 0, st, p, v, n);
+}
+
+void launch_stage(const gkey *ik, const gval *iv, gkey *ok, gval *ov, uint64_t n, uint32_t N, bool guards_ok,
+                  uint32_t *invalid, hipStream_t st) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_stage, dim3(grid_for(n, 256)), dim3(256), 0, st, ik, iv, ok, ov, n, N,
+                       guards_ok ? 1u : 0u, invalid);
+}
+
+void launch_locate(const gkey *ks, const gval *vs, uint64_t n, gkey *keys, gval *values, const gkey *seps,
+                   uint64_t nseg, uint32_t S, uint32_t *win, uint32_t *blkcnt, hipStream_t st) {
+    if (!n) return;
+    uint64_t nb = (n + CMP_BLOCK - 1) / CMP_BLOCK;
+    hipLaunchKernelGGL(k_locate, dim3((uint32_t)nb), dim3(CMP_BLOCK), 0, st, ks, vs, n, keys, values, seps, nseg, S,
+                       win, blkcnt);
+}
+
+void launch_scan_blocks(const uint32_t *cnt, uint64_t nb, uint32_t *off, uint32_t *total, hipStream_t st) {
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, cnt, nb, off, total);
+}
+
+void launch_compact(const gkey *ik, const gval *iv, const uint32_t *iw, const uint8_t *idone, uint64_t n_ub,
+                    const uint32_t *d_n, const uint32_t *off, gkey *ok, gval *ov, uint32_t *ow, uint32_t mask,
+                    hipStream_t st) {
+    if (!n_ub) return;
+    uint64_t nb = (n_ub + CMP_BLOCK - 1) / CMP_BLOCK;
+    hipLaunchKernelGGL(k_compact, dim3((uint32_t)nb), dim3(CMP_BLOCK), 0, st, ik, iv, iw, idone, n_ub, d_n, off, ok,
+                       ov, ow, mask);
+}
+
+void launch_count_pending(const uint8_t *done, uint64_t n_ub, const uint32_t *d_n, uint32_t *blkcnt,
+                          hipStream_t st) {
+    if (!n_ub) return;
+    uint64_t nb = (n_ub + CMP_BLOCK - 1) / CMP_BLOCK;
+    hipLaunchKernelGGL(k_count_pending, dim3((uint32_t)nb), dim3(CMP_BLOCK), 0, st, done, n_ub, d_n, blkcnt);
+}
+
+void launch_leaf(uint32_t S, const gkey *pk, const gval *pv, const uint32_t *pw, uint8_t *pd, uint64_t n_ub,
+                 const uint32_t *d_np, gkey *keys, gval *values, gkey *seps, uint32_t *row_offset, uint32_t lower,
+                 uint32_t upper, hipStream_t st) {
+    if (!n_ub) return;
+    dim3 g((uint32_t)((n_ub + 255) / 256)), b(256);
+    switch (S) {
+    case 2: hipLaunchKernelGGL(k_leaf<2>, g, b, 0, st, pk, pv, pw, pd, d_np, keys, values, seps, row_offset, lower, upper); break;
+    case 4: hipLaunchKernelGGL(k_leaf<4>, g, b, 0, st, pk, pv, pw, pd, d_np, keys, values, seps, row_offset, lower, upper); break;
+    case 8: hipLaunchKernelGGL(k_leaf<8>, g, b, 0, st, pk, pv, pw, pd, d_np, keys, values, seps, row_offset, lower, upper); break;
+    case 16: hipLaunchKernelGGL(k_leaf<16>, g, b, 0, st, pk, pv, pw, pd, d_np, keys, values, seps, row_offset, lower, upper); break;
+    default: hipLaunchKernelGGL(k_leaf<32>, g, b, 0, st, pk, pv, pw, pd, d_np, keys, values, seps, row_offset, lower, upper); break;
+    }
+}
+
+void launch_window(uint32_t S, uint32_t W, const gkey *pk, const gval *pv, const uint32_t *pw, uint8_t *pd,
+                   uint64_t n_ub, const uint32_t *d_np, gkey *keys, gval *values, gkey *seps,
+                   uint32_t *row_offset, uint32_t lower, uint32_t upper, hipStream_t st) {
+    if (!n_ub) return;
+    hipLaunchKernelGGL(k_window, dim3((uint32_t)n_ub), dim3(256), 0, st, S, W, pk, pv, pw, pd, d_np, keys, values,
+                       seps, row_offset, lower, upper);
+}
+
+void launch_run_heads(const uint32_t *pw, uint64_t n_ub, const uint32_t *d_np, uint32_t *blkcnt, uint32_t *off,
+                      uint32_t *heads, uint32_t *d_nruns, hipStream_t st) {
+    if (!n_ub) return;
+    uint64_t nb = (n_ub + CMP_BLOCK - 1) / CMP_BLOCK;
+    hipLaunchKernelGGL(k_head_count, dim3((uint32_t)nb), dim3(CMP_BLOCK), 0, st, pw, n_ub, d_np, blkcnt);
+    launch_scan_blocks(blkcnt, nb, off, d_nruns, st);
+    hipLaunchKernelGGL(k_head_scatter, dim3((uint32_t)nb), dim3(CMP_BLOCK), 0, st, pw, n_ub, d_np, off, heads);
+}
+
+void launch_count_live(const gkey *keys, const gval *values, uint64_t w, uint64_t W, uint32_t *chunkcnt,
+                       hipStream_t st) {
+    uint64_t nb = (W + CHUNK - 1) / CHUNK;
+    hipLaunchKernelGGL(k_count_live, dim3((uint32_t)nb), dim3(256), 0, st, keys, values, w, W, chunkcnt);
+}
+
+void launch_compact_live(const gkey *keys, const gval *values, uint64_t w, uint64_t W, const uint32_t *chunkoff,
+                         gkey *tk, gval *tv, hipStream_t st) {
+    uint64_t nb = (W + CHUNK - 1) / CHUNK;
+    hipLaunchKernelGGL(k_compact_live, dim3((uint32_t)nb), dim3(256), 0, st, keys, values, w, W, chunkoff, tk, tv);
+}
+
+void launch_merge_redistribute(const gkey *tk, const gval *tv, uint64_t c, const gkey *ik, const gval *iv,
+                               uint64_t r, gkey *keys, gval *values, gkey *seps, uint32_t *row_offset,
+                               uint64_t w, uint64_t W, uint32_t S, hipStream_t st) {
+    uint64_t m = c + r;
+    if (m) hipLaunchKernelGGL(k_merge_scatter, dim3(grid_for(m, 256, 8192)), dim3(256), 0, st, tk, tv, c, ik, iv, r,
+                              keys, values, seps, row_offset, w, W, S);
+    hipLaunchKernelGGL(k_fill_gaps, dim3(grid_for(W, 256, 8192)), dim3(256), 0, st, m, keys, values, seps, w, W, S);
+}
+
+void launch_mark_done(uint8_t *pd, uint64_t a, uint64_t b, hipStream_t st) {
+    if (b > a) fill_u8(pd + a, 1, b - a, st);
+}
+
+void launch_write_compacted(const gkey *tk, const gval *tv, uint64_t c, gkey *keys, gval *values, uint64_t C,
+                            gkey *seps, uint32_t *row_offset, uint32_t S, hipStream_t st) {
+    hipLaunchKernelGGL(k_write_compacted, dim3(grid_for(C, 256, 8192)), dim3(256), 0, st, tk, tv, c, keys, values, C,
+                       seps, row_offset, S);
+}
+
+}  // namespace gpma

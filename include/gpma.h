@@ -1,0 +1,142 @@
+/*
+ * gpma.h - C ABI of the MI355X (gfx950) GPMA: a gapped, sorted packed array
+ * of 64-bit edge keys (src << 32 | dst) with per-row guard keys and a CSR
+ * row_offset view, updated in batches and traversed by BFS on the GPU.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns
+ * an int status (GPMA_OK = 0) and never throws. gpma_last_error() returns a
+ * thread-local message for the last failure. One handle is used by one host
+ * thread at a time; each handle owns one HIP stream and all calls are
+ * synchronous with respect to the caller unless stated otherwise.
+ *
+ * Key and value encodings follow the reference (utils.cuh:11,19-21;
+ * gpma.cuh:15-19): key = src << 32 | dst; value 0 = delete / tombstone,
+ * any other value = insert / live. KEY_NONE marks a gap, KEY_MAX is the
+ * sentinel, a key whose low word is 0xFFFFFFFF is row v's guard.
+ */
+#ifndef GPMA_AMD_GPMA_H
+#define GPMA_AMD_GPMA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GPMA_OK 0
+#define GPMA_ERR_ARG 1       /* bad argument (null pointer, size mismatch, start >= N, ...) */
+#define GPMA_ERR_HIP 2       /* a HIP runtime call failed */
+#define GPMA_ERR_OOM 3       /* device allocation failed */
+#define GPMA_ERR_CAPACITY 4  /* caller buffer too small, or u32 slot index would overflow */
+#define GPMA_ERR_KEY_RANGE 5 /* some update keys had src or dst >= row_num; they were skipped */
+#define GPMA_ERR_NO_DEVICE 6 /* no HIP device visible */
+
+#define GPMA_KEY_NONE 0xFFFFFFFFFFFFFFFFull
+#define GPMA_KEY_MAX 0xFFFFFFFFFFFFFFFEull
+#define GPMA_COL_IDX_NONE 0xFFFFFFFFu
+
+typedef struct gpma_handle *gpma_t;
+
+/* Optional construction parameters; pass NULL for the reference's values. */
+typedef struct gpma_config {
+    uint32_t struct_size;      /* sizeof(gpma_config) */
+    int32_t device;            /* HIP device ordinal, -1 = current device */
+    uint32_t bulk_threshold;   /* pending inserts that take the bulk path (reference: 5,000,000) */
+    uint32_t reserved[5];
+} gpma_config;
+
+/* State summary. Pointers are device pointers owned by the handle and are
+ * valid until the next gpma_update / gpma_destroy. */
+typedef struct gpma_csr {
+    const uint64_t *keys;        /* [capacity] */
+    const uint8_t *values;       /* [capacity] */
+    const uint32_t *row_offset;  /* [row_num + 1] */
+    uint64_t capacity;           /* slots, a power of two: segment_length << tree_height */
+    uint32_t segment_length;
+    uint32_t tree_height;
+    uint32_t row_num;
+    uint32_t reserved;
+} gpma_csr;
+
+typedef struct gpma_stats {
+    uint64_t capacity;
+    uint32_t segment_length;
+    uint32_t tree_height;
+    uint32_t row_num;
+    uint32_t last_levels;        /* rebalance levels run by the last update */
+    uint64_t updates_applied;    /* cumulative update records accepted */
+    uint64_t invalid_skipped;    /* cumulative keys skipped for src/dst >= row_num */
+    uint64_t dropped;            /* cumulative inserts lost to a failed post-resize root
+                                    rebalance (reference gpma.cuh:1034-1044 drops them too) */
+    uint64_t resizes;
+    uint64_t bulk_loads;
+    uint64_t device_bytes;       /* device memory held by the handle */
+} gpma_stats;
+
+/* GPMA(row_num) + init_csr_gpma (reference gpma.cuh:54-82): a 4-slot seed
+ * tree, then one update inserting the guard key of every row v < row_num. */
+int gpma_create(uint32_t row_num, const gpma_config *cfg, gpma_t *out);
+int gpma_destroy(gpma_t g);
+
+/* update_gpma(gpma, keys, values) (reference gpma.cuh:968-1050): apply one
+ * batch of n updates. keys/values are host pointers (on_device = 0) or
+ * device pointers (on_device = 1); they are read only (the reference sorts
+ * and compacts the caller's vectors in place; this ABI does not). Keys must
+ * have src < row_num and dst < row_num; other keys are skipped and reported
+ * with GPMA_ERR_KEY_RANGE after the rest of the batch is applied. Duplicate
+ * keys within one batch: the last record in input order wins. */
+int gpma_update(gpma_t g, const uint64_t *keys, const uint8_t *values, uint64_t n, int on_device);
+
+/* gpma_bfs(gpma, node_size, edge_size, start_node, results) (reference
+ * gpma_bfs.cuh:517-533): levels[v] = 1 + hop distance from start over live
+ * edges, 0 if unreachable; *reachable = #{v : levels[v] != 0}
+ * (gpma_bfs_demo.cu:80). levels is a host (on_device = 0) or device
+ * (on_device = 1) array of row_num entries; reachable may be NULL. */
+int gpma_bfs(gpma_t g, uint32_t start, uint32_t *levels, int on_device, uint32_t *reachable);
+
+/* gpma_bfs(keys, values, row_offsets, node_size, edge_size, start_node,
+ * results) (reference gpma_bfs.cuh:511-515): BFS over a caller-provided
+ * device CSR view in the GPMA encoding. edge_size is accepted for signature
+ * compatibility and only used as a sizing hint. results is a host array of
+ * node_size entries. */
+int gpma_csr_bfs(const uint64_t *keys, const uint8_t *values, const uint32_t *row_offsets, uint32_t node_size,
+                 uint32_t edge_size, uint32_t start_node, uint32_t *results);
+
+/* Public members of the reference's GPMA (gpma.cuh:31-52): device pointers
+ * to keys, values, row_offset plus segment_length, tree_height, row_num. */
+int gpma_get_csr(gpma_t g, gpma_csr *out);
+
+/* Host copies of the whole layout (GPMA::mirror(), gpma.cuh:90-101): keys and
+ * values [capacity], row_offset [row_num + 1]. Any pointer may be NULL. */
+int gpma_copy_layout(gpma_t g, uint64_t *keys, uint8_t *values, uint32_t *row_offset);
+
+/* Sorted live keys (key != KEY_NONE and value != 0, guards and the two
+ * KEY_MAX sentinels included) into a host buffer of cap entries; *n gets the
+ * live count. With out == NULL only the count is returned. */
+int gpma_export_live(gpma_t g, uint64_t *out, uint64_t cap, uint64_t *n);
+
+int gpma_get_stats(gpma_t g, gpma_stats *out);
+
+/* Wait for all work queued on the handle's stream. */
+int gpma_sync(gpma_t g);
+
+/* The HIP stream the handle launches on (as void*, a hipStream_t). */
+void *gpma_stream(gpma_t g);
+
+/* Message of the last failed call on this host thread ("" if none). */
+const char *gpma_last_error(void);
+
+/* Seeded generators (gen.cpp). R-MAT: unique, self-loop-free, shuffled;
+ * *n_out = number of unique edges (call with src = dst = NULL to size). */
+int gpma_gen_rmat(uint32_t scale, uint64_t ndraws, double a, double b, double c, uint64_t seed, uint32_t *src,
+                  uint32_t *dst, uint64_t cap, uint64_t *n_out);
+/* Power-law out-degree graph with exactly E unique edges (see gen.cpp). */
+int gpma_gen_powerlaw(uint32_t N, uint64_t E, double alpha, double soft_cap, double p1, uint64_t seed, uint32_t *src,
+                      uint32_t *dst);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPMA_AMD_GPMA_H */

@@ -1,0 +1,170 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Layout checks are exact: keys[], live flags and row_offset[] of the GPU
+array equal the oracle's restatement of the reference after every batch.
+Live-key sets equal the closed form of the sliding window; BFS levels equal
+the oracle and the reference's own cpu_baseline fixtures.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import gpma_bfs_amd as G
+from oracle import oracle as O
+from tests.helpers import assert_invariants, assert_layout_equal, closed_form_live, protocol_batches
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_four_node_notes_traces():
+    """Notes pp.2-4: N=4, insert 1->2, then +1->3 / -1->2."""
+    g = G.GPMA(4)
+    o = O.OracleGPMA(4)
+    assert_layout_equal(g, o, "guards")
+    k, v, r = g.copy_layout()
+    F = 0xFFFFFFFF
+    expect = [(0 << 32) | F, (1 << 32) | F, (2 << 32) | F, G.KEY_NONE, (3 << 32) | F, G.KEY_MAX, G.KEY_MAX, G.KEY_NONE]
+    assert [int(x) for x in k] == expect
+    assert r.tolist() == [0, 0, 1, 2, 4]
+    for batch, vals in [([(1 << 32) | 2], [1]), ([(1 << 32) | 3, (1 << 32) | 2], [1, 0])]:
+        b = np.array(batch, np.uint64)
+        vv = np.array(vals, np.uint8)
+        G.update_gpma(g, b, vv)
+        o.update(b, vv)
+        assert_layout_equal(g, o, f"batch {batch}")
+    k, v, r = g.copy_layout()
+    assert r.tolist() == [0, 0, 4, 5, 8]
+    assert g.get_size() == 16 and g.segment_length == 4 and g.tree_height == 2
+
+
+@pytest.mark.parametrize("seed,N,draw_scale", [(3, 200, 12), (5, 1000, 14), (11, 64, 10)])
+def test_small_protocol_layout_exact(seed, N, draw_scale):
+    rng_src, rng_dst = G.gen_rmat(int(np.ceil(np.log2(N))), 1 << draw_scale, seed=seed)
+    m = (rng_src < N) & (rng_dst < N)
+    src, dst = rng_src[m], rng_dst[m]
+    keys = G.make_keys(src, dst)
+    g = G.GPMA(N)
+    o = O.OracleGPMA(N)
+    assert_layout_equal(g, o, "guards")
+    half = len(keys) // 2
+    ones = np.ones(half, np.uint8)
+    G.update_gpma(g, keys[:half], ones)
+    o.update(keys[:half], ones)
+    assert_layout_equal(g, o, "window")
+    assert np.array_equal(g.export_live(), closed_form_live(keys, N, 0, num_slide=20))
+    for i, b, vals in protocol_batches(keys, 20, num_slide=20):
+        G.update_gpma(g, b, vals)
+        o.update(b, vals)
+        assert_layout_equal(g, o, f"slide {i}")
+        assert np.array_equal(g.export_live(), closed_form_live(keys, N, i + 1, num_slide=20)), f"slide {i}"
+    assert_invariants(g)
+    for start in (0, 1, N - 1):
+        lv, reach = G.gpma_bfs(g, start)
+        olv, oreach = o.bfs(start)
+        assert reach == oreach and np.array_equal(lv, olv), f"bfs from {start}"
+
+
+def test_c1_protocol():
+    """R-MAT 2^16 / 2^20 (seed 1), full 100-slide protocol: live set after every
+    batch equals the closed form; layout equals the oracle every 10 slides;
+    BFS levels equal the reference cpu_baseline fixture."""
+    N = 1 << 16
+    src, dst = G.gen_rmat(16, 1 << 20, seed=1)
+    keys = G.make_keys(src, dst)
+    g = G.GPMA(N)
+    o = O.OracleGPMA(N)
+    half = len(keys) // 2
+    ones = np.ones(half, np.uint8)
+    G.update_gpma(g, keys[:half], ones)
+    o.update(keys[:half], ones)
+    assert_layout_equal(g, o, "window")
+    fx = np.load(os.path.join(GOLDEN, "c1_rmat16_seed1.npz"))
+    lv, reach = G.gpma_bfs(g, 0)
+    assert np.array_equal(lv, fx["levels_before"]) and reach == int(fx["reach_before"])
+    for i, b, vals in protocol_batches(keys, 100):
+        G.update_gpma(g, b, vals)
+        o.update(b, vals)
+        live = g.export_live()
+        assert np.array_equal(live, closed_form_live(keys, N, i + 1)), f"slide {i}"
+        if i % 10 == 9:
+            assert_layout_equal(g, o, f"slide {i}")
+    assert_invariants(g)
+    lv, reach = G.gpma_bfs(g, 0)
+    assert np.array_equal(lv, fx["levels_after"]) and reach == int(fx["reach_after"])
+
+
+def test_update_semantics_edge_cases():
+    N = 50
+    g = G.GPMA(N)
+    o = O.OracleGPMA(N)
+    K = lambda s, d: (s << 32) | d  # noqa: E731
+    # empty batch
+    G.update_gpma(g, np.zeros(0, np.uint64), np.zeros(0, np.uint8))
+    # delete of an absent key is a no-op (U1)
+    b = np.array([K(1, 2)], np.uint64)
+    G.update_gpma(g, b, np.zeros(1, np.uint8))
+    o.update(b, np.zeros(1, np.uint8))
+    assert_layout_equal(g, o, "absent delete")
+    # insert, delete (tombstone), revive in place (U2)
+    for vals in ([1], [0], [1]):
+        G.update_gpma(g, b, np.array(vals, np.uint8))
+        o.update(b, np.array(vals, np.uint8))
+        assert_layout_equal(g, o, f"op {vals}")
+    live = g.export_live()
+    assert K(1, 2) in set(int(x) for x in live)
+    # duplicate keys in one batch: the last record wins
+    d = np.array([K(3, 4), K(3, 4)], np.uint64)
+    G.update_gpma(g, d, np.array([1, 0], np.uint8))
+    assert K(3, 4) not in set(int(x) for x in g.export_live())
+    G.update_gpma(g, d, np.array([0, 1], np.uint8))
+    assert K(3, 4) in set(int(x) for x in g.export_live())
+    # out-of-range keys are skipped and reported; the rest of the batch applies
+    bad = np.array([K(N, 1), K(2, N), K(5, 6)], np.uint64)
+    rc = G.update_gpma(g, bad, np.ones(3, np.uint8), strict=False)
+    assert rc == G.GPMA_ERR_KEY_RANGE
+    live = set(int(x) for x in g.export_live())
+    assert K(5, 6) in live and K(N, 1) not in live
+    assert g.stats()["invalid_skipped"] == 2
+    assert_invariants(g)
+    # BFS argument errors
+    with pytest.raises(G.GpmaError):
+        G.gpma_bfs(g, N)
+
+
+def test_legacy_csr_bfs_matches():
+    N = 300
+    src, dst = G.gen_rmat(9, 4000, seed=21)
+    m = (src < N) & (dst < N)
+    keys = G.make_keys(src[m], dst[m])
+    g = G.GPMA(N)
+    G.update_gpma(g, keys, np.ones(len(keys), np.uint8))
+    c = g.csr()
+    lv_legacy = G.gpma_csr_bfs(c.keys, c.values, c.row_offset, N, len(keys), 0)
+    lv, _ = G.gpma_bfs(g, 0)
+    assert np.array_equal(lv, lv_legacy)
+    olv, _ = O.bfs_edges(N, src[m], dst[m], 0)
+    assert np.array_equal(lv, olv)
+
+
+def test_bulk_path_matches_oracle():
+    """Bulk load (reference significant_insert) with a lowered threshold."""
+    N = 2000
+    src, dst = G.gen_rmat(11, 40000, seed=9)
+    keys = G.make_keys(src, dst)
+    g = G.GPMA(N, bulk_threshold=5000)
+    half = len(keys) // 2
+    ones = np.ones(half, np.uint8)
+    G.update_gpma(g, keys[:half], ones)
+    assert g.stats()["bulk_loads"] == 1
+    assert np.array_equal(g.export_live(), closed_form_live(keys, N, 0, num_slide=10))
+    assert_invariants(g)
+    for i, b, vals in protocol_batches(keys, 10, num_slide=10):
+        G.update_gpma(g, b, vals)
+        assert np.array_equal(g.export_live(), closed_form_live(keys, N, i + 1, num_slide=10))
+    lv, reach = G.gpma_bfs(g, 0)
+    E = len(keys)
+    step = half // 10
+    olv, oreach = O.bfs_edges(N, src[10 * step: half + 10 * step], dst[10 * step: half + 10 * step], 0)
+    assert reach == oreach and np.array_equal(lv, olv)
