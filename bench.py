@@ -1,0 +1,232 @@
+#!/usr/bin/env python3
+"""Benchmark: the plain drop-in update path (update_gpma) and BFS on one MI355X.
+
+Workload (the reference protocol, gpma_bfs_demo.cu:50-111, on a seeded
+synthetic graph): a directed power-law graph with 1,632,803 nodes and
+30,622,564 edges; the first half is loaded as the window, then every step
+slides the window once: one update_gpma batch of 153,112 inserts followed
+by 153,112 deletes (306,224 edge updates). BFS from node 0 runs after the
+timed slides.
+
+A step is one update batch. Batches are staged in HBM before the timed
+region; W warmup slides, then K timed slides bracketed by device syncs.
+value = edge updates / second over the K slides (wall clock). Per-kernel
+durations come from HIP events the library records on its own stream over
+the same timed region. The reference's own cpu_baseline (compiled from its
+sources into oracle/_ref) is timed on a bounded sample of the same workload
+on the host.
+
+Prints progress on stderr and ONE JSON line on stdout.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+S_REF = 16  # reference segment length at every capacity in [2^15, 2^30]
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def update_bytes(n_ins, n_del, S=S_REF):
+    """Algorithmic HBM bytes of one update batch: 9 B per record read plus 9 B
+    per sorted record written; each insert reads and rewrites one S-slot leaf
+    of 9-B slots; each delete scans one leaf's keys and writes a tombstone."""
+    n = n_ins + n_del
+    return {"sort": 18 * n, "leaf": 18 * S * n_ins, "locate": (8 * S + 1) * n_del,
+            "total": 18 * n + 18 * S * n_ins + (8 * S + 1) * n_del}
+
+
+def bfs_bytes(N, outdeg, levels):
+    reached = levels != 0
+    E_t = int(outdeg[reached].sum())
+    return 16 * int(reached.sum()) + 13 * E_t + 4 * N, E_t
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--bfs-reps", type=int, default=5)
+    ap.add_argument("--cpu-slides", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--small", action="store_true", help="R-MAT 2^16/2^20 instead (quick check)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(sharding is described in DESIGN.md, not run)", file=sys.stderr)
+        sys.exit(2)
+    if args.warmup + args.steps > 100:
+        print("bench.py: warmup + steps must be <= 100 (the protocol has 100 slides)", file=sys.stderr)
+        sys.exit(2)
+
+    import gpma_bfs_amd as G
+
+    t0 = time.time()
+    if args.small:
+        N = 1 << 16
+        src, dst = G.gen_rmat(16, 1 << 20, seed=1)
+        workload = "R-MAT N=2^16, 2^20 draws (a,b,c=.57,.19,.19, seed 1), dedup, shuffled"
+    else:
+        N, E = 1632803, 30622564
+        src, dst = G.gen_powerlaw(N, E, alpha=2.1, soft_cap=2800.0, p1=0.24, seed=0)
+        workload = ("power-law N=1,632,803 E=30,622,564 (Pareto out-degree alpha 2.1, soft cap 2800, "
+                    "R-MAT-skewed dst p1=0.24, seed 0), shuffled")
+    keys = G.make_keys(src, dst)
+    E = len(keys)
+    half = E // 2
+    step = half // 100
+    log(f"generated {E} edges over {N} nodes in {time.time() - t0:.1f}s; half={half} step={step}")
+
+    t = time.time()
+    g = G.GPMA(N)
+    t_guard = time.time() - t
+    t = time.time()
+    G.update_gpma(g, keys[:half], np.ones(half, np.uint8))
+    t_load = time.time() - t
+    st = g.stats()
+    log(f"guards {t_guard:.3f}s, initial window {t_load:.3f}s -> C={st['capacity']} S={st['segment_length']} "
+        f"h={st['tree_height']} bulk={st['bulk_loads']}")
+
+    nslides = args.warmup + args.steps
+    nb = 2 * step
+    batches = np.empty(nslides * nb, np.uint64)
+    for i in range(nslides):
+        batches[i * nb: i * nb + step] = keys[half + i * step: half + (i + 1) * step]
+        batches[i * nb + step: (i + 1) * nb] = keys[i * step: (i + 1) * step]
+    vals = np.tile(np.concatenate([np.ones(step, np.uint8), np.zeros(step, np.uint8)]), nslides)
+    dk = G.DeviceBuffer.from_numpy(batches)
+    dv = G.DeviceBuffer.from_numpy(vals)
+    log(f"staged {nslides} batches of {nb} updates in HBM")
+
+    for i in range(args.warmup):
+        G.update_gpma_device(g, dk.ptr + i * nb * 8, dv.ptr + i * nb, nb)
+    g.sync()
+    g.profile(True, reset=True)
+    per = []
+    g.sync()
+    t_start = time.perf_counter()
+    for i in range(args.warmup, nslides):
+        a = time.perf_counter()
+        G.update_gpma_device(g, dk.ptr + i * nb * 8, dv.ptr + i * nb, nb)
+        per.append(time.perf_counter() - a)
+    g.sync()
+    t_total = time.perf_counter() - t_start
+    g.profile(False)
+    prof = g.profile_report()
+    K = args.steps
+    ups = K * nb / t_total
+    per_ms = np.array(per) * 1e3
+    log(f"{K} timed slides: {t_total * 1e3:.2f} ms total, {ups / 1e9:.3f} G edge-updates/s, "
+        f"p50 {np.percentile(per_ms, 50):.3f} ms p99 {np.percentile(per_ms, 99):.3f} ms")
+
+    # BFS after the slides, from node 0
+    win = keys[nslides * step: half + nslides * step]
+    outdeg = np.bincount((win >> np.uint64(32)).astype(np.int64), minlength=N)
+    lv, reach = G.gpma_bfs(g, 0)
+    bfs_b, e_trav = bfs_bytes(N, outdeg, lv)
+    g.profile(True, reset=True)
+    bt = []
+    for _ in range(args.bfs_reps):
+        a = time.perf_counter()
+        _, r2 = G.gpma_bfs(g, 0)
+        bt.append(time.perf_counter() - a)
+        assert r2 == reach
+    g.profile(False)
+    bprof = g.profile_report()["bfs_expand"]
+    bfs_s = float(np.median(bt))
+    bfs_kernel_s = bprof["ms"] / 1e3 / args.bfs_reps
+    log(f"BFS: reach {reach}, {e_trav} edges, {bfs_s * 1e3:.2f} ms wall, {bfs_kernel_s * 1e3:.2f} ms in expand kernels")
+
+    ub = update_bytes(step, step)
+    step_s = t_total / K
+    phase_ms = {k: v["ms"] / K for k, v in prof.items() if k != "bfs_expand"}
+    # attribute algorithmic bytes to the kernels that move them
+    attr = {"sort": ub["sort"], "leaf": ub["leaf"], "locate": ub["locate"]}
+    cands = {k: phase_ms[k] for k in attr if phase_ms.get(k, 0) > 0}
+    dom = max(cands, key=cands.get)
+    dom_s = cands[dom] / 1e3
+    dom_launches = prof[dom]["launches"] / K
+    roofline = {
+        "bound": "hbm", "kernel": {"sort": "k_sort_hist + k_sort_pass (radix sort of the batch)",
+                                   "leaf": "k_leaf<16> (leaf-level rebalance)",
+                                   "locate": "k_locate (batched leaf locate)"}[dom],
+        "achieved": attr[dom] / dom_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": attr[dom] / dom_s / 1e9 / HBM_PEAK_GBS, "traffic": None,
+        "algorithmic_bytes_per_launch": attr[dom] / max(dom_launches, 1), "launches_per_step": dom_launches,
+        "kernel_ms_per_step": cands[dom],
+    }
+    step_roofline = {"algorithmic_bytes": ub["total"], "achieved": ub["total"] / step_s / 1e9,
+                     "frac": ub["total"] / step_s / 1e9 / HBM_PEAK_GBS, "unit": "GB/s"}
+    bfs_info = {"reachable": int(reach), "edges_traversed": e_trav, "ms": bfs_s * 1e3,
+                "gteps": e_trav / bfs_s / 1e9, "kernel_ms": bfs_kernel_s * 1e3,
+                "algorithmic_bytes": bfs_b, "achieved_gbs": bfs_b / bfs_kernel_s / 1e9 if bfs_kernel_s else None,
+                "frac": (bfs_b / bfs_kernel_s / 1e9 / HBM_PEAK_GBS) if bfs_kernel_s else None}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            from oracle import oracle as O
+
+            a = time.time()
+            tm, _, _, _ = O.run_ref_cpu_baseline(N, src, dst, args.cpu_slides, 0, timing_only=True,
+                                                 workdir="/tmp", timeout=600)
+            cpu_ups = 2 * tm["step"] * tm["slides"] / tm["slides_s"]
+            cpu = {"value": cpu_ups, "unit": "edge-updates/s", "cores": 1, "kind": "reference",
+                   "sample": f"reference cpu_baseline (pma_dynamic_graph, g++ -O3) on the same edge list: guard init, "
+                             f"init_edge of the half window, {tm['slides']} slides of {tm['step']} deletes + "
+                             f"{tm['step']} inserts; value = slide updates / slide time",
+                   "cpu_model": cpu_model(), "host_cores": os.cpu_count(),
+                   "bfs_ms": tm["bfs1_s"] * 1e3, "guard_init_s": tm["guard_init_s"],
+                   "init_edge_s": tm["init_edge_s"], "wall_s": time.time() - a}
+            log(f"cpu_baseline: {cpu_ups / 1e6:.3f} M edge-updates/s (1 core), bfs {tm['bfs1_s'] * 1e3:.1f} ms")
+        except Exception as e:  # report, do not fail the GPU measurement
+            log(f"cpu_baseline skipped: {e}")
+
+    out = {
+        "metric": "edge-updates/sec + BFS GTEPS on 1 GPU, % HBM peak; batched vs plain; p99 latency",
+        "value": ups, "unit": "edge-updates/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "ms_per_step": step_s * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64", "data": "synthetic",
+        "config": {"workload": ("plain drop-in update_gpma, " + workload +
+                                f"; window = first {half} edges; step = {nb} updates ({step} inserts + {step} deletes)"),
+                   "form": "plain", "batch": nb, "nodes": N, "edges": E},
+        "roofline": roofline,
+        "step_roofline": step_roofline,
+        "cpu_baseline": cpu,
+        "bfs": bfs_info,
+        "latency_ms": {"p50": float(np.percentile(per_ms, 50)), "p99": float(np.percentile(per_ms, 99))},
+        "phase_ms_per_step": phase_ms,
+        "layout": {k: st[k] for k in ("capacity", "segment_length", "tree_height")},
+        "init": {"guards_s": t_guard, "initial_window_s": t_load},
+    }
+    print(json.dumps(out), flush=True)
+    dk.free()
+    dv.free()
+    g.close()
+
+
+if __name__ == "__main__":
+    main()

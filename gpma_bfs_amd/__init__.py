@@ -121,6 +121,58 @@ class GPMA:
     def sync(self):
         check(lib.gpma_sync(self.handle), "gpma_sync")
 
+    def profile(self, enable=True, reset=False):
+        """Per-phase HIP-event timing on the handle's stream."""
+        check(lib.gpma_profile(self.handle, 2 if (enable and reset) else int(bool(enable))), "gpma_profile")
+
+    def profile_report(self):
+        ms = (ctypes.c_double * len(PHASES))()
+        calls = (ctypes.c_uint64 * len(PHASES))()
+        check(lib.gpma_get_profile(self.handle, ms, calls, len(PHASES)), "gpma_get_profile")
+        return {name: {"ms": ms[i], "launches": calls[i]} for i, name in enumerate(PHASES)}
+
+
+PHASES = ("stage", "sort", "locate", "compact", "leaf", "upper", "rebuild", "bfs_expand")
+
+
+class DeviceBuffer:
+    """Device allocation through the library's own HIP runtime (plumbing for
+    benchmarks and tests: keeps batches resident in HBM)."""
+
+    def __init__(self, nbytes):
+        p = ctypes.c_void_p()
+        check(lib.gpma_device_malloc(ctypes.byref(p), nbytes), "gpma_device_malloc")
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        check(lib.gpma_memcpy(b.ptr, a.ctypes.data, a.nbytes, 0), "gpma_memcpy")
+        return b
+
+    def to_numpy(self, dtype, count, offset_bytes=0):
+        out = np.empty(count, dtype)
+        check(lib.gpma_memcpy(out.ctypes.data, self.ptr + offset_bytes, out.nbytes, 1), "gpma_memcpy")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib.gpma_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.free()
+
+
+def update_gpma_device(gpma, keys_ptr, values_ptr, n, strict=True):
+    """update_gpma on batches already resident in device memory (raw pointers)."""
+    rc = lib.gpma_update(gpma.handle, keys_ptr, values_ptr, n, 1)
+    if strict:
+        check(rc, "gpma_update")
+    return rc
+
 
 def init_csr_gpma(row_num, **kw):
     """Older initialiser name the reference's drivers log (gpma_bfs_demo.cu:63)."""

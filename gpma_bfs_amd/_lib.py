@@ -93,6 +93,12 @@ SIGNATURES = {
     "gpma_sync": (_I, [_P]),
     "gpma_stream": (_P, [_P]),
     "gpma_last_error": (ctypes.c_char_p, []),
+    "gpma_device_count": (_I, []),
+    "gpma_profile": (_I, [_P, _I]),
+    "gpma_get_profile": (_I, [_P, _P, _P, _U32]),
+    "gpma_device_malloc": (_I, [_P, _U64]),
+    "gpma_device_free": (_I, [_P]),
+    "gpma_memcpy": (_I, [_P, _P, _U64, _I]),
     "gpma_gen_rmat": (_I, [_U32, _U64, ctypes.c_double, ctypes.c_double, ctypes.c_double, _U64, _P, _P, _U64, _P]),
     "gpma_gen_powerlaw": (_I, [_U32, _U64, ctypes.c_double, ctypes.c_double, ctypes.c_double, _U64, _P, _P]),
 }

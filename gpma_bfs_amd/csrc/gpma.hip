@@ -127,6 +127,12 @@ struct gpma_handle {
     // stats
     uint64_t applied = 0, invalid = 0, dropped = 0, resizes = 0, bulk = 0;
     uint32_t last_levels = 0;
+    // phase timing (HIP events on st)
+    bool prof = false;
+    hipEvent_t ev_a[GPMA_PHASES] = {}, ev_b[GPMA_PHASES] = {};
+    bool ev_pending[GPMA_PHASES] = {};
+    double ph_ms[GPMA_PHASES] = {};
+    uint64_t ph_calls[GPMA_PHASES] = {};
 };
 
 namespace {
@@ -150,9 +156,37 @@ void recalc_density(gpma_handle *g) {
     }
 }
 
+enum Phase : int { P_STAGE = 0, P_SORT, P_LOCATE, P_COMPACT, P_LEAF, P_UPPER, P_REBUILD, P_BFS };
+
+void phase_begin(gpma_handle *g, int p) {
+    if (g->prof) (void)hipEventRecord(g->ev_a[p], g->st);
+}
+void phase_end(gpma_handle *g, int p, uint64_t launches) {
+    if (!g->prof) return;
+    (void)hipEventRecord(g->ev_b[p], g->st);
+    g->ev_pending[p] = true;
+    g->ph_calls[p] += launches;
+}
+// After a stream synchronisation: fold finished event pairs into the totals.
+void phase_collect(gpma_handle *g) {
+    if (!g->prof) return;
+    for (int p = 0; p < GPMA_PHASES; p++) {
+        if (!g->ev_pending[p]) continue;
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, g->ev_a[p], g->ev_b[p]) == hipSuccess) g->ph_ms[p] += ms;
+        g->ev_pending[p] = false;
+    }
+}
+
+int stream_sync(gpma_handle *g) {
+    HIP_TRY(hipStreamSynchronize(g->st));
+    phase_collect(g);
+    return GPMA_OK;
+}
+
 int sync_read_ctr(gpma_handle *g) {
     HIP_TRY(hipMemcpyAsync(g->hctr, g->ctr.p, C_COUNT * sizeof(uint32_t), hipMemcpyDeviceToHost, g->st));
-    HIP_TRY(hipStreamSynchronize(g->st));
+    TRY(stream_sync(g));
     HIP_TRY(hipGetLastError());
     return GPMA_OK;
 }
@@ -186,7 +220,9 @@ int count_live(gpma_handle *g, uint64_t w, uint64_t W, uint64_t *c) {
 // items and the pending inserts over the whole new array.
 int rebuild(gpma_handle *g, const gkey *pk, const gval *pv, uint64_t np, bool check_bounds) {
     uint64_t c = 0;
+    phase_begin(g, P_REBUILD);
     TRY(count_live(g, 0, g->C, &c));
+    phase_begin(g, P_REBUILD);
     TRY(ensure_tmp(g, g->C));
     launch_compact_live(g->keys.as<gkey>(), g->values.as<gval>(), 0, g->C, g->chunkoff.as<uint32_t>(),
                         g->tmpk.as<gkey>(), g->tmpv.as<gval>(), g->st);
@@ -197,9 +233,23 @@ int rebuild(gpma_handle *g, const gkey *pk, const gval *pv, uint64_t np, bool ch
     g->S = 1u << (fls32(fls32((uint32_t)T)) - 1);
     g->h = fls32((uint32_t)(T / g->S)) - 1;
     recalc_density(g);
+    if (check_bounds) {
+        // The reference checks nothing here and loses the pending inserts when
+        // the trimmed root bound is below live + pending (gpma.cuh:1034-1044);
+        // grow until they fit instead, and place them even below the lower bound.
+        while (merge > g->upper[g->h]) {
+            T *= 2;
+            if (T > (1ull << 31)) return fail(GPMA_ERR_CAPACITY, "packed array would exceed 2^31 slots");
+            g->S = 1u << (fls32(fls32((uint32_t)T)) - 1);
+            g->h = fls32((uint32_t)(T / g->S)) - 1;
+            recalc_density(g);
+        }
+    }
     if (T != g->C) {
         // tmp already holds every live item, so the old arrays can go first
-        HIP_TRY(hipStreamSynchronize(g->st));
+        phase_end(g, P_REBUILD, 1);
+        TRY(stream_sync(g));
+        phase_begin(g, P_REBUILD);
         g->keys.release(&g->dev_bytes);
         g->values.release(&g->dev_bytes);
         g->seps.release(&g->dev_bytes);
@@ -210,18 +260,10 @@ int rebuild(gpma_handle *g, const gkey *pk, const gval *pv, uint64_t np, bool ch
     g->C = T;
     if (check_bounds) g->resizes++;
     else g->bulk++;
-    const uint32_t lo = g->lower[g->h], hi = g->upper[g->h];
-    if (!check_bounds || (lo <= merge && merge <= hi)) {
-        launch_merge_redistribute(g->tmpk.as<gkey>(), g->tmpv.as<gval>(), c, pk, pv, np, g->keys.as<gkey>(),
-                                  g->values.as<gval>(), g->seps.as<gkey>(), g->row_offset.as<uint32_t>(), 0, T,
-                                  g->S, g->st);
-    } else {
-        // The reference leaves the compacted array and drops the inserts.
-        launch_write_compacted(g->tmpk.as<gkey>(), g->tmpv.as<gval>(), c, g->keys.as<gkey>(),
-                               g->values.as<gval>(), T, g->seps.as<gkey>(), g->row_offset.as<uint32_t>(), g->S,
-                               g->st);
-        g->dropped += np;
-    }
+    launch_merge_redistribute(g->tmpk.as<gkey>(), g->tmpv.as<gval>(), c, pk, pv, np, g->keys.as<gkey>(),
+                              g->values.as<gval>(), g->seps.as<gkey>(), g->row_offset.as<uint32_t>(), 0, T, g->S,
+                              g->st);
+    phase_end(g, P_REBUILD, 2);
     return GPMA_OK;
 }
 
@@ -288,7 +330,9 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
         src_k = g->in_k.as<gkey>();
         src_v = g->in_v.as<gval>();
     }
+    phase_begin(g, P_STAGE);
     launch_stage(src_k, src_v, g->sk0.as<gkey>(), g->sv0.as<gval>(), n, g->N, guards_ok, ctr + C_INVALID, st);
+    phase_end(g, P_STAGE, 1);
     // digits: bits of N-1 plus one (so KEY_NONE sorts after every valid key)
     uint32_t nb = std::max<uint32_t>(1, fls32(g->N ? g->N - 1 : 0)) + 1;
     uint32_t dpw = std::min<uint32_t>(4, (nb + 7) / 8);
@@ -296,17 +340,23 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
     uint32_t npass = 0;
     for (uint32_t d = 0; d < dpw; d++) sh.s[npass++] = 8 * d;
     for (uint32_t d = 0; d < dpw; d++) sh.s[npass++] = 32 + 8 * d;
+    phase_begin(g, P_SORT);
     int cur = radix_sort_pairs(g->sk0.as<gkey>(), g->sv0.as<gval>(), g->sk1.as<gkey>(), g->sv1.as<gval>(), n, sh,
                                npass, g->sortwork.as<uint32_t>(), ctr + C_SORTERR, st);
+    phase_end(g, P_SORT, npass + 2);
     const gkey *ks = cur ? g->sk1.as<gkey>() : g->sk0.as<gkey>();
     const gval *vs = cur ? g->sv1.as<gval>() : g->sv0.as<gval>();
     // locate + in-place delete / modify
+    phase_begin(g, P_LOCATE);
     launch_locate(ks, vs, n, g->keys.as<gkey>(), g->values.as<gval>(), g->seps.as<gkey>(), g->C / g->S, g->S,
                   g->win.as<uint32_t>(), g->blkcnt.as<uint32_t>(), st);
+    phase_end(g, P_LOCATE, 1);
     uint64_t nblk = (n + CMP_BLOCK - 1) / CMP_BLOCK;
+    phase_begin(g, P_COMPACT);
     launch_scan_blocks(g->blkcnt.as<uint32_t>(), nblk, g->blkoff.as<uint32_t>(), ctr + C_NP0, st);
     launch_compact(ks, vs, g->win.as<uint32_t>(), nullptr, n, nullptr, g->blkoff.as<uint32_t>(), g->pk0.as<gkey>(),
                    g->pv0.as<gval>(), g->pw0.as<uint32_t>(), 0xFFFFFFFFu, st);
+    phase_end(g, P_COMPACT, 2);
     TRY(sync_read_ctr(g));
     if (g->hctr[C_SORTERR]) return fail(GPMA_ERR_HIP, "radix sort look-back exceeded its spin limit");
     const uint64_t inval = g->hctr[C_INVALID];
@@ -321,7 +371,7 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
     int pc = 0;
     if (np >= g->bulk_threshold) {
         TRY(rebuild(g, g->pk0.as<gkey>(), g->pv0.as<gval>(), np, false));
-        HIP_TRY(hipStreamSynchronize(g->st));
+        TRY(stream_sync(g));
         HIP_TRY(hipGetLastError());
         return inval ? fail(GPMA_ERR_KEY_RANGE, std::to_string(inval) + " keys with src/dst >= row_num skipped")
                      : GPMA_OK;
@@ -336,6 +386,8 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
         uint32_t *d_np = ctr + npslot[pc];
         fill_u8(pd, 0, np, st);
         g->last_levels = level + 1;
+        const int ph = level == 0 ? P_LEAF : P_UPPER;
+        phase_begin(g, ph);
         if (level == 0) {
             launch_leaf(g->S, pk, pv, pw, pd, np, d_np, g->keys.as<gkey>(), g->values.as<gval>(),
                         g->seps.as<gkey>(), g->row_offset.as<uint32_t>(), g->lower[0], g->upper[0], st);
@@ -346,7 +398,8 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
             // wide windows: runs found on the host from the window ids
             std::vector<uint32_t> hw(np);
             HIP_TRY(hipMemcpyAsync(hw.data(), pw, np * 4, hipMemcpyDeviceToHost, st));
-            HIP_TRY(hipStreamSynchronize(st));
+            phase_end(g, ph, 0);
+            TRY(stream_sync(g));
             uint64_t a = 0;
             while (a < np) {
                 uint64_t b = a + 1;
@@ -355,6 +408,7 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
                 uint64_t c = 0;
                 TRY(count_live(g, hw[a], W, &c));
                 const uint64_t m = c + r;
+                phase_begin(g, ph);
                 if (g->lower[level] <= m && m <= g->upper[level]) {
                     launch_compact_live(g->keys.as<gkey>(), g->values.as<gval>(), hw[a], W,
                                         g->chunkoff.as<uint32_t>(), g->tmpk.as<gkey>(), g->tmpv.as<gval>(), st);
@@ -363,16 +417,24 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
                                               g->row_offset.as<uint32_t>(), hw[a], W, g->S, st);
                     launch_mark_done(pd, a, b, st);
                 }
+                phase_end(g, ph, 3);
                 a = b;
+                if (a < np) {
+                    TRY(stream_sync(g));
+                }
             }
+            phase_begin(g, ph);
         }
         // drop placed inserts, move the rest up one level
         const uint32_t mask = ~(uint32_t)(((uint64_t)g->S << (level + 1)) - 1);
         const uint64_t nbp = (np + CMP_BLOCK - 1) / CMP_BLOCK;
+        phase_end(g, ph, 1);
+        phase_begin(g, P_COMPACT);
         launch_count_pending(pd, np, d_np, g->blkcnt.as<uint32_t>(), st);
         launch_scan_blocks(g->blkcnt.as<uint32_t>(), nbp, g->blkoff.as<uint32_t>(), ctr + npslot[pc ^ 1], st);
         launch_compact(pk, pv, pw, pd, np, d_np, g->blkoff.as<uint32_t>(), PK[pc ^ 1]->as<gkey>(),
                        PV[pc ^ 1]->as<gval>(), PW[pc ^ 1]->as<uint32_t>(), mask, st);
+        phase_end(g, P_COMPACT, 3);
         TRY(sync_read_ctr(g));
         pc ^= 1;
         np = g->hctr[npslot[pc]];
@@ -382,7 +444,7 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
         // gpma.cuh:1034-1044: grow (or shrink) and rebalance the root
         TRY(rebuild(g, PK[pc]->as<gkey>(), PV[pc]->as<gval>(), np, true));
     }
-    HIP_TRY(hipStreamSynchronize(g->st));
+    TRY(stream_sync(g));
     HIP_TRY(hipGetLastError());
     if (inval) return fail(GPMA_ERR_KEY_RANGE, std::to_string(inval) + " keys with src/dst >= row_num skipped");
     return GPMA_OK;
@@ -399,14 +461,17 @@ int ensure_bfs(gpma_handle *g) {
 // Level loop of gpma_bfs_multidev_single (gpma_bfs.cuh:446-509) on one device.
 int run_bfs(const gkey *keys, const gval *values, const uint32_t *row_offset, uint32_t N, uint32_t start,
             uint32_t *lev, uint32_t *fa, uint32_t *fb, uint32_t *d_cnt, uint32_t *h_cnt, hipStream_t st,
-            uint32_t *reach_out) {
+            uint32_t *reach_out, gpma_handle *prof = nullptr) {
     launch_bfs_init(lev, N, start, fa, st);
     uint32_t nf = 1, reach = 1, level = 1;
     while (nf) {
         fill_u32(d_cnt, 0, 1, st);
+        if (prof) phase_begin(prof, P_BFS);
         launch_bfs_expand(keys, values, row_offset, fa, nf, fb, d_cnt, lev, level, st);
+        if (prof) phase_end(prof, P_BFS, 1);
         HIP_TRY(hipMemcpyAsync(h_cnt, d_cnt, 4, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
+        if (prof) phase_collect(prof);
         nf = *h_cnt;
         reach += nf;
         std::swap(fa, fb);
@@ -422,6 +487,58 @@ int run_bfs(const gkey *keys, const gval *values, const uint32_t *row_offset, ui
 extern "C" {
 
 const char *gpma_last_error(void) { return g_last_error.c_str(); }
+
+int gpma_profile(gpma_t g, int enable) {
+    if (!g) return fail(GPMA_ERR_ARG, "gpma_profile: null handle");
+    HIP_TRY(hipSetDevice(g->device));
+    if (enable && !g->ev_a[0]) {
+        for (int p = 0; p < GPMA_PHASES; p++) {
+            HIP_TRY(hipEventCreate(&g->ev_a[p]));
+            HIP_TRY(hipEventCreate(&g->ev_b[p]));
+        }
+    }
+    if (enable == 2) {
+        for (int p = 0; p < GPMA_PHASES; p++) {
+            g->ph_ms[p] = 0;
+            g->ph_calls[p] = 0;
+            g->ev_pending[p] = false;
+        }
+    }
+    g->prof = enable != 0;
+    return GPMA_OK;
+}
+
+int gpma_get_profile(gpma_t g, double *ms, uint64_t *calls, uint32_t n) {
+    if (!g) return fail(GPMA_ERR_ARG, "gpma_get_profile: null handle");
+    for (uint32_t p = 0; p < n && p < GPMA_PHASES; p++) {
+        if (ms) ms[p] = g->ph_ms[p];
+        if (calls) calls[p] = g->ph_calls[p];
+    }
+    return GPMA_OK;
+}
+
+int gpma_device_malloc(void **ptr, uint64_t bytes) {
+    if (!ptr) return fail(GPMA_ERR_ARG, "gpma_device_malloc: null ptr");
+    HIP_TRY(hipMalloc(ptr, bytes ? bytes : 1));
+    return GPMA_OK;
+}
+
+int gpma_device_free(void *ptr) {
+    if (ptr) HIP_TRY(hipFree(ptr));
+    return GPMA_OK;
+}
+
+int gpma_memcpy(void *dst, const void *src, uint64_t bytes, int kind) {
+    hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    HIP_TRY(hipMemcpy(dst, src, bytes, k));
+    return GPMA_OK;
+}
+
+int gpma_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
 
 int gpma_create(uint32_t row_num, const gpma_config *cfg, gpma_t *out) {
     if (!out) return fail(GPMA_ERR_ARG, "gpma_create: null out");
@@ -481,7 +598,11 @@ int gpma_destroy(gpma_t g) {
                   &g->pw0, &g->pd0, &g->pk1, &g->pv1, &g->pw1, &g->pd1, &g->tmpk, &g->tmpv, &g->chunkcnt,
                   &g->chunkoff, &g->lev, &g->fr0, &g->fr1, &g->ctr};
     for (Buf *b : all) b->release(&g->dev_bytes);
-    if (g->hctr) hipHostFree(g->hctr);
+    if (g->hctr) (void)hipHostFree(g->hctr);
+    for (int p = 0; p < GPMA_PHASES; p++) {
+        if (g->ev_a[p]) (void)hipEventDestroy(g->ev_a[p]);
+        if (g->ev_b[p]) (void)hipEventDestroy(g->ev_b[p]);
+    }
     if (g->st) hipStreamDestroy(g->st);
     delete g;
     return GPMA_OK;
@@ -500,7 +621,7 @@ int gpma_bfs(gpma_t g, uint32_t start, uint32_t *levels, int on_device, uint32_t
     uint32_t reach = 0;
     TRY(run_bfs(g->keys.as<gkey>(), g->values.as<gval>(), g->row_offset.as<uint32_t>(), g->N, start,
                 g->lev.as<uint32_t>(), g->fr0.as<uint32_t>(), g->fr1.as<uint32_t>(), g->ctr.as<uint32_t>() + C_NEXT,
-                g->hctr + C_NEXT, g->st, &reach));
+                g->hctr + C_NEXT, g->st, &reach, g));
     if (levels) {
         HIP_TRY(hipMemcpyAsync(levels, g->lev.p, (uint64_t)g->N * 4,
                                on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, g->st));

@@ -124,6 +124,26 @@ int gpma_sync(gpma_t g);
 /* The HIP stream the handle launches on (as void*, a hipStream_t). */
 void *gpma_stream(gpma_t g);
 
+/* Per-phase timing with HIP events on the handle's stream (off by default).
+ * Phases: 0 stage, 1 sort, 2 locate, 3 compact, 4 leaf rebalance,
+ * 5 upper-level rebalance, 6 rebuild (resize / bulk load), 7 BFS expand.
+ * gpma_get_profile fills ms[i] (cumulative milliseconds) and calls[i]
+ * (kernel launches timed) for i < n; gpma_profile(g, 0) clears nothing,
+ * gpma_profile(g, 2) resets the counters and enables. */
+#define GPMA_PHASES 8
+int gpma_profile(gpma_t g, int enable);
+int gpma_get_profile(gpma_t g, double *ms, uint64_t *calls, uint32_t n);
+
+/* Device-memory plumbing for callers without their own allocator (tests,
+ * benchmarks): plain hipMalloc / hipFree / hipMemcpy on the current device.
+ * kind: 0 host->device, 1 device->host, 2 device->device. */
+int gpma_device_malloc(void **ptr, uint64_t bytes);
+int gpma_device_free(void *ptr);
+int gpma_memcpy(void *dst, const void *src, uint64_t bytes, int kind);
+
+/* Number of visible HIP devices (0 when none, or when the runtime fails). */
+int gpma_device_count(void);
+
 /* Message of the last failed call on this host thread ("" if none). */
 const char *gpma_last_error(void);
 

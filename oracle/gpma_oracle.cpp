@@ -21,6 +21,10 @@
 //     closed form, by brute force over small (W, m).
 //
 // Deliberate, documented differences from the reference:
+//   * with orc_set_fix_root(1) (off by default), a root rebalance that
+//     would fail after a resize grows the array instead (the product's
+//     behaviour); by default the pending inserts are dropped exactly as
+//     gpma.cuh:1034-1044 does.
 //   * duplicate keys inside one batch are located sequentially in sorted
 //     order (the reference runs them in parallel, with a write race on the
 //     value byte). Workloads contain no intra-batch duplicates.
@@ -35,6 +39,8 @@
 #include <vector>
 
 namespace {
+
+bool g_fix_root = false;
 
 using key_t_ = uint64_t;
 using val_t = uint8_t;  // the reference's VALUE_TYPE is char; only 0 / non-0 matters
@@ -332,9 +338,24 @@ void update_gpma(Gpma &g, std::vector<key_t_> uk, std::vector<val_t> uv) {
     }
     // step 6: grow / shrink and rebalance the root
     if (update_size > 0) {
-        resize_gpma(g, update_size);
+        size_type live = resize_gpma(g, update_size);
         size_type level = g.tree_height;
         size_type lo = g.lower_element[level], hi = g.upper_element[level];
+        if (g_fix_root) {
+            const size_type merge = live + update_size;
+            while (merge > hi) {
+                size_type T = (size_type)g.keys.size() * 2;
+                g.segment_length = 1u << (fls(fls(T)) - 1);
+                g.tree_height = fls(T / g.segment_length) - 1;
+                g.keys.resize(T, KEY_NONE);
+                g.values.resize(T, 0);
+                recalculate_density(g);
+                level = g.tree_height;
+                lo = g.lower_element[level];
+                hi = g.upper_element[level];
+            }
+            if (merge < lo) lo = 0;
+        }
         rebalance_batch(g, level, nodes, uk, uv, uniq, offset, nuniq, lo, hi);
         // Pending updates that still fail here are silently lost by the
         // reference (gpma.cuh:1043 is not checked); count them.
@@ -369,6 +390,7 @@ Gpma *create(size_type row_num) {
 extern "C" {
 
 void *orc_create(uint32_t row_num) { return create(row_num); }
+void orc_set_fix_root(int on) { g_fix_root = on != 0; }
 void orc_destroy(void *h) { delete (Gpma *)h; }
 
 int orc_update(void *h, const uint64_t *keys, const uint8_t *values, uint64_t n) {

@@ -28,6 +28,7 @@ def lib():
         L.orc_create.restype = _P
         L.orc_create.argtypes = [ctypes.c_uint32]
         L.orc_destroy.argtypes = [_P]
+        L.orc_set_fix_root.argtypes = [ctypes.c_int]
         L.orc_update.argtypes = [_P, _P, _P, ctypes.c_uint64]
         L.orc_info.argtypes = [_P, _P]
         L.orc_level_stats.argtypes = [_P, _P, ctypes.c_uint32]
@@ -50,11 +51,18 @@ def lib():
 
 
 class OracleGPMA:
-    """CPU restatement of the reference GPMA (layout-faithful)."""
+    """CPU restatement of the reference GPMA (layout-faithful).
 
-    def __init__(self, row_num):
+    fix_root=True applies the product's one deliberate deviation: a root
+    rebalance that would fail after a resize grows the array instead of
+    dropping the pending inserts (the reference drops them,
+    gpma.cuh:1034-1044). It changes nothing unless that failure occurs."""
+
+    def __init__(self, row_num, fix_root=True):
         self.row_num = row_num
+        lib().orc_set_fix_root(1 if fix_root else 0)
         self.h = lib().orc_create(row_num)
+        self.fix_root = fix_root
 
     def close(self):
         if self.h:
@@ -65,6 +73,7 @@ class OracleGPMA:
         self.close()
 
     def update(self, keys, values):
+        lib().orc_set_fix_root(1 if self.fix_root else 0)
         keys = np.ascontiguousarray(keys, np.uint64)
         values = np.ascontiguousarray(values, np.uint8)
         lib().orc_update(self.h, keys.ctypes.data, values.ctypes.data, len(keys))

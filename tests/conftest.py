@@ -13,10 +13,12 @@ def pytest_configure(config):
 
 
 def gpu_available():
+    # Ask the library's own HIP runtime (importing torch would load a second
+    # HIP runtime into the process).
     try:
-        import torch
+        from gpma_bfs_amd._lib import lib
 
-        return torch.cuda.is_available()
+        return lib.gpma_device_count() > 0
     except Exception:
         return False
 

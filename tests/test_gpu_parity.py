@@ -150,7 +150,7 @@ def test_legacy_csr_bfs_matches():
 
 def test_bulk_path_matches_oracle():
     """Bulk load (reference significant_insert) with a lowered threshold."""
-    N = 2000
+    N = 2048
     src, dst = G.gen_rmat(11, 40000, seed=9)
     keys = G.make_keys(src, dst)
     g = G.GPMA(N, bulk_threshold=5000)
