@@ -1,0 +1,101 @@
+"""CPU tests of the C ABI library and host logic (no GPU compute)."""
+import ctypes
+import hashlib
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import gpma_bfs_amd as G
+from gpma_bfs_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "gpma.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int|void\s*\*|const char\s*\*)\s*(gpma_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_declares_expected_surface():
+    names = declared_functions()
+    for must in ("gpma_create", "gpma_destroy", "gpma_update", "gpma_bfs", "gpma_csr_bfs", "gpma_export_live",
+                 "gpma_get_csr", "gpma_last_error"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, f"not exported: {missing}"
+
+
+def test_python_binding_covers_header():
+    assert set(declared_functions()) <= set(_lib.SIGNATURES)
+
+
+def test_struct_sizes_match_header_layout():
+    assert ctypes.sizeof(_lib.GpmaConfig) == 32
+    assert ctypes.sizeof(_lib.GpmaCsr) == 3 * 8 + 8 + 4 * 4
+    assert ctypes.sizeof(_lib.GpmaStats) == 8 + 4 * 4 + 6 * 8
+
+
+def test_no_device_paths_fail_loudly():
+    if _lib.lib.gpma_device_count() > 0:
+        pytest.skip("a GPU is visible")
+    h = ctypes.c_void_p()
+    rc = _lib.lib.gpma_create(16, None, ctypes.byref(h))
+    assert rc == _lib.GPMA_ERR_NO_DEVICE
+    assert "device" in _lib.last_error()
+    with pytest.raises(G.GpmaError):
+        G.GPMA(16)
+
+
+def test_rmat_generator_deterministic_and_clean():
+    a = G.gen_rmat(12, 50000, seed=3)
+    b = G.gen_rmat(12, 50000, seed=3)
+    c = G.gen_rmat(12, 50000, seed=4)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert not np.array_equal(a[0], c[0])
+    k = G.make_keys(*a)
+    assert len(np.unique(k)) == len(k)
+    assert not np.any(a[0] == a[1])
+    assert a[0].max() < 4096 and a[1].max() < 4096
+
+
+def test_rmat_c1_matches_golden_edge_hash():
+    fx = np.load(os.path.join(ROOT, "tests", "golden", "c1_rmat16_seed1.npz"))
+    src, dst = G.gen_rmat(16, 1 << 20, seed=1)
+    k = G.make_keys(src, dst)
+    assert len(k) == int(fx["n_edges"])
+    assert hashlib.sha256(k.tobytes()).hexdigest() == str(fx["edges_sha256"])
+
+
+def test_powerlaw_generator_shape():
+    N, E = 20000, 300000
+    src, dst = G.gen_powerlaw(N, E, seed=5)
+    assert len(src) == E
+    k = G.make_keys(src, dst)
+    assert len(np.unique(k)) == E
+    assert not np.any(src == dst)
+    assert src.max() < N and dst.max() < N
+    deg = np.bincount(src, minlength=N)
+    assert abs(deg.mean() - E / N) < 1e-9
+    assert deg.max() > 20 * deg.mean()  # heavy tail
+    # R-MAT-skewed destinations: low ids are far more popular than high ones
+    indeg = np.bincount(dst, minlength=N)
+    assert indeg[:N // 16].sum() > indeg[N // 2:].sum()
+    src2, dst2 = G.gen_powerlaw(N, E, seed=5)
+    assert np.array_equal(src, src2) and np.array_equal(dst, dst2)
+
+
+def test_bench_rejects_multi_gpu():
+    p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, text=True, timeout=120)
+    assert p.returncode == 2
+    assert "one GPU" in p.stderr
