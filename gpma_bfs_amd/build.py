@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgpma.so")
+DEMO = os.path.join(HERE, "demo", "gpma_bfs_demo")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 HIP_FLAGS = [
@@ -50,6 +51,10 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp, "-lpthread"])
     os.replace(tmp, LIB)
+    # the reference demo's protocol against the C++ facade (include/gpma.hpp)
+    _run(["g++", "-O2", "-std=c++17", f"-I{os.path.join(os.path.dirname(HERE), 'include')}",
+          os.path.join(HERE, "demo", "gpma_bfs_demo.cpp"), "-o", DEMO, f"-L{HERE}", "-lgpma",
+          f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN"])
     return LIB
 
 
