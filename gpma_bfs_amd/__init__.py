@@ -51,8 +51,8 @@ def _ptr(a, dtype):
 class GPMA:
     """One device-resident GPMA (reference: class GPMA, gpma.cuh:31-102)."""
 
-    def __init__(self, row_num, device=-1, bulk_threshold=0):
-        cfg = GpmaConfig(ctypes.sizeof(GpmaConfig), device, bulk_threshold)
+    def __init__(self, row_num, device=-1, bulk_threshold=0, pull_permille=0):
+        cfg = GpmaConfig(ctypes.sizeof(GpmaConfig), device, bulk_threshold, pull_permille)
         h = ctypes.c_void_p()
         check(lib.gpma_create(row_num, ctypes.byref(cfg), ctypes.byref(h)), "gpma_create")
         self._h = h
@@ -185,6 +185,10 @@ def update_gpma(gpma, keys, values, strict=True):
     Unlike the reference, the inputs are not sorted or compacted in place.
     Returns the status code; with strict=True a non-zero status raises
     (GPMA_ERR_KEY_RANGE still applies the valid part of the batch)."""
+    if isinstance(keys, np.ndarray):
+        keys = np.ascontiguousarray(keys)
+    if isinstance(values, np.ndarray):
+        values = np.ascontiguousarray(values)
     kp, n, kdev = _ptr(keys, np.uint64)
     vp, nv, vdev = _ptr(values, np.uint8)
     if n != nv:
@@ -211,6 +215,14 @@ def gpma_bfs(gpma, start_node, results=None):
     reach = ctypes.c_uint32()
     check(lib.gpma_bfs(gpma.handle, start_node, rp, dev, ctypes.byref(reach)), "gpma_bfs")
     return results, reach.value
+
+
+def gpma_bfs_count(gpma, start_node, levels_device_ptr=None):
+    """BFS that keeps the levels on the device (optionally into a caller's
+    device buffer of row_num uint32); returns only the reachable count."""
+    reach = ctypes.c_uint32()
+    check(lib.gpma_bfs(gpma.handle, start_node, levels_device_ptr, 1, ctypes.byref(reach)), "gpma_bfs")
+    return reach.value
 
 
 def gpma_csr_bfs(keys_ptr, values_ptr, row_offsets_ptr, node_size, edge_size, start_node):

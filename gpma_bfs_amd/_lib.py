@@ -42,7 +42,8 @@ class GpmaConfig(ctypes.Structure):
         ("struct_size", ctypes.c_uint32),
         ("device", ctypes.c_int32),
         ("bulk_threshold", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 5),
+        ("pull_permille", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 4),
     ]
 
 

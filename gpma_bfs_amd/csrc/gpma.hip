@@ -92,6 +92,8 @@ struct Buf {
     }
 };
 
+constexpr uint32_t BFS_MAX_CHUNK = 16;
+
 enum Ctr : uint32_t {
     C_INVALID = 0,
     C_NP0 = 1,
@@ -100,8 +102,12 @@ enum Ctr : uint32_t {
     C_LIVE = 4,
     C_NEXT = 5,
     C_NRUNS = 6,
+    C_LVL = 8,  // C_LVL + L: pending inserts entering level L of the device-driven prefix
     C_COUNT = 16
 };
+// Levels 1..SPEC_LEVELS run without a host round trip; the host reads the
+// counters once after them and drives any higher level itself.
+constexpr uint32_t SPEC_LEVELS = 3;
 
 }  // namespace
 
@@ -121,7 +127,9 @@ struct gpma_handle {
     Buf pk0, pv0, pw0, pd0, pk1, pv1, pw1, pd1;
     Buf tmpk, tmpv, chunkcnt, chunkoff;
     // bfs scratch
-    Buf lev, fr0, fr1;
+    Buf lev, fr0, fr1, fb0, fb1, vis, bcnt, bcntb;
+    unsigned long long *hcnt = nullptr;  // pinned mirror of one chunk of bcnt and bcntb
+    double pull_fraction = 0.25;  // pull when the frontier's slots exceed this share of C
     Buf ctr;
     uint32_t *hctr = nullptr;  // pinned mirror of ctr
     // stats
@@ -330,9 +338,6 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
         src_k = g->in_k.as<gkey>();
         src_v = g->in_v.as<gval>();
     }
-    phase_begin(g, P_STAGE);
-    launch_stage(src_k, src_v, g->sk0.as<gkey>(), g->sv0.as<gval>(), n, g->N, guards_ok, ctr + C_INVALID, st);
-    phase_end(g, P_STAGE, 1);
     // digits: bits of N-1 plus one (so KEY_NONE sorts after every valid key)
     uint32_t nb = std::max<uint32_t>(1, fls32(g->N ? g->N - 1 : 0)) + 1;
     uint32_t dpw = std::min<uint32_t>(4, (nb + 7) / 8);
@@ -341,8 +346,9 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
     for (uint32_t d = 0; d < dpw; d++) sh.s[npass++] = 8 * d;
     for (uint32_t d = 0; d < dpw; d++) sh.s[npass++] = 32 + 8 * d;
     phase_begin(g, P_SORT);
-    int cur = radix_sort_pairs(g->sk0.as<gkey>(), g->sv0.as<gval>(), g->sk1.as<gkey>(), g->sv1.as<gval>(), n, sh,
-                               npass, g->sortwork.as<uint32_t>(), ctr + C_SORTERR, st);
+    int cur = stage_and_sort(src_k, src_v, g->sk0.as<gkey>(), g->sv0.as<gval>(), g->sk1.as<gkey>(), g->sv1.as<gval>(),
+                             n, g->N, guards_ok, ctr + C_INVALID, sh, npass, g->sortwork.as<uint32_t>(),
+                             ctr + C_SORTERR, st);
     phase_end(g, P_SORT, npass + 2);
     const gkey *ks = cur ? g->sk1.as<gkey>() : g->sk0.as<gkey>();
     const gval *vs = cur ? g->sv1.as<gval>() : g->sv0.as<gval>();
@@ -352,31 +358,73 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
                   g->win.as<uint32_t>(), g->blkcnt.as<uint32_t>(), st);
     phase_end(g, P_LOCATE, 1);
     uint64_t nblk = (n + CMP_BLOCK - 1) / CMP_BLOCK;
+    Buf *PK[2] = {&g->pk0, &g->pk1}, *PV[2] = {&g->pv0, &g->pv1}, *PW[2] = {&g->pw0, &g->pw1},
+        *PD[2] = {&g->pd0, &g->pd1};
     phase_begin(g, P_COMPACT);
-    launch_scan_blocks(g->blkcnt.as<uint32_t>(), nblk, g->blkoff.as<uint32_t>(), ctr + C_NP0, st);
+    launch_scan_blocks(g->blkcnt.as<uint32_t>(), nblk, g->blkoff.as<uint32_t>(), ctr + C_LVL, st);
     launch_compact(ks, vs, g->win.as<uint32_t>(), nullptr, n, nullptr, g->blkoff.as<uint32_t>(), g->pk0.as<gkey>(),
-                   g->pv0.as<gval>(), g->pw0.as<uint32_t>(), 0xFFFFFFFFu, st);
+                   g->pv0.as<gval>(), g->pw0.as<uint32_t>(), g->pd0.as<uint8_t>(), 0xFFFFFFFFu, st);
     phase_end(g, P_COMPACT, 2);
+    // Drop placed inserts of list `pc` (count in *d_in), move the rest up to
+    // level+1 into the other list, count in *d_out.
+    auto compact_pending = [&](int pc, uint64_t n_ub, uint32_t level, const uint32_t *d_in, uint32_t *d_out) {
+        const uint32_t mask = ~(uint32_t)(((uint64_t)g->S << (level + 1)) - 1);
+        phase_begin(g, P_COMPACT);
+        launch_count_pending(PD[pc]->as<uint8_t>(), n_ub, d_in, g->blkcnt.as<uint32_t>(), st);
+        launch_scan_blocks(g->blkcnt.as<uint32_t>(), 0, g->blkoff.as<uint32_t>(), d_out, st, d_in);
+        launch_compact(PK[pc]->as<gkey>(), PV[pc]->as<gval>(), PW[pc]->as<uint32_t>(), PD[pc]->as<uint8_t>(), n_ub,
+                       d_in, g->blkoff.as<uint32_t>(), PK[pc ^ 1]->as<gkey>(), PV[pc ^ 1]->as<gval>(),
+                       PW[pc ^ 1]->as<uint32_t>(), PD[pc ^ 1]->as<uint8_t>(), mask, st);
+        phase_end(g, P_COMPACT, 3);
+    };
+    // Device-driven prefix: level 0 and levels 1..spec, launched before the
+    // host knows any pending count; each kernel reads its count on the device,
+    // and all of them stand down if the batch turns out to be a bulk load.
+    phase_begin(g, P_LEAF);
+    launch_leaf(g->S, g->pk0.as<gkey>(), g->pv0.as<gval>(), g->pw0.as<uint32_t>(), g->pd0.as<uint8_t>(), n,
+                ctr + C_LVL, g->keys.as<gkey>(), g->values.as<gval>(), g->seps.as<gkey>(),
+                g->row_offset.as<uint32_t>(), g->lower[0], g->upper[0], g->bulk_threshold, st);
+    phase_end(g, P_LEAF, 1);
+    compact_pending(0, n, 0, ctr + C_LVL, ctr + C_LVL + 1);
+    uint32_t spec = 0;
+    while (spec < SPEC_LEVELS && spec + 1 <= g->h && ((uint64_t)g->S << (spec + 1)) <= WIN_MAX) {
+        const uint32_t L = spec + 1;
+        const int pc = L & 1;
+        phase_begin(g, P_UPPER);
+        launch_window(g->S, g->S << L, PK[pc]->as<gkey>(), PV[pc]->as<gval>(), PW[pc]->as<uint32_t>(),
+                      PD[pc]->as<uint8_t>(), n, ctr + C_LVL + L, g->keys.as<gkey>(), g->values.as<gval>(),
+                      g->seps.as<gkey>(), g->row_offset.as<uint32_t>(), g->lower[L], g->upper[L], ctr + C_LVL,
+                      g->bulk_threshold, st);
+        phase_end(g, P_UPPER, 1);
+        compact_pending(pc, n, L, ctr + C_LVL + L, ctr + C_LVL + L + 1);
+        spec++;
+    }
     TRY(sync_read_ctr(g));
     if (g->hctr[C_SORTERR]) return fail(GPMA_ERR_HIP, "radix sort look-back exceeded its spin limit");
     const uint64_t inval = g->hctr[C_INVALID];
     g->invalid += inval;
     g->applied += n - inval;
-    uint64_t np = g->hctr[C_NP0];
-    g->last_levels = 0;
-    // pending list ping-pong: (pk, pv, pw, pd)[cur] with its count in ctr[np_slot]
-    Buf *PK[2] = {&g->pk0, &g->pk1}, *PV[2] = {&g->pv0, &g->pv1}, *PW[2] = {&g->pw0, &g->pw1},
-        *PD[2] = {&g->pd0, &g->pd1};
-    uint32_t npslot[2] = {C_NP0, C_NP1};
-    int pc = 0;
-    if (np >= g->bulk_threshold) {
-        TRY(rebuild(g, g->pk0.as<gkey>(), g->pv0.as<gval>(), np, false));
+    const uint64_t np0 = g->hctr[C_LVL];
+    if (np0 >= g->bulk_threshold) {
+        g->last_levels = 0;
+        TRY(rebuild(g, g->pk0.as<gkey>(), g->pv0.as<gval>(), np0, false));
         TRY(stream_sync(g));
         HIP_TRY(hipGetLastError());
         return inval ? fail(GPMA_ERR_KEY_RANGE, std::to_string(inval) + " keys with src/dst >= row_num skipped")
                      : GPMA_OK;
     }
-    uint32_t level = 0;
+    g->last_levels = 0;
+    for (uint32_t L = 0; L <= spec; L++)
+        if (g->hctr[C_LVL + L]) g->last_levels = L + 1;
+    // host-driven levels above the prefix (rare: wide windows, growth)
+    uint32_t level = spec + 1;
+    int pc = level & 1;
+    uint64_t np = g->hctr[C_LVL + level];
+    const uint32_t npslot[2] = {C_NP0, C_NP1};
+    if (np) {
+        // move the count into the host loop's slot for this list
+        fill_u32(ctr + npslot[pc], (uint32_t)np, 1, st);
+    }
     while (np > 0 && level <= g->h) {
         const uint64_t W = (uint64_t)g->S << level;
         gkey *pk = PK[pc]->as<gkey>();
@@ -384,21 +432,18 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
         uint32_t *pw = PW[pc]->as<uint32_t>();
         uint8_t *pd = PD[pc]->as<uint8_t>();
         uint32_t *d_np = ctr + npslot[pc];
-        fill_u8(pd, 0, np, st);
         g->last_levels = level + 1;
-        const int ph = level == 0 ? P_LEAF : P_UPPER;
-        phase_begin(g, ph);
-        if (level == 0) {
-            launch_leaf(g->S, pk, pv, pw, pd, np, d_np, g->keys.as<gkey>(), g->values.as<gval>(),
-                        g->seps.as<gkey>(), g->row_offset.as<uint32_t>(), g->lower[0], g->upper[0], st);
-        } else if (W <= WIN_MAX) {
+        phase_begin(g, P_UPPER);
+        if (W <= WIN_MAX) {
             launch_window(g->S, (uint32_t)W, pk, pv, pw, pd, np, d_np, g->keys.as<gkey>(), g->values.as<gval>(),
-                          g->seps.as<gkey>(), g->row_offset.as<uint32_t>(), g->lower[level], g->upper[level], st);
+                          g->seps.as<gkey>(), g->row_offset.as<uint32_t>(), g->lower[level], g->upper[level],
+                          nullptr, 0, st);
+            phase_end(g, P_UPPER, 1);
         } else {
             // wide windows: runs found on the host from the window ids
             std::vector<uint32_t> hw(np);
             HIP_TRY(hipMemcpyAsync(hw.data(), pw, np * 4, hipMemcpyDeviceToHost, st));
-            phase_end(g, ph, 0);
+            phase_end(g, P_UPPER, 0);
             TRY(stream_sync(g));
             uint64_t a = 0;
             while (a < np) {
@@ -408,7 +453,7 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
                 uint64_t c = 0;
                 TRY(count_live(g, hw[a], W, &c));
                 const uint64_t m = c + r;
-                phase_begin(g, ph);
+                phase_begin(g, P_UPPER);
                 if (g->lower[level] <= m && m <= g->upper[level]) {
                     launch_compact_live(g->keys.as<gkey>(), g->values.as<gval>(), hw[a], W,
                                         g->chunkoff.as<uint32_t>(), g->tmpk.as<gkey>(), g->tmpv.as<gval>(), st);
@@ -417,24 +462,12 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
                                               g->row_offset.as<uint32_t>(), hw[a], W, g->S, st);
                     launch_mark_done(pd, a, b, st);
                 }
-                phase_end(g, ph, 3);
+                phase_end(g, P_UPPER, 3);
                 a = b;
-                if (a < np) {
-                    TRY(stream_sync(g));
-                }
+                TRY(stream_sync(g));
             }
-            phase_begin(g, ph);
         }
-        // drop placed inserts, move the rest up one level
-        const uint32_t mask = ~(uint32_t)(((uint64_t)g->S << (level + 1)) - 1);
-        const uint64_t nbp = (np + CMP_BLOCK - 1) / CMP_BLOCK;
-        phase_end(g, ph, 1);
-        phase_begin(g, P_COMPACT);
-        launch_count_pending(pd, np, d_np, g->blkcnt.as<uint32_t>(), st);
-        launch_scan_blocks(g->blkcnt.as<uint32_t>(), nbp, g->blkoff.as<uint32_t>(), ctr + npslot[pc ^ 1], st);
-        launch_compact(pk, pv, pw, pd, np, d_np, g->blkoff.as<uint32_t>(), PK[pc ^ 1]->as<gkey>(),
-                       PV[pc ^ 1]->as<gval>(), PW[pc ^ 1]->as<uint32_t>(), mask, st);
-        phase_end(g, P_COMPACT, 3);
+        compact_pending(pc, np, level, d_np, ctr + npslot[pc ^ 1]);
         TRY(sync_read_ctr(g));
         pc ^= 1;
         np = g->hctr[npslot[pc]];
@@ -452,33 +485,66 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
 
 int ensure_bfs(gpma_handle *g) {
     uint64_t n = (uint64_t)g->N + 1;
-    TRY(g->lev.ensure(n * 4, &g->dev_bytes));
-    TRY(g->fr0.ensure(n * 4, &g->dev_bytes));
-    TRY(g->fr1.ensure(n * 4, &g->dev_bytes));
+    uint64_t *t = &g->dev_bytes;
+    TRY(g->lev.ensure(n * 4, t));
+    TRY(g->fr0.ensure(n * 4, t));
+    TRY(g->fr1.ensure(n * 4, t));
+    TRY(g->fb0.ensure(n * 4, t));
+    TRY(g->fb1.ensure(n * 4, t));
+    TRY(g->vis.ensure((n / 32 + 1) * 4, t));
+    TRY(g->bcnt.ensure((n + BFS_MAX_CHUNK + 2) * 8, t));
+    TRY(g->bcntb.ensure((n + BFS_MAX_CHUNK + 2) * 8, t));
+    if (!g->hcnt) HIP_TRY(hipHostMalloc((void **)&g->hcnt, 2 * (BFS_MAX_CHUNK + 1) * 8, hipHostMallocDefault));
     return GPMA_OK;
 }
 
-// Level loop of gpma_bfs_multidev_single (gpma_bfs.cuh:446-509) on one device.
-int run_bfs(const gkey *keys, const gval *values, const uint32_t *row_offset, uint32_t N, uint32_t start,
-            uint32_t *lev, uint32_t *fa, uint32_t *fb, uint32_t *d_cnt, uint32_t *h_cnt, hipStream_t st,
-            uint32_t *reach_out, gpma_handle *prof = nullptr) {
-    launch_bfs_init(lev, N, start, fa, st);
-    uint32_t nf = 1, reach = 1, level = 1;
-    while (nf) {
-        fill_u32(d_cnt, 0, 1, st);
+BfsState bfs_state(gpma_handle *g) {
+    BfsState b;
+    b.levels = g->lev.as<uint32_t>();
+    b.vis = g->vis.as<uint32_t>();
+    b.q[0] = g->fr0.as<uint32_t>();
+    b.q[1] = g->fr1.as<uint32_t>();
+    b.qbig[0] = g->fb0.as<uint32_t>();
+    b.qbig[1] = g->fb1.as<uint32_t>();
+    b.cnt = g->bcnt.as<unsigned long long>();
+    b.cnt_big = g->bcntb.as<unsigned long long>();
+    return b;
+}
+
+// The level loop of gpma_bfs_multidev_single (gpma_bfs.cuh:446-509) on one
+// device, without a host round trip per level: levels are launched in
+// chunks of growing size and the host reads the chunk's frontier sizes once.
+// hcnt: pinned host buffer of 2 * (BFS_MAX_CHUNK + 1) u64 words.
+int run_bfs(const gkey *keys, const gval *values, uint64_t C, const uint32_t *row_offset, uint32_t N, uint32_t start,
+            const BfsState &b, unsigned long long *hcnt, hipStream_t st, uint32_t *reach_out, double pull_fraction,
+            gpma_handle *prof = nullptr) {
+    launch_bfs_init(b, N, start, row_offset, st);
+    const unsigned long long thr =
+        C ? (unsigned long long)(pull_fraction * (double)C) : ~0ull;  // no pull without the array size
+    uint64_t reach = 1;
+    uint32_t level = 1, chunk = 4;
+    bool done = false;
+    while (!done) {
+        if ((uint64_t)level + chunk + 1 > (uint64_t)N + BFS_MAX_CHUNK) chunk = 1;
+        launch_bfs_clear(b, level + 1, level + chunk + 1, st);
         if (prof) phase_begin(prof, P_BFS);
-        launch_bfs_expand(keys, values, row_offset, fa, nf, fb, d_cnt, lev, level, st);
-        if (prof) phase_end(prof, P_BFS, 1);
-        HIP_TRY(hipMemcpyAsync(h_cnt, d_cnt, 4, hipMemcpyDeviceToHost, st));
+        for (uint32_t j = 0; j < chunk; j++) launch_bfs_level(keys, values, C, N, row_offset, b, level + j, thr, st);
+        if (prof) phase_end(prof, P_BFS, chunk);
+        HIP_TRY(hipMemcpyAsync(hcnt, b.cnt + level + 1, chunk * 8, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(hcnt + BFS_MAX_CHUNK + 1, b.cnt_big + level + 1, chunk * 8, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         if (prof) phase_collect(prof);
-        nf = *h_cnt;
-        reach += nf;
-        std::swap(fa, fb);
-        level++;
+        for (uint32_t j = 0; j < chunk && !done; j++) {
+            const uint64_t c = (uint64_t)(uint32_t)hcnt[j] + (uint32_t)hcnt[BFS_MAX_CHUNK + 1 + j];
+            if (c == 0) done = true;
+            reach += c;
+        }
+        level += chunk;
+        chunk = std::min<uint32_t>(chunk * 2, BFS_MAX_CHUNK);
+        if (level >= N + 1) done = true;
     }
     HIP_TRY(hipGetLastError());
-    if (reach_out) *reach_out = reach;
+    if (reach_out) *reach_out = (uint32_t)reach;
     return GPMA_OK;
 }
 
@@ -552,6 +618,7 @@ int gpma_create(uint32_t row_num, const gpma_config *cfg, gpma_t *out) {
         if (cfg->device >= 0) g->device = cfg->device;
         else hipGetDevice(&g->device);
         if (cfg->bulk_threshold) g->bulk_threshold = cfg->bulk_threshold;
+        if (cfg->pull_permille) g->pull_fraction = cfg->pull_permille / 1000.0;
     } else {
         hipGetDevice(&g->device);
     }
@@ -596,9 +663,11 @@ int gpma_destroy(gpma_t g) {
     Buf *all[] = {&g->keys, &g->values, &g->seps, &g->row_offset, &g->in_k, &g->in_v, &g->sk0, &g->sv0,
                   &g->sk1, &g->sv1, &g->win, &g->blkcnt, &g->blkoff, &g->sortwork, &g->pk0, &g->pv0,
                   &g->pw0, &g->pd0, &g->pk1, &g->pv1, &g->pw1, &g->pd1, &g->tmpk, &g->tmpv, &g->chunkcnt,
-                  &g->chunkoff, &g->lev, &g->fr0, &g->fr1, &g->ctr};
+                  &g->chunkoff, &g->lev, &g->fr0, &g->fr1, &g->fb0, &g->fb1, &g->vis, &g->bcnt, &g->bcntb,
+                  &g->ctr};
     for (Buf *b : all) b->release(&g->dev_bytes);
     if (g->hctr) (void)hipHostFree(g->hctr);
+    if (g->hcnt) (void)hipHostFree(g->hcnt);
     for (int p = 0; p < GPMA_PHASES; p++) {
         if (g->ev_a[p]) (void)hipEventDestroy(g->ev_a[p]);
         if (g->ev_b[p]) (void)hipEventDestroy(g->ev_b[p]);
@@ -619,9 +688,8 @@ int gpma_bfs(gpma_t g, uint32_t start, uint32_t *levels, int on_device, uint32_t
     HIP_TRY(hipSetDevice(g->device));
     TRY(ensure_bfs(g));
     uint32_t reach = 0;
-    TRY(run_bfs(g->keys.as<gkey>(), g->values.as<gval>(), g->row_offset.as<uint32_t>(), g->N, start,
-                g->lev.as<uint32_t>(), g->fr0.as<uint32_t>(), g->fr1.as<uint32_t>(), g->ctr.as<uint32_t>() + C_NEXT,
-                g->hctr + C_NEXT, g->st, &reach, g));
+    TRY(run_bfs(g->keys.as<gkey>(), g->values.as<gval>(), g->C, g->row_offset.as<uint32_t>(), g->N, start,
+                bfs_state(g), g->hcnt, g->st, &reach, g->pull_fraction, g));
     if (levels) {
         HIP_TRY(hipMemcpyAsync(levels, g->lev.p, (uint64_t)g->N * 4,
                                on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, g->st));
@@ -636,29 +704,46 @@ int gpma_csr_bfs(const uint64_t *keys, const uint8_t *values, const uint32_t *ro
     (void)edge_size;
     if (!keys || !values || !row_offsets || !results) return fail(GPMA_ERR_ARG, "gpma_csr_bfs: null pointer");
     if (start_node >= node_size) return fail(GPMA_ERR_ARG, "gpma_csr_bfs: start_node >= node_size");
-    uint32_t *buf = nullptr, *hc = nullptr;
+    // The caller's array length is unknown here, so only the push kernel runs.
     hipStream_t st = nullptr;
     HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    uint64_t n = (uint64_t)node_size + 1;
-    hipError_t e = hipMalloc((void **)&buf, (3 * n + 1) * 4);
-    if (e != hipSuccess) {
-        hipStreamDestroy(st);
-        return fail(GPMA_ERR_OOM, "gpma_csr_bfs: hipMalloc failed");
-    }
-    e = hipHostMalloc((void **)&hc, 4, hipHostMallocDefault);
+    const uint64_t n = (uint64_t)node_size + 1;
+    const uint64_t cw = n + BFS_MAX_CHUNK + 2;
+    const uint64_t words = 5 * n + (n / 32 + 1) + 4 * cw + 2;
+    uint32_t *buf = nullptr;
+    unsigned long long *hc = nullptr;
     int rc = GPMA_OK;
-    if (e != hipSuccess) rc = fail(GPMA_ERR_OOM, "gpma_csr_bfs: hipHostMalloc failed");
-    if (rc == GPMA_OK)
-        rc = run_bfs(keys, values, row_offsets, node_size, start_node, buf, buf + n, buf + 2 * n, buf + 3 * n, hc,
-                     st, nullptr);
+    if (hipMalloc((void **)&buf, words * 4) != hipSuccess) rc = fail(GPMA_ERR_OOM, "gpma_csr_bfs: hipMalloc failed");
+    if (rc == GPMA_OK && hipHostMalloc((void **)&hc, 2 * (BFS_MAX_CHUNK + 1) * 8, hipHostMallocDefault) != hipSuccess)
+        rc = fail(GPMA_ERR_OOM, "gpma_csr_bfs: hipHostMalloc failed");
     if (rc == GPMA_OK) {
-        if (hipMemcpyAsync(results, buf, (uint64_t)node_size * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
-            hipStreamSynchronize(st) != hipSuccess)
+        uint32_t *p = buf;
+        BfsState b;
+        b.levels = p;
+        p += n;
+        b.q[0] = p;
+        p += n;
+        b.q[1] = p;
+        p += n;
+        b.qbig[0] = p;
+        p += n;
+        b.qbig[1] = p;
+        p += n;
+        b.vis = p;
+        p += n / 32 + 1;
+        if ((uintptr_t)p & 7) p++;
+        b.cnt = (unsigned long long *)p;
+        p += 2 * cw;
+        b.cnt_big = (unsigned long long *)p;
+        rc = run_bfs(keys, values, 0, row_offsets, node_size, start_node, b, hc, st, nullptr, 0.0);
+        if (rc == GPMA_OK && (hipMemcpyAsync(results, b.levels, (uint64_t)node_size * 4, hipMemcpyDeviceToHost, st) !=
+                                  hipSuccess ||
+                              hipStreamSynchronize(st) != hipSuccess))
             rc = fail(GPMA_ERR_HIP, "gpma_csr_bfs: copy-out failed");
     }
-    hipFree(buf);
-    if (hc) hipHostFree(hc);
-    hipStreamDestroy(st);
+    if (buf) (void)hipFree(buf);
+    if (hc) (void)hipHostFree(hc);
+    (void)hipStreamDestroy(st);
     return rc;
 }
 

@@ -21,42 +21,48 @@ void fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t st);
 
 // ---- sort.hip
 uint64_t sort_status_words(uint64_t n, uint32_t npass);
-int radix_sort_pairs(gkey *k0, gval *v0, gkey *k1, gval *v1, uint64_t n, const SortShifts &shifts,
-                     uint32_t npass, uint32_t *work, uint32_t *d_err, hipStream_t st);
+// Validate + copy (ik, iv) into (k0, v0) (keys with src/dst >= N become
+// KEY_NONE and are counted in *invalid; with guards_ok dst == 0xFFFFFFFF is
+// accepted), then stable-sort by the digits at `shifts`. Returns which
+// buffer pair holds the result (0: k0/v0, 1: k1/v1).
+int stage_and_sort(const gkey *ik, const gval *iv, gkey *k0, gval *v0, gkey *k1, gval *v1, uint64_t n, uint32_t N,
+                   bool guards_ok, uint32_t *invalid, const SortShifts &shifts, uint32_t npass, uint32_t *work,
+                   uint32_t *d_err, hipStream_t st);
 
 // ---- update.hip
 constexpr uint32_t CMP_BLOCK = 256;  // items per compaction block
 constexpr uint32_t WIN_MAX = 4096;   // widest window rebalanced by one workgroup in LDS
 constexpr uint32_t CHUNK = 4096;     // slots per workgroup in device-wide window passes
 
-// Copy a batch into sort buffers; keys with src/dst >= N become KEY_NONE
-// (sorted last, skipped by locate) and are counted in *invalid. With
-// guards_ok, dst == 0xFFFFFFFF (a row guard) is accepted.
-void launch_stage(const gkey *ik, const gval *iv, gkey *ok, gval *ov, uint64_t n, uint32_t N, bool guards_ok,
-                  uint32_t *invalid, hipStream_t st);
 // Locate + in-place delete/modify. win[i] = leaf base or SIZE_NONE; per-block
 // pending counts into blkcnt[ceil(n/CMP_BLOCK)].
 void launch_locate(const gkey *ks, const gval *vs, uint64_t n, gkey *keys, gval *values, const gkey *seps,
                    uint64_t nseg, uint32_t S, uint32_t *win, uint32_t *blkcnt, hipStream_t st);
-// Exclusive scan of nb block counts (one workgroup); *total gets the sum.
-void launch_scan_blocks(const uint32_t *cnt, uint64_t nb, uint32_t *off, uint32_t *total, hipStream_t st);
-// Stable compaction of entries whose win != SIZE_NONE (done == nullptr) or
-// whose done flag is 0, writing win & mask. Entries at or beyond *d_n (or
-// n_ub when d_n is null) are ignored. Uses off[] from the block scan.
+// Exclusive scan of per-chunk counts (one workgroup); *total gets the sum.
+// The chunk count is nb, or ceil(*d_items / CMP_BLOCK) when d_items is given.
+void launch_scan_blocks(const uint32_t *cnt, uint64_t nb, uint32_t *off, uint32_t *total, hipStream_t st,
+                        const uint32_t *d_items = nullptr);
+// Stable compaction of entries whose win != SIZE_NONE (idone == nullptr) or
+// whose done flag is 0, writing win & mask and clearing odone. Entries at or
+// beyond *d_n (or n_ub when d_n is null) are ignored; grid-stride over chunks.
 void launch_compact(const gkey *ik, const gval *iv, const uint32_t *iw, const uint8_t *idone, uint64_t n_ub,
-                    const uint32_t *d_n, const uint32_t *off, gkey *ok, gval *ov, uint32_t *ow, uint32_t mask,
-                    hipStream_t st);
-// Per-block counts of entries with done == 0 among the first *d_n.
+                    const uint32_t *d_n, const uint32_t *off, gkey *ok, gval *ov, uint32_t *ow, uint8_t *odone,
+                    uint32_t mask, hipStream_t st);
+// Per-chunk counts of entries with done == 0 among the first *d_n.
 void launch_count_pending(const uint8_t *done, uint64_t n_ub, const uint32_t *d_n, uint32_t *blkcnt,
                           hipStream_t st);
-// Level-0 rebalance: one thread per leaf run.
+// Level-0 rebalance: one thread per leaf run; does nothing when *d_np >=
+// bulk_threshold (that batch takes the bulk path).
 void launch_leaf(uint32_t S, const gkey *pk, const gval *pv, const uint32_t *pw, uint8_t *pd, uint64_t n_ub,
                  const uint32_t *d_np, gkey *keys, gval *values, gkey *seps, uint32_t *row_offset, uint32_t lower,
-                 uint32_t upper, hipStream_t st);
-// Levels whose window fits in LDS: one workgroup per window run.
+                 uint32_t upper, uint32_t bulk_threshold, hipStream_t st);
+// Levels whose window fits in LDS: workgroups stride over the first *d_np
+// pending entries, one window per run head. Does nothing when d_gate is set
+// and *d_gate >= gate (that batch takes the bulk path).
 void launch_window(uint32_t S, uint32_t W, const gkey *pk, const gval *pv, const uint32_t *pw, uint8_t *pd,
                    uint64_t n_ub, const uint32_t *d_np, gkey *keys, gval *values, gkey *seps,
-                   uint32_t *row_offset, uint32_t lower, uint32_t upper, hipStream_t st);
+                   uint32_t *row_offset, uint32_t lower, uint32_t upper, const uint32_t *d_gate, uint32_t gate,
+                   hipStream_t st);
 // Run heads of the pending list: heads[j] = index of the j-th run start.
 void launch_run_heads(const uint32_t *pw, uint64_t n_ub, const uint32_t *d_np, uint32_t *flags_cnt,
                       uint32_t *off, uint32_t *heads, uint32_t *d_nruns, hipStream_t st);
@@ -77,9 +83,20 @@ void launch_write_compacted(const gkey *tk, const gval *tv, uint64_t c, gkey *ke
                             gkey *seps, uint32_t *row_offset, uint32_t S, hipStream_t st);
 
 // ---- bfs.hip
-void launch_bfs_init(uint32_t *levels, uint32_t N, uint32_t start, uint32_t *frontier, hipStream_t st);
-void launch_bfs_expand(const gkey *keys, const gval *values, const uint32_t *row_offset, const uint32_t *front,
-                       uint32_t nf, uint32_t *next, uint32_t *next_cnt, uint32_t *levels, uint32_t level,
-                       hipStream_t st);
+// Device state of one BFS: levels[N], visited bitmap, two frontier queues per
+// parity (short ranges / long ranges) and per-level counts and slot work.
+struct BfsState {
+    uint32_t *levels, *vis;
+    uint32_t *q[2], *qbig[2];
+    // per level: low 32 bits = queued vertices, high 32 bits = their slot count
+    unsigned long long *cnt, *cnt_big;
+};
+// levels = 0 except start (1); bitmap cleared except start; start queued for level 1.
+void launch_bfs_init(const BfsState &b, uint32_t N, uint32_t start, const uint32_t *row_offset, hipStream_t st);
+void launch_bfs_clear(const BfsState &b, uint32_t a, uint32_t e, hipStream_t st);
+// Expand level `level` into level+1: push when the level's slot count is at
+// most pull_threshold, else stream all C slots (pull; C = 0 disables it).
+void launch_bfs_level(const gkey *keys, const gval *values, uint64_t C, uint32_t N, const uint32_t *row_offset,
+                      const BfsState &b, uint32_t level, unsigned long long pull_threshold, hipStream_t st);
 
 }  // namespace gpma

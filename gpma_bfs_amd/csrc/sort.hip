@@ -6,14 +6,17 @@
 // key has src, dst < row_num, so bits above the row-number width are zero in
 // both 32-bit halves; the caller passes that digit list.
 //
-// One launch per digit pass ("onesweep", Merrill & Garland / Adinets &
-// Merrill 2022): each 2048-item tile ranks its items per digit with wave64
+// Pass 0 (k_stage_hist) copies the batch into the sort buffer, replaces
+// out-of-range keys by KEY_NONE (counted; it sorts after every valid key),
+// and builds the digit histograms of every pass from one read.
+// Then one launch per digit pass ("onesweep", Merrill & Garland / Adinets &
+// Merrill 2022): each 4096-item tile ranks its items per digit with wave64
 // ballots, publishes its per-digit counts, looks back over earlier tiles'
 // published counts to find its global offsets, and scatters. Tiles take
-// tickets in launch order, and a tile publishes its own counts before it
-// waits on anyone, so every wait is on a tile that is already running and
-// will publish without waiting; the wait is also bounded by a spin limit
-// that raises an error flag instead of hanging.
+// tickets in launch order and publish their own counts before they wait on
+// anyone, so every wait is on a tile that is already running and will publish
+// without waiting; the look-back reads 16 predecessors per round trip, and is
+// bounded by a spin limit that raises an error flag instead of hanging.
 #include "common.h"
 #include "kernels.h"
 
@@ -23,28 +26,44 @@ namespace {
 
 constexpr uint32_t SORT_THREADS = 256;
 constexpr uint32_t SORT_WAVES = SORT_THREADS / WAVE;
-constexpr uint32_t SORT_IPT = 8;                       // items per lane
-constexpr uint32_t WAVE_ITEMS = WAVE * SORT_IPT;       // 512
-constexpr uint32_t TILE = SORT_THREADS * SORT_IPT;     // 2048
+constexpr uint32_t SORT_IPT = 16;                      // items per lane
+constexpr uint32_t WAVE_ITEMS = WAVE * SORT_IPT;       // 1024
+constexpr uint32_t TILE = SORT_THREADS * SORT_IPT;     // 4096
+constexpr uint32_t LOOKBACK = 16;                      // predecessors read per round trip
 constexpr uint32_t FLAG_AGG = 1u << 30;
 constexpr uint32_t FLAG_INC = 2u << 30;
 constexpr uint32_t VAL_MASK = (1u << 30) - 1;
 constexpr uint32_t SPIN_LIMIT = 1u << 22;
+constexpr uint32_t HIST_GRID_MAX = 128;
 
-// Global digit histograms for every pass at once; also clears the
-// look-back status words and tile tickets for the passes that follow.
-__global__ __launch_bounds__(SORT_THREADS) void k_sort_hist(const gkey *__restrict__ keys, uint64_t n,
-                                                            SortShifts shifts, uint32_t npass,
-                                                            uint32_t *__restrict__ hist, uint32_t *__restrict__ status,
-                                                            uint64_t nstatus) {
+// Validate + copy the batch and build every pass's digit histogram at once;
+// also clears the look-back status words of the passes.
+__global__ __launch_bounds__(SORT_THREADS) void k_stage_hist(const gkey *__restrict__ ik, const gval *__restrict__ iv,
+                                                             gkey *__restrict__ ok, gval *__restrict__ ov,
+                                                             uint64_t n, uint32_t N, uint32_t guards_ok,
+                                                             uint32_t *__restrict__ invalid, SortShifts shifts,
+                                                             uint32_t npass, uint32_t *__restrict__ hist,
+                                                             uint32_t *__restrict__ status, uint64_t nstatus) {
     __shared__ uint32_t h[8][256];
     for (uint32_t i = threadIdx.x; i < 8 * 256; i += SORT_THREADS) (&h[0][0])[i] = 0;
     __syncthreads();
     const uint64_t stride = (uint64_t)gridDim.x * SORT_THREADS;
-    for (uint64_t i = (uint64_t)blockIdx.x * SORT_THREADS + threadIdx.x; i < n; i += stride) {
-        gkey k = keys[i];
-        for (uint32_t p = 0; p < npass; p++) atomicAdd(&h[p][(uint32_t)(k >> shifts.s[p]) & 0xFF], 1u);
+    uint32_t nbad = 0;
+    for (uint64_t i0 = (uint64_t)blockIdx.x * SORT_THREADS; i0 < n; i0 += stride) {
+        const uint64_t i = i0 + threadIdx.x;
+        bool bad = false;
+        if (i < n) {
+            gkey k = ik[i];
+            const uint32_t dst = (uint32_t)k;
+            bad = (uint32_t)(k >> 32) >= N || (dst >= N && !(guards_ok && dst == COL_NONE));
+            if (bad) k = KEY_NONE;
+            ok[i] = k;
+            ov[i] = iv[i];
+            for (uint32_t p = 0; p < npass; p++) atomicAdd(&h[p][(uint32_t)(k >> shifts.s[p]) & 0xFF], 1u);
+        }
+        nbad += (uint32_t)__popcll(ballot(bad));
     }
+    if (lane_id() == 0 && nbad) atomicAdd(invalid, nbad);
     for (uint64_t i = (uint64_t)blockIdx.x * SORT_THREADS + threadIdx.x; i < nstatus; i += stride) status[i] = 0;
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < npass * 256; i += SORT_THREADS) {
@@ -115,31 +134,41 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(const gkey *__restri
         cnt[w][d] = tot;
         tot += c;
     }
-    // Publish this tile's aggregate, then look back.
+    // Publish this tile's aggregate, then look back LOOKBACK tiles per round.
     uint32_t *my = status + (uint64_t)tile * 256 + d;
-    if (tile == 0) {
-        __hip_atomic_store(my, FLAG_INC | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-        __hip_atomic_store(my, FLAG_AGG | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    __hip_atomic_store(my, (tile == 0 ? FLAG_INC : FLAG_AGG) | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t excl = 0;
     if (tile > 0) {
         int64_t t = (int64_t)tile - 1;
         uint32_t spins = 0;
         while (t >= 0) {
-            uint32_t s = __hip_atomic_load(status + (uint64_t)t * 256 + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            uint32_t flag = s & ~VAL_MASK;
-            if (flag == 0) {
+            uint32_t s[LOOKBACK];
+#pragma unroll
+            for (uint32_t j = 0; j < LOOKBACK; j++)
+                s[j] = (t - (int64_t)j >= 0)
+                           ? __hip_atomic_load(status + (uint64_t)(t - j) * 256 + d, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT)
+                           : FLAG_INC;
+            uint32_t used = 0;
+            bool stop = false;
+#pragma unroll
+            for (uint32_t j = 0; j < LOOKBACK; j++) {
+                if (stop || used != j) break;
+                const uint32_t flag = s[j] & ~VAL_MASK;
+                if (flag == 0) break;  // not published yet: poll again from here
+                excl += s[j] & VAL_MASK;
+                used++;
+                if (flag == FLAG_INC) stop = true;
+            }
+            if (stop) break;
+            if (used == 0) {
                 if (++spins > SPIN_LIMIT) {
                     atomicOr(err, 1u);
                     break;
                 }
                 __builtin_amdgcn_s_sleep(1);
-                continue;
             }
-            excl += s & VAL_MASK;
-            if (flag == FLAG_INC) break;
-            t--;
+            t -= used;
         }
         __hip_atomic_store(my, FLAG_INC | (excl + tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -165,21 +194,24 @@ uint64_t sort_status_words(uint64_t n, uint32_t npass) {
     return tiles * 256 * npass;
 }
 
-// Sorts (k0, v0) by the byte digits at `shifts` (LSD first), ping-ponging
-// with (k1, v1). Returns 0 if the result is in (k0, v0), 1 if in (k1, v1).
-// work: hist [npass*256] + tickets [npass] + err [1] + status words.
-int radix_sort_pairs(gkey *k0, gval *v0, gkey *k1, gval *v1, uint64_t n, const SortShifts &shifts,
-                     uint32_t npass, uint32_t *work, uint32_t *d_err, hipStream_t st) {
-    if (n == 0 || npass == 0) return 0;
+// Stages (ik, iv) into (k0, v0) and sorts them by the byte digits at
+// `shifts` (LSD first), ping-ponging with (k1, v1). Returns 0 if the result
+// is in (k0, v0), 1 if in (k1, v1).
+// work: hist [npass*256] + tickets [npass] + status words.
+int stage_and_sort(const gkey *ik, const gval *iv, gkey *k0, gval *v0, gkey *k1, gval *v1, uint64_t n, uint32_t N,
+                   bool guards_ok, uint32_t *invalid, const SortShifts &shifts, uint32_t npass, uint32_t *work,
+                   uint32_t *d_err, hipStream_t st) {
+    if (n == 0) return 0;
     uint32_t *hist = work;
     uint32_t *tickets = hist + npass * 256;
     uint32_t *status = tickets + npass;
-    uint64_t tiles = (n + TILE - 1) / TILE;
-    uint64_t nstatus = tiles * 256 * npass;
+    const uint64_t tiles = (n + TILE - 1) / TILE;
+    const uint64_t nstatus = tiles * 256 * npass;
     fill_u32(hist, 0, npass * 256 + npass, st);
-    uint32_t hb = (uint32_t)std::min<uint64_t>(1024, (std::max<uint64_t>(n, nstatus) + SORT_THREADS - 1) / SORT_THREADS);
-    if (hb == 0) hb = 1;
-    hipLaunchKernelGGL(k_sort_hist, dim3(hb), dim3(SORT_THREADS), 0, st, k0, n, shifts, npass, hist, status, nstatus);
+    uint64_t want = (std::max<uint64_t>(n, nstatus) + SORT_THREADS * 8 - 1) / (SORT_THREADS * 8);
+    uint32_t hb = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(HIST_GRID_MAX, want));
+    hipLaunchKernelGGL(k_stage_hist, dim3(hb), dim3(SORT_THREADS), 0, st, ik, iv, k0, v0, n, N, guards_ok ? 1u : 0u,
+                       invalid, shifts, npass, hist, status, nstatus);
     int cur = 0;
     for (uint32_t p = 0; p < npass; p++) {
         gkey *ki = cur ? k1 : k0;

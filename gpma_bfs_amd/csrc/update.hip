@@ -44,22 +44,6 @@ __device__ __forceinline__ void wave_count(uint32_t *ctr, bool p) {
     if (m && lane_id() == (uint32_t)__ffsll((unsigned long long)m) - 1) atomicAdd(ctr, (uint32_t)__popcll(m));
 }
 
-__global__ __launch_bounds__(256) void k_stage(const gkey *__restrict__ ik, const gval *__restrict__ iv,
-                                               gkey *__restrict__ ok, gval *__restrict__ ov, uint64_t n,
-                                               uint32_t N, uint32_t guards_ok, uint32_t *invalid) {
-    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i - threadIdx.x < n; i += (uint64_t)gridDim.x * 256) {
-        bool bad = false;
-        if (i < n) {
-            gkey k = ik[i];
-            const uint32_t dst = (uint32_t)k;
-            bad = (uint32_t)(k >> 32) >= N || (dst >= N && !(guards_ok && dst == COL_NONE));
-            ok[i] = bad ? KEY_NONE : k;
-            ov[i] = iv[i];
-        }
-        wave_count(invalid, bad);
-    }
-}
-
 // One update per thread (gpma.cuh:204-226). Sorted input, so the last of
 // equal keys is the one applied (earlier duplicates are skipped).
 __global__ __launch_bounds__(CMP_BLOCK) void k_locate(const gkey *__restrict__ ks, const gval *__restrict__ vs,
@@ -104,9 +88,13 @@ __global__ __launch_bounds__(CMP_BLOCK) void k_locate(const gkey *__restrict__ k
     }
 }
 
+// Exclusive scan of per-chunk counts (one workgroup). The number of chunks
+// is nb, or ceil(*d_items / CMP_BLOCK) when d_items is given.
 __global__ __launch_bounds__(1024) void k_scan_blocks(const uint32_t *__restrict__ cnt, uint64_t nb,
+                                                      const uint32_t *__restrict__ d_items,
                                                       uint32_t *__restrict__ off, uint32_t *__restrict__ total) {
     __shared__ uint32_t scratch[1024 / WAVE + 1];
+    if (d_items) nb = ((uint64_t)*d_items + CMP_BLOCK - 1) / CMP_BLOCK;
     const uint64_t per = (nb + 1023) / 1024;
     const uint64_t a = std::min<uint64_t>(nb, threadIdx.x * per), b = std::min<uint64_t>(nb, a + per);
     uint32_t s = 0;
@@ -121,42 +109,53 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(const uint32_t *__restrict
     if (threadIdx.x == 0) *total = tot;
 }
 
-__global__ __launch_bounds__(CMP_BLOCK) void k_count_pending(const uint8_t *__restrict__ done, uint64_t n_ub,
+// Per-chunk counts of pending (done == 0) entries among the first *d_n;
+// grid-stride over chunks of CMP_BLOCK entries.
+__global__ __launch_bounds__(CMP_BLOCK) void k_count_pending(const uint8_t *__restrict__ done,
                                                              const uint32_t *__restrict__ d_n,
                                                              uint32_t *__restrict__ blkcnt) {
     __shared__ uint32_t wcnt[CMP_BLOCK / WAVE];
-    const uint64_t n = d_n ? *d_n : n_ub;
-    const uint64_t i = (uint64_t)blockIdx.x * CMP_BLOCK + threadIdx.x;
-    bool keep = i < n && done[i] == 0;
-    uint64_t m = ballot(keep);
-    if (lane_id() == 0) wcnt[threadIdx.x / WAVE] = (uint32_t)__popcll(m);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t s = 0;
-        for (uint32_t w = 0; w < CMP_BLOCK / WAVE; w++) s += wcnt[w];
-        blkcnt[blockIdx.x] = s;
+    const uint64_t n = *d_n;
+    for (uint64_t c = blockIdx.x; c * CMP_BLOCK < n; c += gridDim.x) {
+        const uint64_t i = c * CMP_BLOCK + threadIdx.x;
+        bool keep = i < n && done[i] == 0;
+        uint64_t m = ballot(keep);
+        if (lane_id() == 0) wcnt[threadIdx.x / WAVE] = (uint32_t)__popcll(m);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t t = 0;
+            for (uint32_t w = 0; w < CMP_BLOCK / WAVE; w++) t += wcnt[w];
+            blkcnt[c] = t;
+        }
+        __syncthreads();
     }
 }
 
+// Stable compaction of entries whose win != SIZE_NONE (idone == nullptr) or
+// whose done flag is 0, over the first *d_n entries (n_ub when d_n is null),
+// grid-stride over chunks; writes win & mask and clears the output's done flags.
 __global__ __launch_bounds__(CMP_BLOCK) void k_compact(const gkey *__restrict__ ik, const gval *__restrict__ iv,
                                                        const uint32_t *__restrict__ iw,
                                                        const uint8_t *__restrict__ idone, uint64_t n_ub,
                                                        const uint32_t *__restrict__ d_n,
                                                        const uint32_t *__restrict__ off, gkey *__restrict__ ok,
                                                        gval *__restrict__ ov, uint32_t *__restrict__ ow,
-                                                       uint32_t mask) {
+                                                       uint8_t *__restrict__ odone, uint32_t mask) {
     __shared__ uint32_t scratch[CMP_BLOCK / WAVE + 1];
     const uint64_t n = d_n ? *d_n : n_ub;
-    const uint64_t i = (uint64_t)blockIdx.x * CMP_BLOCK + threadIdx.x;
-    bool keep = false;
-    if (i < n) keep = idone ? (idone[i] == 0) : (iw[i] != SIZE_NONE);
-    uint32_t tot;
-    uint32_t pre = block_exclusive_scan(keep ? 1u : 0u, scratch, &tot);
-    if (keep) {
-        uint64_t o = (uint64_t)off[blockIdx.x] + pre;
-        ok[o] = ik[i];
-        ov[o] = iv[i];
-        ow[o] = iw[i] & mask;
+    for (uint64_t c = blockIdx.x; c * CMP_BLOCK < n; c += gridDim.x) {
+        const uint64_t i = c * CMP_BLOCK + threadIdx.x;
+        bool keep = false;
+        if (i < n) keep = idone ? (idone[i] == 0) : (iw[i] != SIZE_NONE);
+        uint32_t tot;
+        uint32_t pre = block_exclusive_scan(keep ? 1u : 0u, scratch, &tot);
+        if (keep) {
+            uint64_t o = (uint64_t)off[c] + pre;
+            ok[o] = ik[i];
+            ov[o] = iv[i];
+            ow[o] = iw[i] & mask;
+            odone[o] = 0;
+        }
     }
 }
 
@@ -168,10 +167,12 @@ __global__ __launch_bounds__(256) void k_leaf(const gkey *__restrict__ pk, const
                                               const uint32_t *__restrict__ pw, uint8_t *__restrict__ pd,
                                               const uint32_t *__restrict__ d_np, gkey *__restrict__ keys,
                                               gval *__restrict__ values, gkey *__restrict__ seps,
-                                              uint32_t *__restrict__ row_offset, uint32_t lower, uint32_t upper) {
+                                              uint32_t *__restrict__ row_offset, uint32_t lower, uint32_t upper,
+                                              uint32_t bulk_threshold) {
     const uint64_t np = *d_np;
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= np) return;
+    // a batch with this many pending inserts takes the bulk path instead
+    if (i >= np || np >= bulk_threshold) return;
     const uint32_t w = pw[i];
     if (i > 0 && pw[i - 1] == w) return;
     uint32_t r = 1;
@@ -244,25 +245,17 @@ __device__ __forceinline__ uint64_t lower_bound_g(const gkey *a, uint64_t n, gke
     return lo;
 }
 
-// One workgroup per window run for S < W <= WIN_MAX: compact the window's
-// live items into LDS, check the level's bounds, and write merged items
-// straight to their redistributed slots.
-__global__ __launch_bounds__(256) void k_window(uint32_t S, uint32_t W, const gkey *__restrict__ pk,
-                                                const gval *__restrict__ pv, const uint32_t *__restrict__ pw,
-                                                uint8_t *__restrict__ pd, const uint32_t *__restrict__ d_np,
-                                                gkey *__restrict__ keys, gval *__restrict__ values,
-                                                gkey *__restrict__ seps, uint32_t *__restrict__ row_offset,
-                                                uint32_t lower, uint32_t upper) {
-    __shared__ gkey lk[WIN_MAX];
-    __shared__ gval lv[WIN_MAX];
-    __shared__ uint32_t scratch[256 / WAVE + 1];
-    __shared__ uint32_t s_r;
-    const uint64_t np = *d_np;
-    const uint64_t i = blockIdx.x;
-    if (i >= np) return;
-    const uint32_t w = pw[i];
-    if (i > 0 && pw[i - 1] == w) return;
+// One window run for S < W <= WIN_MAX, by one workgroup: compact the
+// window's live items into LDS, check the level's bounds, and write merged
+// items straight to their redistributed slots (run = entries i.. with base w).
+__device__ void window_one(uint32_t S, uint32_t W, const gkey *__restrict__ pk, const gval *__restrict__ pv,
+                           const uint32_t *__restrict__ pw, uint8_t *__restrict__ pd, uint64_t np, uint64_t i,
+                           uint32_t w, gkey *__restrict__ keys, gval *__restrict__ values, gkey *__restrict__ seps,
+                           uint32_t *__restrict__ row_offset, uint32_t lower, uint32_t upper, gkey *lk, gval *lv,
+                           uint32_t *scratch, uint32_t *s_r_ptr) {
+    uint32_t &s_r = *s_r_ptr;
     const uint32_t tid = threadIdx.x;
+    __syncthreads();  // the previous run of this workgroup is done with LDS
     if (tid == 0) s_r = upper + 1;
     __syncthreads();
     for (uint32_t t = tid; t <= upper; t += 256) {
@@ -329,6 +322,30 @@ __global__ __launch_bounds__(256) void k_window(uint32_t S, uint32_t W, const gk
             values[(uint64_t)w + p] = 0;
             if (l == 0) seps[(w + p) / S] = KEY_NONE;
         }
+    }
+}
+
+
+// Workgroups stride over the pending list; each run head is one window.
+__global__ __launch_bounds__(256) void k_window(uint32_t S, uint32_t W, const gkey *__restrict__ pk,
+                                                const gval *__restrict__ pv, const uint32_t *__restrict__ pw,
+                                                uint8_t *__restrict__ pd, const uint32_t *__restrict__ d_np,
+                                                gkey *__restrict__ keys, gval *__restrict__ values,
+                                                gkey *__restrict__ seps, uint32_t *__restrict__ row_offset,
+                                                uint32_t lower, uint32_t upper, const uint32_t *__restrict__ d_gate,
+                                                uint32_t gate) {
+    __shared__ gkey lk[WIN_MAX];
+    __shared__ gval lv[WIN_MAX];
+    __shared__ uint32_t scratch[256 / WAVE + 1];
+    __shared__ uint32_t s_r;
+    // a batch whose level-0 pending count reached the bulk threshold takes the bulk path
+    if (d_gate && *d_gate >= gate) return;
+    const uint64_t np = *d_np;
+    for (uint64_t i = blockIdx.x; i < np; i += gridDim.x) {
+        const uint32_t w = pw[i];
+        if (i > 0 && pw[i - 1] == w) continue;
+        window_one(S, W, pk, pv, pw, pd, np, i, w, keys, values, seps, row_offset, lower, upper, lk, lv, scratch,
+                   &s_r);
     }
 }
 
@@ -483,13 +500,6 @@ void fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256, 2048)), dim3(256), 0, st, p, v, n);
 }
 
-void launch_stage(const gkey *ik, const gval *iv, gkey *ok, gval *ov, uint64_t n, uint32_t N, bool guards_ok,
-                  uint32_t *invalid, hipStream_t st) {
-    if (!n) return;
-    hipLaunchKernelGGL(k_stage, dim3(grid_for(n, 256)), dim3(256), 0, st, ik, iv, ok, ov, n, N,
-                       guards_ok ? 1u : 0u, invalid);
-}
-
 void launch_locate(const gkey *ks, const gval *vs, uint64_t n, gkey *keys, gval *values, const gkey *seps,
                    uint64_t nseg, uint32_t S, uint32_t *win, uint32_t *blkcnt, hipStream_t st) {
     if (!n) return;
@@ -498,46 +508,53 @@ void launch_locate(const gkey *ks, const gval *vs, uint64_t n, gkey *keys, gval 
                        win, blkcnt);
 }
 
-void launch_scan_blocks(const uint32_t *cnt, uint64_t nb, uint32_t *off, uint32_t *total, hipStream_t st) {
-    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, cnt, nb, off, total);
+void launch_scan_blocks(const uint32_t *cnt, uint64_t nb, uint32_t *off, uint32_t *total, hipStream_t st,
+                        const uint32_t *d_items) {
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, cnt, nb, d_items, off, total);
 }
 
 void launch_compact(const gkey *ik, const gval *iv, const uint32_t *iw, const uint8_t *idone, uint64_t n_ub,
-                    const uint32_t *d_n, const uint32_t *off, gkey *ok, gval *ov, uint32_t *ow, uint32_t mask,
-                    hipStream_t st) {
+                    const uint32_t *d_n, const uint32_t *off, gkey *ok, gval *ov, uint32_t *ow, uint8_t *odone,
+                    uint32_t mask, hipStream_t st) {
     if (!n_ub) return;
     uint64_t nb = (n_ub + CMP_BLOCK - 1) / CMP_BLOCK;
-    hipLaunchKernelGGL(k_compact, dim3((uint32_t)nb), dim3(CMP_BLOCK), 0, st, ik, iv, iw, idone, n_ub, d_n, off, ok,
-                       ov, ow, mask);
+    hipLaunchKernelGGL(k_compact, dim3((uint32_t)std::min<uint64_t>(nb, 2048)), dim3(CMP_BLOCK), 0, st, ik, iv, iw,
+                       idone, n_ub, d_n, off, ok, ov, ow, odone, mask);
 }
 
 void launch_count_pending(const uint8_t *done, uint64_t n_ub, const uint32_t *d_n, uint32_t *blkcnt,
                           hipStream_t st) {
     if (!n_ub) return;
     uint64_t nb = (n_ub + CMP_BLOCK - 1) / CMP_BLOCK;
-    hipLaunchKernelGGL(k_count_pending, dim3((uint32_t)nb), dim3(CMP_BLOCK), 0, st, done, n_ub, d_n, blkcnt);
+    hipLaunchKernelGGL(k_count_pending, dim3((uint32_t)std::min<uint64_t>(nb, 2048)), dim3(CMP_BLOCK), 0, st, done,
+                       d_n, blkcnt);
 }
 
 void launch_leaf(uint32_t S, const gkey *pk, const gval *pv, const uint32_t *pw, uint8_t *pd, uint64_t n_ub,
                  const uint32_t *d_np, gkey *keys, gval *values, gkey *seps, uint32_t *row_offset, uint32_t lower,
-                 uint32_t upper, hipStream_t st) {
+                 uint32_t upper, uint32_t bulk_threshold, hipStream_t st) {
     if (!n_ub) return;
     dim3 g((uint32_t)((n_ub + 255) / 256)), b(256);
+#define GPMA_LEAF(SS)                                                                                            \
+    hipLaunchKernelGGL(k_leaf<SS>, g, b, 0, st, pk, pv, pw, pd, d_np, keys, values, seps, row_offset, lower, upper, \
+                       bulk_threshold)
     switch (S) {
-    case 2: hipLaunchKernelGGL(k_leaf<2>, g, b, 0, st, pk, pv, pw, pd, d_np, keys, values, seps, row_offset, lower, upper); break;
-    case 4: hipLaunchKernelGGL(k_leaf<4>, g, b, 0, st, pk, pv, pw, pd, d_np, keys, values, seps, row_offset, lower, upper); break;
-    case 8: hipLaunchKernelGGL(k_leaf<8>, g, b, 0, st, pk, pv, pw, pd, d_np, keys, values, seps, row_offset, lower, upper); break;
-    case 16: hipLaunchKernelGGL(k_leaf<16>, g, b, 0, st, pk, pv, pw, pd, d_np, keys, values, seps, row_offset, lower, upper); break;
-    default: hipLaunchKernelGGL(k_leaf<32>, g, b, 0, st, pk, pv, pw, pd, d_np, keys, values, seps, row_offset, lower, upper); break;
+    case 2: GPMA_LEAF(2); break;
+    case 4: GPMA_LEAF(4); break;
+    case 8: GPMA_LEAF(8); break;
+    case 16: GPMA_LEAF(16); break;
+    default: GPMA_LEAF(32); break;
     }
+#undef GPMA_LEAF
 }
 
 void launch_window(uint32_t S, uint32_t W, const gkey *pk, const gval *pv, const uint32_t *pw, uint8_t *pd,
                    uint64_t n_ub, const uint32_t *d_np, gkey *keys, gval *values, gkey *seps,
-                   uint32_t *row_offset, uint32_t lower, uint32_t upper, hipStream_t st) {
+                   uint32_t *row_offset, uint32_t lower, uint32_t upper, const uint32_t *d_gate, uint32_t gate,
+                   hipStream_t st) {
     if (!n_ub) return;
-    hipLaunchKernelGGL(k_window, dim3((uint32_t)n_ub), dim3(256), 0, st, S, W, pk, pv, pw, pd, d_np, keys, values,
-                       seps, row_offset, lower, upper);
+    hipLaunchKernelGGL(k_window, dim3((uint32_t)std::min<uint64_t>(n_ub, 1024)), dim3(256), 0, st, S, W, pk, pv, pw,
+                       pd, d_np, keys, values, seps, row_offset, lower, upper, d_gate, gate);
 }
 
 void launch_run_heads(const uint32_t *pw, uint64_t n_ub, const uint32_t *d_np, uint32_t *blkcnt, uint32_t *off,
@@ -545,7 +562,7 @@ void launch_run_heads(const uint32_t *pw, uint64_t n_ub, const uint32_t *d_np, u
     if (!n_ub) return;
     uint64_t nb = (n_ub + CMP_BLOCK - 1) / CMP_BLOCK;
     hipLaunchKernelGGL(k_head_count, dim3((uint32_t)nb), dim3(CMP_BLOCK), 0, st, pw, n_ub, d_np, blkcnt);
-    launch_scan_blocks(blkcnt, nb, off, d_nruns, st);
+    launch_scan_blocks(blkcnt, nb, off, d_nruns, st, nullptr);
     hipLaunchKernelGGL(k_head_scatter, dim3((uint32_t)nb), dim3(CMP_BLOCK), 0, st, pw, n_ub, d_np, off, heads);
 }
 

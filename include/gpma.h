@@ -43,7 +43,9 @@ typedef struct gpma_config {
     uint32_t struct_size;      /* sizeof(gpma_config) */
     int32_t device;            /* HIP device ordinal, -1 = current device */
     uint32_t bulk_threshold;   /* pending inserts that take the bulk path (reference: 5,000,000) */
-    uint32_t reserved[5];
+    uint32_t pull_permille;    /* BFS streams the whole array when the frontier's slots exceed
+                                  this many thousandths of the capacity (0 = 250) */
+    uint32_t reserved[4];
 } gpma_config;
 
 /* State summary. Pointers are device pointers owned by the handle and are

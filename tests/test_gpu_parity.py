@@ -168,3 +168,32 @@ def test_bulk_path_matches_oracle():
     step = half // 10
     olv, oreach = O.bfs_edges(N, src[10 * step: half + 10 * step], dst[10 * step: half + 10 * step], 0)
     assert reach == oreach and np.array_equal(lv, olv)
+
+
+@pytest.mark.parametrize("pull_permille", [0, 1, 1000])
+def test_bfs_hubs_and_direction_modes(pull_permille):
+    """Hub vertices (ranges far above the 2048-slot split) and both expansion
+    modes: 1 forces the streaming pull step at every level, 1000 disables it."""
+    N = 20000
+    rng = np.random.default_rng(17)
+    hubs = np.array([0, 5, 77], np.uint32)
+    src = [np.full(9000, h, np.uint32) for h in hubs]
+    dst = [rng.choice(N, 9000, replace=False).astype(np.uint32) for _ in hubs]
+    s2, d2 = G.gen_rmat(15, 60000, seed=23)
+    m = (s2 < N) & (d2 < N)
+    src = np.concatenate(src + [s2[m]])
+    dst = np.concatenate(dst + [d2[m]])
+    keys = np.unique(G.make_keys(src, dst))
+    keys = keys[(keys >> np.uint64(32)) != (keys & np.uint64(0xFFFFFFFF))]
+    src = (keys >> np.uint64(32)).astype(np.uint32)
+    dst = (keys & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    g = G.GPMA(N, pull_permille=pull_permille)
+    G.update_gpma(g, keys, np.ones(len(keys), np.uint8))
+    # tombstone some edges so skipped slots are exercised
+    G.update_gpma(g, keys[::7], np.zeros(len(keys[::7]), np.uint8))
+    keep = np.ones(len(keys), bool)
+    keep[::7] = False
+    for start in (0, 5, 1234, N - 1):
+        lv, reach = G.gpma_bfs(g, start)
+        olv, oreach = O.bfs_edges(N, src[keep], dst[keep], start)
+        assert reach == oreach and np.array_equal(lv, olv), f"start {start}"
