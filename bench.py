@@ -64,6 +64,118 @@ def bfs_bytes(N, outdeg, levels):
     return 16 * int(reached.sum()) + 13 * E_t + 4 * N, E_t
 
 
+def ms_bfs_bytes(N, outdeg, lv_all, nsrc):
+    """Multi-source algorithmic bytes: a vertex is expanded once per distinct
+    level at which any source reaches it (16 B of offsets and masks, then 9 B
+    slot + 8 B neighbour mask per edge), plus the u32 level outputs. Also
+    returns the aggregate edges traversed over all sources (for TEPS)."""
+    srt = np.sort(np.ascontiguousarray(lv_all.T), axis=1)  # [N][nsrc]
+    distinct = srt != 0
+    distinct[:, 1:] &= srt[:, 1:] != srt[:, :-1]
+    D = distinct.sum(axis=1).astype(np.int64)
+    b = int((D * (16 + 17 * outdeg)).sum()) + nsrc * N * 4
+    teps_edges = int(((lv_all != 0).sum(axis=0).astype(np.int64) * outdeg).sum())
+    return b, teps_edges
+
+
+def run_batched(G, N, keys, half, steps, warmup, nsrc=64, fused=8, check=True):
+    """Batched form: uneven batches (log-uniform 1e3..1e6 updates, 1:1
+    insert:delete, seed 2), `fused` batches per update call, then one
+    `nsrc`-source BFS (sources uniform over the nodes, seed 2) per step."""
+    rs = np.random.default_rng(2)
+    total = steps + warmup
+    sizes = np.exp(rs.uniform(np.log(1e3), np.log(1e6), total * fused)).astype(np.int64) // 2 * 2
+    starts_all = rs.integers(0, N, (total, nsrc)).astype(np.uint32)
+    need = int(sizes.sum() // 2)
+    if half + need > len(keys):
+        raise RuntimeError("not enough edges for the requested batched steps")
+    g = G.GPMA(N)
+    G.update_gpma(g, keys[:half], np.ones(half, np.uint8))
+    # stage every step's fused batches in HBM
+    ins, dele = half, 0
+    step_keys, step_offs = [], []
+    for st_i in range(total):
+        ks, offs = [], [0]
+        for b in range(fused):
+            h = int(sizes[st_i * fused + b] // 2)
+            ks.append(keys[ins: ins + h])
+            ks.append(keys[dele: dele + h])
+            ins += h
+            dele += h
+            offs.append(offs[-1] + 2 * h)
+        step_keys.append(np.concatenate(ks))
+        step_offs.append(np.array(offs, np.uint64))
+    vals = []
+    for st_i in range(total):
+        v = np.empty(len(step_keys[st_i]), np.uint8)
+        o = step_offs[st_i]
+        for b in range(fused):
+            a0, a1 = int(o[b]), int(o[b + 1])
+            h = (a1 - a0) // 2
+            v[a0: a0 + h] = 1
+            v[a0 + h: a1] = 0
+        vals.append(v)
+    dk = [G.DeviceBuffer.from_numpy(k) for k in step_keys]
+    dv = [G.DeviceBuffer.from_numpy(v) for v in vals]
+    dlev = G.DeviceBuffer(nsrc * N * 4)
+    for i in range(warmup):
+        G.update_gpma_batches_device(g, dk[i].ptr, dv[i].ptr, step_offs[i])
+        G.gpma_bfs_multi(g, starts_all[i], dlev.ptr)
+    g.sync()
+    upd_t, bfs_t, per = 0.0, 0.0, []
+    g.profile(True, reset=True)
+    t_all = time.perf_counter()
+    for i in range(warmup, total):
+        a = time.perf_counter()
+        G.update_gpma_batches_device(g, dk[i].ptr, dv[i].ptr, step_offs[i])
+        b = time.perf_counter()
+        G.gpma_bfs_multi(g, starts_all[i], dlev.ptr)
+        c = time.perf_counter()
+        upd_t += b - a
+        bfs_t += c - b
+        per.append(c - a)
+    t_all = time.perf_counter() - t_all
+    g.profile(False)
+    prof = g.profile_report()
+    n_upd = int(sum(len(step_keys[i]) for i in range(warmup, total)))
+    out = {"steps": steps, "warmup": warmup, "fused_batches": fused, "sources": nsrc,
+           "updates": n_upd, "mean_batch": float(sizes[warmup * fused:].mean()),
+           "edge_updates_per_s": n_upd / upd_t, "update_ms_per_step": upd_t / steps * 1e3,
+           "msbfs_ms_per_step": bfs_t / steps * 1e3, "step_ms": t_all / steps * 1e3,
+           "step_p50_ms": float(np.percentile(np.array(per) * 1e3, 50)),
+           "step_p99_ms": float(np.percentile(np.array(per) * 1e3, 99)),
+           "phase_ms_per_step": {k: v["ms"] / steps for k, v in prof.items()}}
+    # algorithmic bytes and TEPS of the last step's BFS, and parity spot checks
+    win = keys[dele: ins]
+    outdeg = np.bincount((win >> np.uint64(32)).astype(np.int64), minlength=N)
+    lv_all, reach = G.gpma_bfs_multi(g, starts_all[total - 1])
+    mb, me = ms_bfs_bytes(N, outdeg, lv_all, nsrc)
+    one_ms = out["msbfs_ms_per_step"]
+    out.update({"msbfs_algorithmic_bytes": mb, "msbfs_edges": me, "msbfs_gteps": me / (one_ms / 1e3) / 1e9,
+                "msbfs_achieved_gbs": mb / (one_ms / 1e3) / 1e9,
+                "msbfs_frac": mb / (one_ms / 1e3) / 1e9 / HBM_PEAK_GBS})
+    n_ins = n_upd // 2
+    ub = update_bytes(n_ins, n_upd - n_ins)["total"]
+    out.update({"update_algorithmic_bytes_per_step": ub / steps,
+                "update_achieved_gbs": ub / upd_t / 1e9, "update_frac": ub / upd_t / 1e9 / HBM_PEAK_GBS,
+                "step_frac": (ub / steps + mb) / (t_all / steps) / 1e9 / HBM_PEAK_GBS})
+    if check:
+        live = g.export_live()
+        guards = (np.arange(N, dtype=np.uint64) << np.uint64(32)) | np.uint64(0xFFFFFFFF)
+        expect = np.sort(np.concatenate([win, guards, np.array([G.KEY_MAX, G.KEY_MAX], np.uint64)]))
+        ok_live = bool(np.array_equal(live, expect))
+        ok_bfs = True
+        for i in (0, 17, nsrc - 1):
+            one, r1 = G.gpma_bfs(g, int(starts_all[total - 1][i]))
+            ok_bfs &= bool(np.array_equal(one, lv_all[i]) and r1 == reach[i])
+        out["parity"] = {"live_set_closed_form": ok_live, "sources_equal_plain_bfs": ok_bfs}
+    for b in dk + dv:
+        b.free()
+    dlev.free()
+    g.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +185,7 @@ def main():
     ap.add_argument("--cpu-slides", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--small", action="store_true", help="R-MAT 2^16/2^20 instead (quick check)")
+    ap.add_argument("--batched-steps", type=int, default=10, help="batched-form steps (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -190,6 +303,17 @@ def main():
                 "algorithmic_bytes": bfs_b, "achieved_gbs": bfs_b / bfs_kernel_s / 1e9 if bfs_kernel_s else None,
                 "frac": (bfs_b / bfs_kernel_s / 1e9 / HBM_PEAK_GBS) if bfs_kernel_s else None}
 
+    batched = None
+    if args.batched_steps > 0 and not args.small:
+        g.close()
+        log("batched form: 8 fused uneven batches + 64-source BFS per step")
+        batched = run_batched(G, N, keys, half, args.batched_steps, 2)
+        batched["vs_plain_update_throughput"] = batched["edge_updates_per_s"] / ups
+        log(f"batched: {batched['edge_updates_per_s'] / 1e9:.3f} G edge-updates/s "
+            f"({batched['vs_plain_update_throughput']:.2f}x plain), 64-source BFS "
+            f"{batched['msbfs_ms_per_step']:.2f} ms ({batched['msbfs_gteps']:.1f} GTEPS), "
+            f"step p99 {batched['step_p99_ms']:.2f} ms, parity {batched.get('parity')}")
+
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -222,6 +346,7 @@ def main():
         "step_roofline": step_roofline,
         "cpu_baseline": cpu,
         "bfs": bfs_info,
+        "batched": batched,
         "latency_ms": {"p50": float(np.percentile(per_ms, 50)), "p99": float(np.percentile(per_ms, 99))},
         "phase_ms_per_step": phase_ms,
         "layout": {k: st[k] for k in ("capacity", "segment_length", "tree_height")},

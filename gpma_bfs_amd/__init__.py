@@ -201,6 +201,31 @@ def update_gpma(gpma, keys, values, strict=True):
     return rc
 
 
+def update_gpma_batches(gpma, batches_keys, batches_values, strict=True):
+    """Apply several batches in one fused call (same live keys as applying
+    them one after another). Host numpy arrays, one pair per batch."""
+    keys = np.ascontiguousarray(np.concatenate(batches_keys), np.uint64)
+    values = np.ascontiguousarray(np.concatenate(batches_values), np.uint8)
+    offsets = np.zeros(len(batches_keys) + 1, np.uint64)
+    offsets[1:] = np.cumsum([len(k) for k in batches_keys])
+    if int(offsets[-1]) != len(values):
+        raise GpmaError(GPMA_ERR_ARG, "keys and values differ in length")
+    rc = lib.gpma_update_batches(gpma.handle, keys.ctypes.data, values.ctypes.data, offsets.ctypes.data,
+                                 len(batches_keys), 0)
+    if strict:
+        check(rc, "gpma_update_batches")
+    return rc
+
+
+def update_gpma_batches_device(gpma, keys_ptr, values_ptr, offsets, strict=True):
+    """Fused batches already resident in device memory; offsets on the host."""
+    offsets = np.ascontiguousarray(offsets, np.uint64)
+    rc = lib.gpma_update_batches(gpma.handle, keys_ptr, values_ptr, offsets.ctypes.data, len(offsets) - 1, 1)
+    if strict:
+        check(rc, "gpma_update_batches")
+    return rc
+
+
 def gpma_bfs(gpma, start_node, results=None):
     """BFS levels from start_node (reference gpma_bfs.cuh:517-533).
 
@@ -215,6 +240,26 @@ def gpma_bfs(gpma, start_node, results=None):
     reach = ctypes.c_uint32()
     check(lib.gpma_bfs(gpma.handle, start_node, rp, dev, ctypes.byref(reach)), "gpma_bfs")
     return results, reach.value
+
+
+def gpma_bfs_multi(gpma, starts, levels_device_ptr=None, return_levels=True):
+    """BFS from up to 64 sources in one traversal. Returns (levels, reachable):
+    levels[i] is the level array gpma_bfs(starts[i]) returns (None if
+    return_levels is False or levels_device_ptr is given)."""
+    starts = np.ascontiguousarray(starts, np.uint32)
+    ns = len(starts)
+    reach = np.empty(ns, np.uint32)
+    out = None
+    if levels_device_ptr is not None:
+        check(lib.gpma_bfs_multi(gpma.handle, starts.ctypes.data, ns, levels_device_ptr, 1, reach.ctypes.data),
+              "gpma_bfs_multi")
+    elif return_levels:
+        out = np.empty((ns, gpma.row_num), np.uint32)
+        check(lib.gpma_bfs_multi(gpma.handle, starts.ctypes.data, ns, out.ctypes.data, 0, reach.ctypes.data),
+              "gpma_bfs_multi")
+    else:
+        check(lib.gpma_bfs_multi(gpma.handle, starts.ctypes.data, ns, None, 0, reach.ctypes.data), "gpma_bfs_multi")
+    return out, reach
 
 
 def gpma_bfs_count(gpma, start_node, levels_device_ptr=None):

@@ -85,7 +85,9 @@ SIGNATURES = {
     "gpma_create": (_I, [_U32, _P, _P]),
     "gpma_destroy": (_I, [_P]),
     "gpma_update": (_I, [_P, _P, _P, _U64, _I]),
+    "gpma_update_batches": (_I, [_P, _P, _P, _P, _U32, _I]),
     "gpma_bfs": (_I, [_P, _U32, _P, _I, _P]),
+    "gpma_bfs_multi": (_I, [_P, _P, _U32, _P, _I, _P]),
     "gpma_csr_bfs": (_I, [_P, _P, _P, _U32, _U32, _U32, _P]),
     "gpma_get_csr": (_I, [_P, _P]),
     "gpma_copy_layout": (_I, [_P, _P, _P, _P]),

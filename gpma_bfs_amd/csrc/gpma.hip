@@ -128,6 +128,10 @@ struct gpma_handle {
     Buf tmpk, tmpv, chunkcnt, chunkoff;
     // bfs scratch
     Buf lev, fr0, fr1, fb0, fb1, vis, bcnt, bcntb;
+    // multi-source bfs scratch
+    Buf ms_seen, ms_fr0, ms_fr1, ms_nx, ms_lv16, ms_q0, ms_q1, ms_qb0, ms_qb1, ms_touched, ms_cnt, ms_cntb, ms_tcnt,
+        ms_partial, ms_starts, ms_out, ms_reach;
+    unsigned long long *ms_hcnt = nullptr;
     unsigned long long *hcnt = nullptr;  // pinned mirror of one chunk of bcnt and bcntb
     double pull_fraction = 0.25;  // pull when the frontier's slots exceed this share of C
     Buf ctr;
@@ -548,6 +552,88 @@ int run_bfs(const gkey *keys, const gval *values, uint64_t C, const uint32_t *ro
     return GPMA_OK;
 }
 
+int ensure_msbfs(gpma_handle *g) {
+    const uint64_t n = (uint64_t)g->N + 1;
+    uint64_t *t = &g->dev_bytes;
+    TRY(g->ms_seen.ensure(n * 8, t));
+    TRY(g->ms_fr0.ensure(n * 8, t));
+    TRY(g->ms_fr1.ensure(n * 8, t));
+    TRY(g->ms_nx.ensure(n * 8, t));
+    TRY(g->ms_lv16.ensure(n * 128, t));
+    TRY(g->ms_q0.ensure(n * 4, t));
+    TRY(g->ms_q1.ensure(n * 4, t));
+    TRY(g->ms_qb0.ensure(n * 4, t));
+    TRY(g->ms_qb1.ensure(n * 4, t));
+    TRY(g->ms_touched.ensure(n * 4, t));
+    TRY(g->ms_cnt.ensure((n + BFS_MAX_CHUNK + 2) * 8, t));
+    TRY(g->ms_cntb.ensure((n + BFS_MAX_CHUNK + 2) * 8, t));
+    TRY(g->ms_tcnt.ensure((n + BFS_MAX_CHUNK + 2) * 4, t));
+    TRY(g->ms_partial.ensure((uint64_t)ms_output_blocks(g->N) * 64 * 4, t));
+    TRY(g->ms_starts.ensure(64 * 4, t));
+    TRY(g->ms_reach.ensure(64 * 4, t));
+    if (!g->ms_hcnt)
+        HIP_TRY(hipHostMalloc((void **)&g->ms_hcnt, 2 * (BFS_MAX_CHUNK + 1) * 8, hipHostMallocDefault));
+    return GPMA_OK;
+}
+
+MsState ms_state(gpma_handle *g) {
+    MsState m;
+    m.seen = g->ms_seen.as<uint64_t>();
+    m.fr[0] = g->ms_fr0.as<uint64_t>();
+    m.fr[1] = g->ms_fr1.as<uint64_t>();
+    m.nx = g->ms_nx.as<uint64_t>();
+    m.lv16 = g->ms_lv16.as<uint16_t>();
+    m.q[0] = g->ms_q0.as<uint32_t>();
+    m.q[1] = g->ms_q1.as<uint32_t>();
+    m.qbig[0] = g->ms_qb0.as<uint32_t>();
+    m.qbig[1] = g->ms_qb1.as<uint32_t>();
+    m.touched = g->ms_touched.as<uint32_t>();
+    m.cnt = g->ms_cnt.as<unsigned long long>();
+    m.cnt_big = g->ms_cntb.as<unsigned long long>();
+    m.tcnt = g->ms_tcnt.as<uint32_t>();
+    m.partial = g->ms_partial.as<uint32_t>();
+    return m;
+}
+
+// Multi-source level loop: chunks of levels without a host round trip, like
+// run_bfs; levels land in out[nsrc][N] (device), counts in reach[nsrc] (device).
+int run_msbfs(gpma_handle *g, const uint32_t *d_starts, uint32_t nsrc, uint32_t *out, uint32_t *reach) {
+    const MsState m = ms_state(g);
+    hipStream_t st = g->st;
+    const uint32_t N = g->N;
+    const unsigned long long thr = (unsigned long long)(g->pull_fraction * (double)g->C);
+    ms_init(m, N, d_starts, nsrc, g->row_offset.as<uint32_t>(), st);
+    uint32_t level = 1, chunk = 4;
+    bool done = false;
+    while (!done) {
+        if ((uint64_t)level + chunk + 1 > (uint64_t)N + BFS_MAX_CHUNK) chunk = 1;
+        // counts of the chunk's levels (touched counts are indexed by the
+        // expanding level; the first level's is cleared by ms_init or the
+        // previous chunk)
+        ms_clear(m, level + 1, level + chunk + 1, st);
+        phase_begin(g, P_BFS);
+        for (uint32_t j = 0; j < chunk; j++)
+            ms_level(g->keys.as<gkey>(), g->values.as<gval>(), g->C, N, g->row_offset.as<uint32_t>(), m, level + j,
+                     thr, st);
+        phase_end(g, P_BFS, 2 * chunk);
+        HIP_TRY(hipMemcpyAsync(g->ms_hcnt, m.cnt + level + 1, chunk * 8, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(g->ms_hcnt + BFS_MAX_CHUNK + 1, m.cnt_big + level + 1, chunk * 8, hipMemcpyDeviceToHost,
+                               st));
+        TRY(stream_sync(g));
+        for (uint32_t j = 0; j < chunk && !done; j++)
+            if ((uint32_t)g->ms_hcnt[j] == 0 && (uint32_t)g->ms_hcnt[BFS_MAX_CHUNK + 1 + j] == 0) done = true;
+        level += chunk;
+        chunk = std::min<uint32_t>(chunk * 2, BFS_MAX_CHUNK);
+        if (level >= N + 1) done = true;
+    }
+    phase_begin(g, P_BFS);
+    ms_output(m, N, nsrc, out, reach, st);
+    phase_end(g, P_BFS, 2);
+    TRY(stream_sync(g));
+    HIP_TRY(hipGetLastError());
+    return GPMA_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -664,10 +750,13 @@ int gpma_destroy(gpma_t g) {
                   &g->sk1, &g->sv1, &g->win, &g->blkcnt, &g->blkoff, &g->sortwork, &g->pk0, &g->pv0,
                   &g->pw0, &g->pd0, &g->pk1, &g->pv1, &g->pw1, &g->pd1, &g->tmpk, &g->tmpv, &g->chunkcnt,
                   &g->chunkoff, &g->lev, &g->fr0, &g->fr1, &g->fb0, &g->fb1, &g->vis, &g->bcnt, &g->bcntb,
-                  &g->ctr};
+                  &g->ms_seen, &g->ms_fr0, &g->ms_fr1, &g->ms_nx, &g->ms_lv16, &g->ms_q0, &g->ms_q1, &g->ms_qb0,
+                  &g->ms_qb1, &g->ms_touched, &g->ms_cnt, &g->ms_cntb, &g->ms_tcnt, &g->ms_partial, &g->ms_starts,
+                  &g->ms_out, &g->ms_reach, &g->ctr};
     for (Buf *b : all) b->release(&g->dev_bytes);
     if (g->hctr) (void)hipHostFree(g->hctr);
     if (g->hcnt) (void)hipHostFree(g->hcnt);
+    if (g->ms_hcnt) (void)hipHostFree(g->ms_hcnt);
     for (int p = 0; p < GPMA_PHASES; p++) {
         if (g->ev_a[p]) (void)hipEventDestroy(g->ev_a[p]);
         if (g->ev_b[p]) (void)hipEventDestroy(g->ev_b[p]);
@@ -680,6 +769,18 @@ int gpma_destroy(gpma_t g) {
 int gpma_update(gpma_t g, const uint64_t *keys, const uint8_t *values, uint64_t n, int on_device) {
     if (!g) return fail(GPMA_ERR_ARG, "gpma_update: null handle");
     return do_update(g, keys, values, n, on_device);
+}
+
+int gpma_update_batches(gpma_t g, const uint64_t *keys, const uint8_t *values, const uint64_t *offsets,
+                        uint32_t nbatches, int on_device) {
+    if (!g || !offsets) return fail(GPMA_ERR_ARG, "gpma_update_batches: null argument");
+    if (nbatches == 0) return GPMA_OK;
+    if (offsets[0] != 0) return fail(GPMA_ERR_ARG, "gpma_update_batches: offsets[0] must be 0");
+    for (uint32_t b = 0; b < nbatches; b++)
+        if (offsets[b + 1] < offsets[b]) return fail(GPMA_ERR_ARG, "gpma_update_batches: offsets must not decrease");
+    // Batches are contiguous in (batch, position) order, which the stable sort
+    // keeps within each key: the last record of a key is its last update.
+    return do_update(g, keys, values, offsets[nbatches], on_device);
 }
 
 int gpma_bfs(gpma_t g, uint32_t start, uint32_t *levels, int on_device, uint32_t *reachable) {
@@ -696,6 +797,28 @@ int gpma_bfs(gpma_t g, uint32_t start, uint32_t *levels, int on_device, uint32_t
         HIP_TRY(hipStreamSynchronize(g->st));
     }
     if (reachable) *reachable = reach;
+    return GPMA_OK;
+}
+
+int gpma_bfs_multi(gpma_t g, const uint32_t *starts, uint32_t nsrc, uint32_t *levels, int on_device,
+                   uint32_t *reachable) {
+    if (!g || !starts) return fail(GPMA_ERR_ARG, "gpma_bfs_multi: null argument");
+    if (nsrc == 0 || nsrc > 64) return fail(GPMA_ERR_ARG, "gpma_bfs_multi: nsrc must be in [1, 64]");
+    for (uint32_t i = 0; i < nsrc; i++)
+        if (starts[i] >= g->N) return fail(GPMA_ERR_ARG, "gpma_bfs_multi: start node >= row_num");
+    HIP_TRY(hipSetDevice(g->device));
+    TRY(ensure_msbfs(g));
+    HIP_TRY(hipMemcpyAsync(g->ms_starts.p, starts, nsrc * 4, hipMemcpyHostToDevice, g->st));
+    uint32_t *out = levels;
+    if (!on_device || !levels) {
+        TRY(g->ms_out.ensure((uint64_t)nsrc * g->N * 4, &g->dev_bytes));
+        out = g->ms_out.as<uint32_t>();
+    }
+    TRY(run_msbfs(g, g->ms_starts.as<uint32_t>(), nsrc, out, g->ms_reach.as<uint32_t>()));
+    if (levels && !on_device)
+        HIP_TRY(hipMemcpyAsync(levels, out, (uint64_t)nsrc * g->N * 4, hipMemcpyDeviceToHost, g->st));
+    if (reachable) HIP_TRY(hipMemcpyAsync(reachable, g->ms_reach.p, nsrc * 4, hipMemcpyDeviceToHost, g->st));
+    HIP_TRY(hipStreamSynchronize(g->st));
     return GPMA_OK;
 }
 

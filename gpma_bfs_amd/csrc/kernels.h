@@ -99,4 +99,23 @@ void launch_bfs_clear(const BfsState &b, uint32_t a, uint32_t e, hipStream_t st)
 void launch_bfs_level(const gkey *keys, const gval *values, uint64_t C, uint32_t N, const uint32_t *row_offset,
                       const BfsState &b, uint32_t level, unsigned long long pull_threshold, hipStream_t st);
 
+// ---- msbfs.hip
+// Device state of one multi-source BFS (<= 64 sources, bit i = starts[i]).
+struct MsState {
+    uint64_t *seen, *fr[2], *nx;
+    uint16_t *lv16;            // [N][64] levels, vertex-major
+    uint32_t *q[2], *qbig[2];  // frontier lists by parity (short / long ranges)
+    uint32_t *touched;         // vertices whose nx went non-zero this level
+    unsigned long long *cnt, *cnt_big;  // per level: low = vertices, high = slots
+    uint32_t *tcnt;            // per level touched count
+    uint32_t *partial;         // [ms_output_blocks(N)][64] reachable partials
+};
+void ms_init(const MsState &m, uint32_t N, const uint32_t *starts, uint32_t nsrc, const uint32_t *ro, hipStream_t st);
+void ms_clear(const MsState &m, uint32_t a, uint32_t b, hipStream_t st);
+void ms_level(const gkey *keys, const gval *values, uint64_t C, uint32_t N, const uint32_t *ro, const MsState &m,
+              uint32_t level, unsigned long long pull_threshold, hipStream_t st);
+// Source-major u32 levels out[nsrc][N] and reachable counts reach[nsrc].
+void ms_output(const MsState &m, uint32_t N, uint32_t nsrc, uint32_t *out, uint32_t *reach, hipStream_t st);
+uint32_t ms_output_blocks(uint32_t N);
+
 }  // namespace gpma

@@ -1,0 +1,460 @@
+// Multi-source BFS: up to 64 sources at once over the gapped packed array,
+// one bit per source in a 64-bit mask per vertex (the "MS-BFS" bit-parallel
+// scheme of Then et al., VLDB 2015), so one traversal of an edge serves every
+// source that reaches its tail at the same level.
+//
+// Per level l (masks: seen, fr = sources for which v is at level l):
+//   expand:   for every frontier vertex v and live slot (v -> w),
+//             add = fr[v] & ~seen[w]; atomicOr(nx[w], add); the first lane
+//             to make nx[w] non-zero queues w as touched. Push walks the
+//             frontier list's slot ranges (long ranges cut into 2048-slot
+//             pieces); pull streams every slot of the array when the
+//             frontier's slots exceed a share of it.
+//   finalize: for every touched w: new = nx[w] & ~seen[w]; nx[w] = 0;
+//             seen[w] |= new; fr_next[w] = new; level(s, w) = l + 1 for each
+//             bit s of new; w joins the next frontier if new != 0. The old
+//             frontier's fr entries are cleared in the same pass.
+// Levels are kept as u16 in a vertex-major scratch (a vertex's 64 levels in
+// one 128-byte line) and transposed at the end into the caller's
+// source-major u32 arrays, with per-source reachable counts.
+//
+// The level of source s at w is 1 + hop distance, as in the single-source
+// BFS (gpma_bfs.cuh:446-509 semantics), so every source's array equals the
+// single-source result bit for bit.
+#include "common.h"
+#include "kernels.h"
+
+namespace gpma {
+
+namespace {
+
+constexpr uint32_t T = 256;
+constexpr uint32_t MS_TILE = 64;
+constexpr uint32_t MS_UNROLL = 4;
+constexpr uint32_t MS_BIG_RANGE = 2048;
+constexpr uint32_t MS_BIG_CHUNK = 2048;
+constexpr uint32_t MS_BIG_LIST = 2048;
+constexpr uint32_t MS_PULL_PER_THREAD = 16;
+constexpr uint32_t LQ = 2048;
+
+__device__ __forceinline__ uint32_t rlen(const uint32_t *ro, uint32_t v) {
+    const uint32_t b = ro[v], e = ro[v + 1];
+    return e > b ? e - b : 0;
+}
+
+__global__ void k_ms_init(uint64_t *__restrict__ seen, uint64_t *__restrict__ fr0, uint64_t *__restrict__ fr1,
+                          uint64_t *__restrict__ nx, uint16_t *__restrict__ lv16, uint32_t N) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint64_t i = t; i < N; i += stride) {
+        seen[i] = 0;
+        fr0[i] = 0;
+        fr1[i] = 0;
+        nx[i] = 0;
+    }
+    uint4 *l = (uint4 *)lv16;
+    const uint64_t nl = (uint64_t)N * 64 * 2 / 16;
+    for (uint64_t i = t; i < nl; i += stride) l[i] = make_uint4(0, 0, 0, 0);
+}
+
+// Sources: bit i for starts[i]; the frontier list gets each distinct start
+// once (the first index holding it queues it).
+__global__ void k_ms_sources(const uint32_t *__restrict__ starts, uint32_t nsrc, uint64_t *__restrict__ seen,
+                             uint64_t *__restrict__ fr, uint16_t *__restrict__ lv16, uint32_t *__restrict__ q,
+                             uint32_t *__restrict__ qbig, unsigned long long *__restrict__ cnt,
+                             unsigned long long *__restrict__ cnt_big, uint32_t *__restrict__ tcnt,
+                             const uint32_t *__restrict__ ro) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    cnt[0] = cnt_big[0] = 0;
+    tcnt[0] = tcnt[1] = 0;
+    unsigned long long c = 0, cb = 0;
+    for (uint32_t i = 0; i < nsrc; i++) {
+        const uint32_t s = starts[i];
+        const uint64_t bit = 1ull << i;
+        bool first = true;
+        for (uint32_t j = 0; j < i; j++)
+            if (starts[j] == s) first = false;
+        seen[s] |= bit;
+        fr[s] |= bit;
+        lv16[(uint64_t)s * 64 + i] = 1;
+        if (first) {
+            const unsigned long long len = rlen(ro, s);
+            if (len > MS_BIG_RANGE) qbig[(uint32_t)cb] = s, cb += (len << 32) | 1ull;
+            else q[(uint32_t)c] = s, c += (len << 32) | 1ull;
+        }
+    }
+    cnt[1] = c;
+    cnt_big[1] = cb;
+}
+
+struct MsQueue {  // LDS staging of touched vertices
+    uint32_t *buf, *n, *base;
+};
+
+__device__ __forceinline__ void touch(bool first, uint32_t w, const MsQueue &lq) {
+    const uint64_t m = __ballot(first);
+    if (!m) return;
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1;
+    uint32_t b = 0;
+    if (lane_id() == leader) b = atomicAdd(lq.n, (uint32_t)__popcll(m));
+    b = __shfl(b, leader, WAVE);
+    if (first) lq.buf[b + (uint32_t)__popcll(m & lanemask_lt())] = w;
+}
+
+__device__ __forceinline__ void lq_flush(const MsQueue &lq, uint32_t *touched, uint32_t *tcnt, uint32_t room) {
+    __syncthreads();
+    const uint32_t n = *lq.n;
+    if (n == 0 || (room <= LQ && n + room <= LQ)) return;
+    if (threadIdx.x == 0) *lq.base = atomicAdd(tcnt, n);
+    __syncthreads();
+    const uint32_t b = *lq.base;
+    for (uint32_t i = threadIdx.x; i < n; i += T) touched[b + i] = lq.buf[i];
+    __syncthreads();
+    if (threadIdx.x == 0) *lq.n = 0;
+    __syncthreads();
+}
+
+// Three-stage batch of slots (source masks already known): seen words, then
+// the atomic ORs, then touched-queue appends.
+template <uint32_t U>
+__device__ __forceinline__ void ms_visit(const gkey (&k)[U], const gval (&v)[U], const uint64_t (&m)[U],
+                                         const uint64_t *seen, uint64_t *nx, const MsQueue &lq) {
+    uint32_t w[U];
+    uint64_t add[U], old[U];
+    bool cand[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++) {
+        w[u] = (uint32_t)k[u];
+        cand[u] = m[u] != 0 && w[u] != COL_NONE && v[u] != 0;
+        add[u] = cand[u] ? m[u] & ~*(volatile const uint64_t *)(seen + w[u]) : 0;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++) old[u] = add[u] ? atomicOr((unsigned long long *)(nx + w[u]), add[u]) : ~0ull;
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++) touch(add[u] != 0 && old[u] == 0, w[u], lq);
+}
+
+__global__ __launch_bounds__(T) void k_ms_expand(const gkey *__restrict__ keys, const gval *__restrict__ values,
+                                                 uint64_t C, uint32_t N, const uint32_t *__restrict__ ro,
+                                                 const uint32_t *__restrict__ q, const uint32_t *__restrict__ qbig,
+                                                 const unsigned long long *__restrict__ cnt,
+                                                 const unsigned long long *__restrict__ cnt_big,
+                                                 const uint64_t *__restrict__ fr, const uint64_t *__restrict__ seen,
+                                                 uint64_t *__restrict__ nx, uint32_t *__restrict__ touched,
+                                                 uint32_t *__restrict__ tcnt, uint32_t level,
+                                                 unsigned long long pull_threshold, uint32_t small_grid) {
+    __shared__ uint32_t s_beg[MS_BIG_LIST];
+    __shared__ uint32_t s_ofs[MS_BIG_LIST + 1];
+    __shared__ uint32_t scratch[T / WAVE + 1];
+    __shared__ uint32_t lq_buf[LQ];
+    __shared__ uint32_t lq_n, lq_base;
+    const unsigned long long a = cnt[level], b = cnt_big[level];
+    const uint32_t nf = (uint32_t)a, nbig = (uint32_t)b;
+    if (nf == 0 && nbig == 0) return;
+    const MsQueue lq{lq_buf, &lq_n, &lq_base};
+    if (threadIdx.x == 0) lq_n = 0;
+    __syncthreads();
+    const uint32_t tid = threadIdx.x;
+    const unsigned long long work = (a >> 32) + (b >> 32);
+    if (C && work > pull_threshold) {
+        // streaming pull over every slot
+        const uint64_t per = (uint64_t)T * MS_PULL_PER_THREAD;
+        for (uint64_t base = (uint64_t)blockIdx.x * per; base < C; base += (uint64_t)gridDim.x * per) {
+            for (uint32_t r = 0; r < MS_PULL_PER_THREAD; r += MS_UNROLL) {
+                gkey k[MS_UNROLL];
+                gval v[MS_UNROLL];
+                uint64_t m[MS_UNROLL];
+#pragma unroll
+                for (uint32_t u = 0; u < MS_UNROLL; u++) {
+                    const uint64_t p = base + (uint64_t)(r + u) * T + tid;
+                    k[u] = p < C ? keys[p] : KEY_NONE;
+                    v[u] = p < C ? values[p] : 0;
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < MS_UNROLL; u++) {
+                    const uint32_t src = (uint32_t)(k[u] >> 32);
+                    m[u] = src < N ? fr[src] : 0;
+                }
+                ms_visit<MS_UNROLL>(k, v, m, seen, nx, lq);
+                lq_flush(lq, touched, tcnt, T * MS_UNROLL);
+            }
+        }
+    } else if (blockIdx.x < small_grid) {
+        for (uint64_t tile = (uint64_t)blockIdx.x * MS_TILE; tile < nf; tile += (uint64_t)small_grid * MS_TILE) {
+            const uint64_t fi = tile + tid;
+            uint32_t bb = 0, len = 0;
+            if (tid < MS_TILE && fi < nf) {
+                const uint32_t vv = q[fi];
+                bb = ro[vv];
+                len = rlen(ro, vv);
+                s_beg[MS_TILE + tid] = vv;  // vertex ids after the offsets
+            }
+            uint32_t total;
+            const uint32_t ofs = block_exclusive_scan(len, scratch, &total);
+            if (tid < MS_TILE) {
+                s_beg[tid] = bb;
+                s_ofs[tid] = ofs;
+            }
+            __syncthreads();
+            for (uint32_t j0 = 0; j0 < total; j0 += T * MS_UNROLL) {
+                gkey k[MS_UNROLL];
+                gval v[MS_UNROLL];
+                uint64_t m[MS_UNROLL];
+#pragma unroll
+                for (uint32_t u = 0; u < MS_UNROLL; u++) {
+                    const uint32_t j = j0 + u * T + tid;
+                    k[u] = KEY_NONE;
+                    v[u] = 0;
+                    m[u] = 0;
+                    if (j < total) {
+                        uint32_t lo = 0, hi = MS_TILE;
+                        while (hi - lo > 1) {
+                            const uint32_t mid = (lo + hi) >> 1;
+                            if (s_ofs[mid] <= j) lo = mid;
+                            else hi = mid;
+                        }
+                        const uint64_t slot = (uint64_t)s_beg[lo] + (j - s_ofs[lo]);
+                        k[u] = keys[slot];
+                        v[u] = values[slot];
+                        m[u] = fr[s_beg[MS_TILE + lo]];
+                    }
+                }
+                ms_visit<MS_UNROLL>(k, v, m, seen, nx, lq);
+                lq_flush(lq, touched, tcnt, T * MS_UNROLL);
+            }
+            __syncthreads();
+        }
+    } else {
+        const uint32_t bgrid = gridDim.x - small_grid;
+        const uint32_t bid = blockIdx.x - small_grid;
+        for (uint32_t l0 = 0; l0 < nbig; l0 += MS_BIG_LIST) {
+            const uint32_t lcount = min(MS_BIG_LIST, nbig - l0);
+            uint32_t run = 0;
+            for (uint32_t i0 = 0; i0 < lcount; i0 += T) {
+                const uint32_t i = i0 + tid;
+                uint32_t nch = 0, bb = 0;
+                if (i < lcount) {
+                    const uint32_t vv = qbig[l0 + i];
+                    bb = ro[vv];
+                    nch = (rlen(ro, vv) + MS_BIG_CHUNK - 1) / MS_BIG_CHUNK;
+                }
+                uint32_t tot;
+                const uint32_t pre = block_exclusive_scan(nch, scratch, &tot);
+                if (i < lcount) {
+                    s_beg[i] = bb;
+                    s_ofs[i] = run + pre;
+                }
+                run += tot;
+            }
+            if (tid == 0) s_ofs[lcount] = run;
+            __syncthreads();
+            for (uint32_t c = bid; c < run; c += bgrid) {
+                uint32_t lo = 0, hi = lcount;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (s_ofs[mid] <= c) lo = mid;
+                    else hi = mid;
+                }
+                const uint32_t vv = qbig[l0 + lo];
+                const uint64_t mask = fr[vv];
+                const uint64_t beg = (uint64_t)s_beg[lo] + (uint64_t)(c - s_ofs[lo]) * MS_BIG_CHUNK;
+                const uint64_t end = min((uint64_t)ro[vv + 1], beg + MS_BIG_CHUNK);
+                for (uint64_t p0 = beg; p0 < end; p0 += T * MS_UNROLL) {
+                    gkey k[MS_UNROLL];
+                    gval v[MS_UNROLL];
+                    uint64_t m[MS_UNROLL];
+#pragma unroll
+                    for (uint32_t u = 0; u < MS_UNROLL; u++) {
+                        const uint64_t p = p0 + u * T + tid;
+                        const bool in = p < end;
+                        k[u] = in ? keys[p] : KEY_NONE;
+                        v[u] = in ? values[p] : 0;
+                        m[u] = in ? mask : 0;
+                    }
+                    ms_visit<MS_UNROLL>(k, v, m, seen, nx, lq);
+                    lq_flush(lq, touched, tcnt, T * MS_UNROLL);
+                }
+            }
+            __syncthreads();
+        }
+    }
+    lq_flush(lq, touched, tcnt, LQ + 1);
+}
+
+// Finalize level `level`: clear the old frontier masks, turn touched
+// vertices' accumulated masks into newly seen sources, record their levels,
+// and queue the next frontier (short / long ranges, packed count + slots).
+__global__ __launch_bounds__(T) void k_ms_finalize(const uint32_t *__restrict__ q, const uint32_t *__restrict__ qbig,
+                                                   const unsigned long long *__restrict__ cnt,
+                                                   const unsigned long long *__restrict__ cnt_big,
+                                                   uint64_t *__restrict__ fr_cur, uint64_t *__restrict__ fr_next,
+                                                   uint64_t *__restrict__ seen, uint64_t *__restrict__ nx,
+                                                   uint16_t *__restrict__ lv16, const uint32_t *__restrict__ touched,
+                                                   const uint32_t *__restrict__ tcnt, uint32_t *__restrict__ q2,
+                                                   uint32_t *__restrict__ qbig2, unsigned long long *__restrict__ cnt2,
+                                                   unsigned long long *__restrict__ cnt_big2,
+                                                   const uint32_t *__restrict__ ro, uint32_t level) {
+    __shared__ uint32_t lq_buf[LQ];
+    __shared__ uint32_t lq_n, lq_work, lq_base;
+    const uint32_t nf = (uint32_t)cnt[level], nbig = (uint32_t)cnt_big[level];
+    const uint32_t nt = *tcnt;
+    if (nf == 0 && nbig == 0) return;
+    const uint64_t stride = (uint64_t)gridDim.x * T;
+    const uint64_t t0 = (uint64_t)blockIdx.x * T + threadIdx.x;
+    for (uint64_t i = t0; i < nf; i += stride) fr_cur[q[i]] = 0;
+    for (uint64_t i = t0; i < nbig; i += stride) fr_cur[qbig[i]] = 0;
+    if (threadIdx.x == 0) {
+        lq_n = 0;
+        lq_work = 0;
+    }
+    __syncthreads();
+    const uint16_t newlev = (uint16_t)(level + 1);
+    for (uint64_t i0 = (uint64_t)blockIdx.x * T; i0 < nt; i0 += stride) {
+        const uint64_t i = i0 + threadIdx.x;
+        bool small = false, big = false;
+        uint32_t w = 0, len = 0;
+        if (i < nt) {
+            w = touched[i];
+            const uint64_t acc = nx[w];
+            nx[w] = 0;
+            const uint64_t sn = seen[w];
+            const uint64_t fresh = acc & ~sn;
+            if (fresh) {
+                seen[w] = sn | fresh;
+                fr_next[w] = fresh;
+                uint64_t bits = fresh;
+                while (bits) {
+                    const uint32_t s = (uint32_t)__ffsll((unsigned long long)bits) - 1;
+                    bits &= bits - 1;
+                    lv16[(uint64_t)w * 64 + s] = newlev;
+                }
+                len = rlen(ro, w);
+                small = len <= MS_BIG_RANGE;
+                big = !small;
+            }
+        }
+        // short ranges through LDS, long ranges directly
+        const uint64_t ms = __ballot(small);
+        if (ms) {
+            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)ms) - 1;
+            uint32_t b = 0;
+            if (lane_id() == leader) b = atomicAdd(&lq_n, (uint32_t)__popcll(ms));
+            b = __shfl(b, leader, WAVE);
+            if (small) {
+                lq_buf[b + (uint32_t)__popcll(ms & lanemask_lt())] = w;
+                atomicAdd(&lq_work, len);
+            }
+        }
+        const uint64_t mb = __ballot(big);
+        if (mb) {
+            uint64_t wsum = big ? len : 0;
+            for (int d = WAVE / 2; d > 0; d >>= 1) wsum += __shfl_xor(wsum, d, WAVE);
+            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mb) - 1;
+            uint32_t b = 0;
+            if (lane_id() == leader)
+                b = (uint32_t)atomicAdd(cnt_big2, (wsum << 32) | (unsigned long long)__popcll(mb));
+            b = __shfl(b, leader, WAVE);
+            if (big) qbig2[b + (uint32_t)__popcll(mb & lanemask_lt())] = w;
+        }
+        __syncthreads();
+        if (lq_n + T > LQ) {
+            if (threadIdx.x == 0) lq_base = (uint32_t)atomicAdd(cnt2, ((unsigned long long)lq_work << 32) | lq_n);
+            __syncthreads();
+            for (uint32_t j = threadIdx.x; j < lq_n; j += T) q2[lq_base + j] = lq_buf[j];
+            __syncthreads();
+            if (threadIdx.x == 0) lq_n = lq_work = 0;
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    if (lq_n) {
+        if (threadIdx.x == 0) lq_base = (uint32_t)atomicAdd(cnt2, ((unsigned long long)lq_work << 32) | lq_n);
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < lq_n; j += T) q2[lq_base + j] = lq_buf[j];
+    }
+}
+
+// Transpose the vertex-major u16 levels into source-major u32 arrays and
+// count reachable vertices per source (partials per workgroup).
+__global__ __launch_bounds__(T) void k_ms_output(const uint16_t *__restrict__ lv16, uint32_t N, uint32_t nsrc,
+                                                 uint32_t *__restrict__ out, uint32_t *__restrict__ partial) {
+    __shared__ uint16_t tile[64][64 + 2];
+    __shared__ uint32_t reach[64];
+    if (threadIdx.x < 64) reach[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint64_t v0 = (uint64_t)blockIdx.x * 64; v0 < N; v0 += (uint64_t)gridDim.x * 64) {
+        // load 64 vertices x 64 sources (each vertex's 128-byte line)
+        for (uint32_t i = threadIdx.x; i < 64 * 32; i += T) {
+            const uint32_t vv = i / 32, pair = i % 32;
+            uint32_t x = 0;
+            if (v0 + vv < N) x = ((const uint32_t *)(lv16 + (v0 + vv) * 64))[pair];
+            tile[vv][2 * pair] = (uint16_t)x;
+            tile[vv][2 * pair + 1] = (uint16_t)(x >> 16);
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < 64 * 64; i += T) {
+            const uint32_t s = i / 64, vv = i % 64;
+            if (s < nsrc && v0 + vv < N) {
+                const uint32_t l = tile[vv][s];
+                out[(uint64_t)s * N + v0 + vv] = l;
+                if (l) atomicAdd(&reach[s], 1u);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < nsrc) partial[(uint64_t)blockIdx.x * 64 + threadIdx.x] = reach[threadIdx.x];
+}
+
+__global__ void k_ms_reach(const uint32_t *__restrict__ partial, uint32_t nblocks, uint32_t nsrc,
+                           uint32_t *__restrict__ reach) {
+    const uint32_t s = threadIdx.x;
+    if (s >= nsrc) return;
+    uint32_t r = 0;
+    for (uint32_t b = 0; b < nblocks; b++) r += partial[(uint64_t)b * 64 + s];
+    reach[s] = r;
+}
+
+__global__ void k_ms_clear(unsigned long long *__restrict__ cnt, unsigned long long *__restrict__ cnt_big,
+                           uint32_t *__restrict__ tcnt, uint32_t a, uint32_t b) {
+    for (uint32_t i = a + blockIdx.x * blockDim.x + threadIdx.x; i < b; i += gridDim.x * blockDim.x) {
+        cnt[i] = 0;
+        cnt_big[i] = 0;
+        tcnt[i] = 0;
+    }
+}
+
+}  // namespace
+
+void ms_init(const MsState &m, uint32_t N, const uint32_t *starts, uint32_t nsrc, const uint32_t *ro,
+             hipStream_t st) {
+    hipLaunchKernelGGL(k_ms_init, dim3(2048), dim3(256), 0, st, m.seen, m.fr[0], m.fr[1], m.nx, m.lv16, N);
+    hipLaunchKernelGGL(k_ms_sources, dim3(1), dim3(64), 0, st, starts, nsrc, m.seen, m.fr[1], m.lv16, m.q[1],
+                       m.qbig[1], m.cnt, m.cnt_big, m.tcnt, ro);
+}
+
+void ms_clear(const MsState &m, uint32_t a, uint32_t b, hipStream_t st) {
+    if (b <= a) return;
+    hipLaunchKernelGGL(k_ms_clear, dim3(1), dim3(256), 0, st, m.cnt, m.cnt_big, m.tcnt, a, b);
+}
+
+void ms_level(const gkey *keys, const gval *values, uint64_t C, uint32_t N, const uint32_t *ro, const MsState &m,
+              uint32_t level, unsigned long long pull_threshold, hipStream_t st) {
+    const int cur = level & 1, nxt = (level + 1) & 1;
+    const uint32_t small_grid = 1536, big_grid = 512;
+    hipLaunchKernelGGL(k_ms_expand, dim3(small_grid + big_grid), dim3(T), 0, st, keys, values, C, N, ro, m.q[cur],
+                       m.qbig[cur], m.cnt, m.cnt_big, m.fr[cur], m.seen, m.nx, m.touched, m.tcnt + level, level,
+                       pull_threshold, small_grid);
+    hipLaunchKernelGGL(k_ms_finalize, dim3(1024), dim3(T), 0, st, m.q[cur], m.qbig[cur], m.cnt, m.cnt_big, m.fr[cur],
+                       m.fr[nxt], m.seen, m.nx, m.lv16, m.touched, m.tcnt + level, m.q[nxt], m.qbig[nxt],
+                       m.cnt + level + 1, m.cnt_big + level + 1, ro, level);
+}
+
+void ms_output(const MsState &m, uint32_t N, uint32_t nsrc, uint32_t *out, uint32_t *reach, hipStream_t st) {
+    const uint32_t nb = (uint32_t)std::min<uint64_t>(2048, ((uint64_t)N + 63) / 64);
+    hipLaunchKernelGGL(k_ms_output, dim3(std::max(nb, 1u)), dim3(T), 0, st, m.lv16, N, nsrc, out, m.partial);
+    hipLaunchKernelGGL(k_ms_reach, dim3(1), dim3(64), 0, st, m.partial, std::max(nb, 1u), nsrc, reach);
+}
+
+uint32_t ms_output_blocks(uint32_t N) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(2048, ((uint64_t)N + 63) / 64)); }
+
+}  // namespace gpma

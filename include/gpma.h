@@ -90,12 +90,32 @@ int gpma_destroy(gpma_t g);
  * keys within one batch: the last record in input order wins. */
 int gpma_update(gpma_t g, const uint64_t *keys, const uint8_t *values, uint64_t n, int on_device);
 
+/* Several update batches in one call (no reference counterpart; the
+ * batched form): batch b is records [offsets[b], offsets[b+1]) of
+ * keys/values, offsets has nbatches + 1 entries (host), offsets[0] = 0.
+ * The result equals applying the batches one after another with
+ * gpma_update (same live keys): all records go through one sort, locate and
+ * rebalance pass in which the last record of each key in (batch, position)
+ * order decides. */
+int gpma_update_batches(gpma_t g, const uint64_t *keys, const uint8_t *values, const uint64_t *offsets,
+                        uint32_t nbatches, int on_device);
+
 /* gpma_bfs(gpma, node_size, edge_size, start_node, results) (reference
  * gpma_bfs.cuh:517-533): levels[v] = 1 + hop distance from start over live
  * edges, 0 if unreachable; *reachable = #{v : levels[v] != 0}
  * (gpma_bfs_demo.cu:80). levels is a host (on_device = 0) or device
  * (on_device = 1) array of row_num entries; reachable may be NULL. */
 int gpma_bfs(gpma_t g, uint32_t start, uint32_t *levels, int on_device, uint32_t *reachable);
+
+/* Multi-source BFS (no reference counterpart; the batched form's query):
+ * nsrc <= 64 sources (host array starts[nsrc], duplicates allowed) share one
+ * traversal with a 64-bit source mask per vertex. levels is nsrc x row_num
+ * uint32, source-major (levels[i * row_num + v] is what gpma_bfs(starts[i])
+ * writes for v), on the host (on_device = 0) or the device (1); levels may
+ * be NULL. reachable (host, nsrc entries, may be NULL) gets each source's
+ * reachable count. */
+int gpma_bfs_multi(gpma_t g, const uint32_t *starts, uint32_t nsrc, uint32_t *levels, int on_device,
+                   uint32_t *reachable);
 
 /* gpma_bfs(keys, values, row_offsets, node_size, edge_size, start_node,
  * results) (reference gpma_bfs.cuh:511-515): BFS over a caller-provided

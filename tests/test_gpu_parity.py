@@ -197,3 +197,72 @@ def test_bfs_hubs_and_direction_modes(pull_permille):
         lv, reach = G.gpma_bfs(g, start)
         olv, oreach = O.bfs_edges(N, src[keep], dst[keep], start)
         assert reach == oreach and np.array_equal(lv, olv), f"start {start}"
+
+
+@pytest.mark.parametrize("pull_permille", [0, 1, 1000])
+def test_multi_source_bfs_matches_single_source(pull_permille):
+    """64 sources in one traversal: every source's levels and reachable count
+    equal the single-source BFS (and the oracle), including duplicate
+    sources, hubs, tombstoned edges and both expansion modes."""
+    N = 20000
+    rng = np.random.default_rng(5)
+    hubs = np.array([3, 900], np.uint32)
+    src = [np.full(7000, h, np.uint32) for h in hubs]
+    dst = [rng.choice(N, 7000, replace=False).astype(np.uint32) for _ in hubs]
+    s2, d2 = G.gen_rmat(15, 90000, seed=31)
+    m = (s2 < N) & (d2 < N)
+    keys = np.unique(G.make_keys(np.concatenate(src + [s2[m]]), np.concatenate(dst + [d2[m]])))
+    keys = keys[(keys >> np.uint64(32)) != (keys & np.uint64(0xFFFFFFFF))]
+    g = G.GPMA(N, pull_permille=pull_permille)
+    G.update_gpma(g, keys, np.ones(len(keys), np.uint8))
+    G.update_gpma(g, keys[::9], np.zeros(len(keys[::9]), np.uint8))
+    live = np.setdiff1d(keys, keys[::9])
+    srcs = (live >> np.uint64(32)).astype(np.uint32)
+    dsts = (live & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    starts = rng.integers(0, N, 64).astype(np.uint32)
+    starts[5] = starts[6]  # a duplicate source
+    starts[0], starts[1] = 3, 900
+    lv, reach = G.gpma_bfs_multi(g, starts)
+    for i, s in enumerate(starts):
+        one, r1 = G.gpma_bfs(g, int(s))
+        assert np.array_equal(lv[i], one) and reach[i] == r1, f"source {i} ({s})"
+        if i % 8 == 0:
+            olv, _ = O.bfs_edges(N, srcs, dsts, int(s))
+            assert np.array_equal(lv[i], olv)
+    lv2, reach2 = G.gpma_bfs_multi(g, starts[:7])
+    assert np.array_equal(lv2, lv[:7]) and np.array_equal(reach2, reach[:7])
+
+
+def test_fused_batches_equal_sequential():
+    """gpma_update_batches over uneven batches leaves the same live keys (and
+    BFS levels) as applying the batches one by one, including a key inserted
+    in one batch and deleted in a later one, and a delete+reinsert."""
+    N = 4096
+    src, dst = G.gen_rmat(12, 80000, seed=8)
+    keys = G.make_keys(src, dst)
+    half = len(keys) // 2
+    a = G.GPMA(N)
+    b = G.GPMA(N)
+    for g in (a, b):
+        G.update_gpma(g, keys[:half], np.ones(half, np.uint8))
+    rng = np.random.default_rng(3)
+    ins, dele = half, 0
+    bk, bv = [], []
+    for size in rng.integers(10, 3000, 8):
+        k = np.concatenate([keys[ins: ins + size], keys[dele: dele + size]])
+        v = np.concatenate([np.ones(size, np.uint8), np.zeros(size, np.uint8)])
+        ins += size
+        dele += size
+        bk.append(k)
+        bv.append(v)
+    # a key inserted early and deleted later, and one deleted then re-inserted
+    bk.append(np.array([keys[half], keys[0]], np.uint64))
+    bv.append(np.array([0, 1], np.uint8))
+    for k, v in zip(bk, bv):
+        G.update_gpma(a, k, v)
+    G.update_gpma_batches(b, bk, bv)
+    la, lb = a.export_live(), b.export_live()
+    assert np.array_equal(la, lb)
+    assert keys[half] not in set(la.tolist()) and keys[0] in set(la.tolist())
+    for s in (0, 1, 100):
+        assert np.array_equal(G.gpma_bfs(a, s)[0], G.gpma_bfs(b, s)[0])
