@@ -125,6 +125,11 @@ class GPMA:
         """Per-phase HIP-event timing on the handle's stream."""
         check(lib.gpma_profile(self.handle, 2 if (enable and reset) else int(bool(enable))), "gpma_profile")
 
+    def profile_phases(self, names=None):
+        """Time only the named phases (None: all of them)."""
+        mask = (1 << len(PHASES)) - 1 if names is None else sum(1 << PHASES.index(n) for n in names)
+        check(lib.gpma_profile_phases(self.handle, mask), "gpma_profile_phases")
+
     def profile_report(self):
         ms = (ctypes.c_double * len(PHASES))()
         calls = (ctypes.c_uint64 * len(PHASES))()

@@ -98,6 +98,7 @@ SIGNATURES = {
     "gpma_last_error": (ctypes.c_char_p, []),
     "gpma_device_count": (_I, []),
     "gpma_profile": (_I, [_P, _I]),
+    "gpma_profile_phases": (_I, [_P, _U32]),
     "gpma_get_profile": (_I, [_P, _P, _P, _U32]),
     "gpma_device_malloc": (_I, [_P, _U64]),
     "gpma_device_free": (_I, [_P]),

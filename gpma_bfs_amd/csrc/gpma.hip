@@ -103,8 +103,10 @@ enum Ctr : uint32_t {
     C_NEXT = 5,
     C_NRUNS = 6,
     C_LVL = 8,  // C_LVL + L: pending inserts entering level L of the device-driven prefix
-    C_COUNT = 16
+    C_TICKET = 16,  // C_TICKET + k: tile tickets of the k-th compaction of one update
+    C_COUNT = 64
 };
+constexpr uint32_t N_TICKETS = C_COUNT - C_TICKET;
 // Levels 1..SPEC_LEVELS run without a host round trip; the host reads the
 // counters once after them and drives any higher level itself.
 constexpr uint32_t SPEC_LEVELS = 3;
@@ -126,6 +128,9 @@ struct gpma_handle {
     Buf in_k, in_v, sk0, sv0, sk1, sv1, win, blkcnt, blkoff, sortwork;
     Buf pk0, pv0, pw0, pd0, pk1, pv1, pw1, pd1;
     Buf tmpk, tmpv, chunkcnt, chunkoff;
+    Buf ccstat;              // look-back status words of the chained compaction
+    uint32_t cc_epoch = 0;   // tag of the latest compaction call (never 0)
+    uint32_t cc_ticket = 0;  // next ticket slot of this update
     // bfs scratch
     Buf lev, fr0, fr1, fb0, fb1, vis, bcnt, bcntb;
     // multi-source bfs scratch
@@ -139,10 +144,17 @@ struct gpma_handle {
     // stats
     uint64_t applied = 0, invalid = 0, dropped = 0, resizes = 0, bulk = 0;
     uint32_t last_levels = 0;
-    // phase timing (HIP events on st)
+    // phase timing (HIP events on st): events come from a pool; every
+    // begin/end pair is kept until the next stream synchronisation
     bool prof = false;
-    hipEvent_t ev_a[GPMA_PHASES] = {}, ev_b[GPMA_PHASES] = {};
-    bool ev_pending[GPMA_PHASES] = {};
+    uint32_t prof_mask = (1u << GPMA_PHASES) - 1;
+    struct EvPair {
+        int phase;
+        hipEvent_t a, b;
+    };
+    std::vector<hipEvent_t> ev_pool, ev_all;
+    std::vector<EvPair> ev_done;
+    hipEvent_t ev_open[GPMA_PHASES] = {};
     double ph_ms[GPMA_PHASES] = {};
     uint64_t ph_calls[GPMA_PHASES] = {};
 };
@@ -170,24 +182,49 @@ void recalc_density(gpma_handle *g) {
 
 enum Phase : int { P_STAGE = 0, P_SORT, P_LOCATE, P_COMPACT, P_LEAF, P_UPPER, P_REBUILD, P_BFS };
 
+// Events skip the system-scope fence: they only time work on the stream.
+hipEvent_t pool_event(gpma_handle *g) {
+    if (g->ev_pool.empty()) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
+        g->ev_all.push_back(e);
+        return e;
+    }
+    hipEvent_t e = g->ev_pool.back();
+    g->ev_pool.pop_back();
+    return e;
+}
+bool phase_on(const gpma_handle *g, int p) { return g->prof && ((g->prof_mask >> p) & 1u); }
 void phase_begin(gpma_handle *g, int p) {
-    if (g->prof) (void)hipEventRecord(g->ev_a[p], g->st);
+    if (!phase_on(g, p)) return;
+    if (g->ev_open[p]) g->ev_pool.push_back(g->ev_open[p]);
+    hipEvent_t e = pool_event(g);
+    if (e && hipEventRecord(e, g->st) == hipSuccess) g->ev_open[p] = e;
+    else {
+        if (e) g->ev_pool.push_back(e);
+        g->ev_open[p] = nullptr;
+    }
 }
 void phase_end(gpma_handle *g, int p, uint64_t launches) {
-    if (!g->prof) return;
-    (void)hipEventRecord(g->ev_b[p], g->st);
-    g->ev_pending[p] = true;
+    if (!phase_on(g, p) || !g->ev_open[p]) return;
+    hipEvent_t e = pool_event(g);
+    if (!e || hipEventRecord(e, g->st) != hipSuccess) {
+        if (e) g->ev_pool.push_back(e);
+        return;
+    }
+    g->ev_done.push_back({p, g->ev_open[p], e});
+    g->ev_open[p] = nullptr;
     g->ph_calls[p] += launches;
 }
 // After a stream synchronisation: fold finished event pairs into the totals.
 void phase_collect(gpma_handle *g) {
-    if (!g->prof) return;
-    for (int p = 0; p < GPMA_PHASES; p++) {
-        if (!g->ev_pending[p]) continue;
+    for (const auto &r : g->ev_done) {
         float ms = 0;
-        if (hipEventElapsedTime(&ms, g->ev_a[p], g->ev_b[p]) == hipSuccess) g->ph_ms[p] += ms;
-        g->ev_pending[p] = false;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) g->ph_ms[r.phase] += ms;
+        g->ev_pool.push_back(r.a);
+        g->ev_pool.push_back(r.b);
     }
+    g->ev_done.clear();
 }
 
 int stream_sync(gpma_handle *g) {
@@ -321,7 +358,32 @@ int ensure_update_buffers(gpma_handle *g, uint64_t n) {
     TRY(g->pv1.ensure(n, t));
     TRY(g->pw1.ensure(n * 4, t));
     TRY(g->pd1.ensure(n, t));
+    void *before = g->ccstat.p;
+    const uint64_t nstat = compact_status_words(n);
+    TRY(g->ccstat.ensure(nstat * 8, t));
+    if (g->ccstat.p != before) fill_u64(g->ccstat.as<uint64_t>(), 0, g->ccstat.bytes / 8, g->st);
     return GPMA_OK;
+}
+
+// Stable compaction through the chained single-pass kernel; see
+// launch_compact_chained for the arguments.
+void compact_step(gpma_handle *g, const gkey *ik, const gval *iv, const uint32_t *iw, const uint8_t *idone,
+                  uint64_t n_ub, const uint32_t *d_n, gkey *ok, gval *ov, uint32_t *ow, uint8_t *odone, uint32_t mask,
+                  uint32_t *d_out) {
+    uint32_t *ctr = g->ctr.as<uint32_t>();
+    if (g->cc_ticket >= N_TICKETS) {
+        fill_u32(ctr + C_TICKET, 0, N_TICKETS, g->st);
+        g->cc_ticket = 0;
+    }
+    if (++g->cc_epoch == 0) {
+        // the tag wrapped: clear every status word so no stale tag can match
+        fill_u64(g->ccstat.as<uint64_t>(), 0, g->ccstat.bytes / 8, g->st);
+        g->cc_epoch = 1;
+    }
+    launch_compact_chained(ik, iv, iw, idone, n_ub, d_n, ok, ov, ow, odone, mask,
+                           g->ccstat.as<unsigned long long>(), g->cc_epoch, ctr + C_TICKET + g->cc_ticket, d_out,
+                           ctr + C_SORTERR, g->st);
+    g->cc_ticket++;
 }
 
 int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint64_t n, int on_device,
@@ -334,6 +396,7 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
     hipStream_t st = g->st;
     uint32_t *ctr = g->ctr.as<uint32_t>();
     fill_u32(ctr, 0, C_COUNT, st);
+    g->cc_ticket = 0;
     const gkey *src_k = keys;
     const gval *src_v = values;
     if (!on_device) {
@@ -361,25 +424,21 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
     launch_locate(ks, vs, n, g->keys.as<gkey>(), g->values.as<gval>(), g->seps.as<gkey>(), g->C / g->S, g->S,
                   g->win.as<uint32_t>(), g->blkcnt.as<uint32_t>(), st);
     phase_end(g, P_LOCATE, 1);
-    uint64_t nblk = (n + CMP_BLOCK - 1) / CMP_BLOCK;
     Buf *PK[2] = {&g->pk0, &g->pk1}, *PV[2] = {&g->pv0, &g->pv1}, *PW[2] = {&g->pw0, &g->pw1},
         *PD[2] = {&g->pd0, &g->pd1};
     phase_begin(g, P_COMPACT);
-    launch_scan_blocks(g->blkcnt.as<uint32_t>(), nblk, g->blkoff.as<uint32_t>(), ctr + C_LVL, st);
-    launch_compact(ks, vs, g->win.as<uint32_t>(), nullptr, n, nullptr, g->blkoff.as<uint32_t>(), g->pk0.as<gkey>(),
-                   g->pv0.as<gval>(), g->pw0.as<uint32_t>(), g->pd0.as<uint8_t>(), 0xFFFFFFFFu, st);
-    phase_end(g, P_COMPACT, 2);
+    compact_step(g, ks, vs, g->win.as<uint32_t>(), nullptr, n, nullptr, g->pk0.as<gkey>(), g->pv0.as<gval>(),
+                 g->pw0.as<uint32_t>(), g->pd0.as<uint8_t>(), 0xFFFFFFFFu, ctr + C_LVL);
+    phase_end(g, P_COMPACT, 1);
     // Drop placed inserts of list `pc` (count in *d_in), move the rest up to
     // level+1 into the other list, count in *d_out.
     auto compact_pending = [&](int pc, uint64_t n_ub, uint32_t level, const uint32_t *d_in, uint32_t *d_out) {
         const uint32_t mask = ~(uint32_t)(((uint64_t)g->S << (level + 1)) - 1);
         phase_begin(g, P_COMPACT);
-        launch_count_pending(PD[pc]->as<uint8_t>(), n_ub, d_in, g->blkcnt.as<uint32_t>(), st);
-        launch_scan_blocks(g->blkcnt.as<uint32_t>(), 0, g->blkoff.as<uint32_t>(), d_out, st, d_in);
-        launch_compact(PK[pc]->as<gkey>(), PV[pc]->as<gval>(), PW[pc]->as<uint32_t>(), PD[pc]->as<uint8_t>(), n_ub,
-                       d_in, g->blkoff.as<uint32_t>(), PK[pc ^ 1]->as<gkey>(), PV[pc ^ 1]->as<gval>(),
-                       PW[pc ^ 1]->as<uint32_t>(), PD[pc ^ 1]->as<uint8_t>(), mask, st);
-        phase_end(g, P_COMPACT, 3);
+        compact_step(g, PK[pc]->as<gkey>(), PV[pc]->as<gval>(), PW[pc]->as<uint32_t>(), PD[pc]->as<uint8_t>(), n_ub,
+                     d_in, PK[pc ^ 1]->as<gkey>(), PV[pc ^ 1]->as<gval>(), PW[pc ^ 1]->as<uint32_t>(),
+                     PD[pc ^ 1]->as<uint8_t>(), mask, d_out);
+        phase_end(g, P_COMPACT, 1);
     };
     // Device-driven prefix: level 0 and levels 1..spec, launched before the
     // host knows any pending count; each kernel reads its count on the device,
@@ -404,7 +463,7 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
         spec++;
     }
     TRY(sync_read_ctr(g));
-    if (g->hctr[C_SORTERR]) return fail(GPMA_ERR_HIP, "radix sort look-back exceeded its spin limit");
+    if (g->hctr[C_SORTERR]) return fail(GPMA_ERR_HIP, "sort or compaction look-back exceeded its spin limit");
     const uint64_t inval = g->hctr[C_INVALID];
     g->invalid += inval;
     g->applied += n - inval;
@@ -473,6 +532,7 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
         }
         compact_pending(pc, np, level, d_np, ctr + npslot[pc ^ 1]);
         TRY(sync_read_ctr(g));
+        if (g->hctr[C_SORTERR]) return fail(GPMA_ERR_HIP, "compaction look-back exceeded its spin limit");
         pc ^= 1;
         np = g->hctr[npslot[pc]];
         level++;
@@ -643,20 +703,21 @@ const char *gpma_last_error(void) { return g_last_error.c_str(); }
 int gpma_profile(gpma_t g, int enable) {
     if (!g) return fail(GPMA_ERR_ARG, "gpma_profile: null handle");
     HIP_TRY(hipSetDevice(g->device));
-    if (enable && !g->ev_a[0]) {
-        for (int p = 0; p < GPMA_PHASES; p++) {
-            HIP_TRY(hipEventCreate(&g->ev_a[p]));
-            HIP_TRY(hipEventCreate(&g->ev_b[p]));
-        }
-    }
     if (enable == 2) {
+        HIP_TRY(hipStreamSynchronize(g->st));
+        phase_collect(g);
         for (int p = 0; p < GPMA_PHASES; p++) {
             g->ph_ms[p] = 0;
             g->ph_calls[p] = 0;
-            g->ev_pending[p] = false;
         }
     }
     g->prof = enable != 0;
+    return GPMA_OK;
+}
+
+int gpma_profile_phases(gpma_t g, uint32_t mask) {
+    if (!g) return fail(GPMA_ERR_ARG, "gpma_profile_phases: null handle");
+    g->prof_mask = mask & ((1u << GPMA_PHASES) - 1);
     return GPMA_OK;
 }
 
@@ -757,10 +818,8 @@ int gpma_destroy(gpma_t g) {
     if (g->hctr) (void)hipHostFree(g->hctr);
     if (g->hcnt) (void)hipHostFree(g->hcnt);
     if (g->ms_hcnt) (void)hipHostFree(g->ms_hcnt);
-    for (int p = 0; p < GPMA_PHASES; p++) {
-        if (g->ev_a[p]) (void)hipEventDestroy(g->ev_a[p]);
-        if (g->ev_b[p]) (void)hipEventDestroy(g->ev_b[p]);
-    }
+    if (g->st) (void)hipStreamSynchronize(g->st);
+    for (hipEvent_t e : g->ev_all) (void)hipEventDestroy(e);
     if (g->st) hipStreamDestroy(g->st);
     delete g;
     return GPMA_OK;

@@ -49,6 +49,17 @@ void launch_compact(const gkey *ik, const gval *iv, const uint32_t *iw, const ui
                     const uint32_t *d_n, const uint32_t *off, gkey *ok, gval *ov, uint32_t *ow, uint8_t *odone,
                     uint32_t mask, hipStream_t st);
 // Per-chunk counts of entries with done == 0 among the first *d_n.
+// Single-pass stable compaction of the first *d_n entries (n_ub when d_n is
+// null) keeping done == 0 (idone given) or win != SIZE_NONE; writes win & mask,
+// clears the output done flags and stores the kept count in *d_out. status:
+// compact_status_words(n_ub) words, zeroed once at allocation; epoch: non-zero
+// and distinct per call; *ticket zero before the call; *err set if the
+// look-back spin limit is exceeded.
+uint64_t compact_status_words(uint64_t n_ub);
+void launch_compact_chained(const gkey *ik, const gval *iv, const uint32_t *iw, const uint8_t *idone, uint64_t n_ub,
+                            const uint32_t *d_n, gkey *ok, gval *ov, uint32_t *ow, uint8_t *odone, uint32_t mask,
+                            unsigned long long *status, uint32_t epoch, uint32_t *ticket, uint32_t *d_out,
+                            uint32_t *err, hipStream_t st);
 void launch_count_pending(const uint8_t *done, uint64_t n_ub, const uint32_t *d_n, uint32_t *blkcnt,
                           hipStream_t st);
 // Level-0 rebalance: one thread per leaf run; does nothing when *d_np >=

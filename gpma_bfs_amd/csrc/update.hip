@@ -159,6 +159,121 @@ __global__ __launch_bounds__(CMP_BLOCK) void k_compact(const gkey *__restrict__ 
     }
 }
 
+// Single-pass stable compaction (chained scan with decoupled look-back):
+// tiles of CC_TILE entries take tickets in launch order, count their kept
+// entries with wave ballots, publish the count, add the counts of earlier
+// tiles (read 64 at a time by one wave), publish the inclusive prefix and
+// scatter. Status words carry the caller's epoch in their high half, so no
+// clearing pass is needed between calls; tiles past *d_n exit at once.
+constexpr uint32_t CC_THREADS = 256;
+constexpr uint32_t CC_ROWS = 8;                        // 64-entry rows per wave
+constexpr uint32_t CC_WAVE_ITEMS = WAVE * CC_ROWS;     // 512
+constexpr uint32_t CC_TILE = CC_THREADS * CC_ROWS;     // 2048
+constexpr uint64_t CC_AGG = 1ull << 30, CC_INC = 2ull << 30, CC_CNT = (1ull << 30) - 1;
+constexpr uint32_t CC_SPIN_LIMIT = 1u << 22;
+
+__global__ __launch_bounds__(CC_THREADS) void k_compact_chained(
+    const gkey *__restrict__ ik, const gval *__restrict__ iv, const uint32_t *__restrict__ iw,
+    const uint8_t *__restrict__ idone, uint64_t n_ub, const uint32_t *__restrict__ d_n, gkey *__restrict__ ok,
+    gval *__restrict__ ov, uint32_t *__restrict__ ow, uint8_t *__restrict__ odone, uint32_t mask,
+    unsigned long long *__restrict__ status, uint32_t epoch, uint32_t *__restrict__ ticket,
+    uint32_t *__restrict__ d_out, uint32_t *__restrict__ err) {
+    __shared__ uint32_t s_tile, s_excl;
+    __shared__ uint32_t wtot[CC_THREADS / WAVE];
+    const uint64_t n = d_n ? std::min<uint64_t>(*d_n, n_ub) : n_ub;
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint64_t base = (uint64_t)tile * CC_TILE;
+    if (base >= n) {
+        if (tile == 0 && threadIdx.x == 0) *d_out = 0;
+        return;
+    }
+    const uint32_t wid = threadIdx.x / WAVE, lane = lane_id();
+    const uint64_t wbase = base + (uint64_t)wid * CC_WAVE_ITEMS;
+    uint64_t km[CC_ROWS];
+    uint32_t wcount = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < CC_ROWS; j++) {
+        const uint64_t i = wbase + (uint64_t)j * WAVE + lane;
+        bool keep = false;
+        if (i < n) keep = idone ? (idone[i] == 0) : (iw[i] != SIZE_NONE);
+        km[j] = ballot(keep);
+        wcount += (uint32_t)__popcll(km[j]);
+    }
+    if (lane == 0) wtot[wid] = wcount;
+    __syncthreads();
+    uint32_t tile_total = 0, wpre = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < CC_THREADS / WAVE; w++) {
+        if (w < wid) wpre += wtot[w];
+        tile_total += wtot[w];
+    }
+    if (wid == 0) {
+        const uint64_t tag = (uint64_t)epoch << 32;
+        if (lane == 0)
+            __hip_atomic_store(status + tile, tag | (tile == 0 ? CC_INC : CC_AGG) | tile_total, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t excl = 0;
+        if (tile > 0) {
+            int64_t t = (int64_t)tile - 1;  // next predecessor not yet added
+            uint32_t spins = 0;
+            while (true) {
+                const int64_t q = t - (int64_t)lane;
+                unsigned long long w = 0;
+                if (q >= 0)
+                    w = __hip_atomic_load(status + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const bool mine = q >= 0 && (w >> 32) == epoch;
+                const uint64_t fl = w & ~(CC_CNT | (0xFFFFFFFFull << 32));
+                const bool ready = q < 0 || (mine && fl != 0);
+                const bool inc = q < 0 || (mine && fl == CC_INC);
+                const uint64_t notready = ballot(!ready);
+                const uint64_t incm = ballot(inc);
+                // lanes before the first not-ready lane are usable; stop at the first INC among them
+                const uint64_t usable = notready ? ((notready & (~notready + 1)) - 1) : ~0ull;
+                const uint64_t stop = incm & usable;
+                const uint64_t take = stop ? (((stop & (~stop + 1)) << 1) - 1) : usable;
+                const uint32_t c = ((take >> lane) & 1ull) && q >= 0 ? (uint32_t)(w & CC_CNT) : 0u;
+                // wave sum of c
+                uint32_t sum = c;
+                for (uint32_t off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, WAVE);
+                excl += sum;
+                if (stop) break;
+                const uint32_t used = (uint32_t)__popcll(take);
+                t -= used;
+                if (used == 0) {
+                    if (++spins > CC_SPIN_LIMIT) {
+                        if (lane == 0) atomicOr(err, 1u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            if (lane == 0)
+                __hip_atomic_store(status + tile, tag | CC_INC | (excl + tile_total), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (lane == 0) {
+            s_excl = excl;
+            if (base + CC_TILE >= n) *d_out = excl + tile_total;
+        }
+    }
+    __syncthreads();
+    uint32_t o = s_excl + wpre;
+#pragma unroll
+    for (uint32_t j = 0; j < CC_ROWS; j++) {
+        if ((km[j] >> lane) & 1ull) {
+            const uint64_t i = wbase + (uint64_t)j * WAVE + lane;
+            const uint64_t d = (uint64_t)o + (uint64_t)__popcll(km[j] & lanemask_lt());
+            ok[d] = ik[i];
+            ov[d] = iv[i];
+            ow[d] = iw[i] & mask;
+            odone[d] = 0;
+        }
+        o += (uint32_t)__popcll(km[j]);
+    }
+}
+
 // Level 0 (window = one leaf of S slots), one thread per run of pending
 // inserts that share a leaf. With f = W/S = 1 the redistribution packs the
 // merged items at the front of the leaf (gpma.cuh:636-654 with f = 1).
@@ -520,6 +635,17 @@ void launch_compact(const gkey *ik, const gval *iv, const uint32_t *iw, const ui
     uint64_t nb = (n_ub + CMP_BLOCK - 1) / CMP_BLOCK;
     hipLaunchKernelGGL(k_compact, dim3((uint32_t)std::min<uint64_t>(nb, 2048)), dim3(CMP_BLOCK), 0, st, ik, iv, iw,
                        idone, n_ub, d_n, off, ok, ov, ow, odone, mask);
+}
+
+uint64_t compact_status_words(uint64_t n_ub) { return (n_ub + CC_TILE - 1) / CC_TILE + 1; }
+
+void launch_compact_chained(const gkey *ik, const gval *iv, const uint32_t *iw, const uint8_t *idone, uint64_t n_ub,
+                            const uint32_t *d_n, gkey *ok, gval *ov, uint32_t *ow, uint8_t *odone, uint32_t mask,
+                            unsigned long long *status, uint32_t epoch, uint32_t *ticket, uint32_t *d_out,
+                            uint32_t *err, hipStream_t st) {
+    const uint64_t nt = std::max<uint64_t>(1, (n_ub + CC_TILE - 1) / CC_TILE);
+    hipLaunchKernelGGL(k_compact_chained, dim3((uint32_t)nt), dim3(CC_THREADS), 0, st, ik, iv, iw, idone, n_ub, d_n,
+                       ok, ov, ow, odone, mask, status, epoch, ticket, d_out, err);
 }
 
 void launch_count_pending(const uint8_t *done, uint64_t n_ub, const uint32_t *d_n, uint32_t *blkcnt,

@@ -154,6 +154,10 @@ void *gpma_stream(gpma_t g);
  * gpma_profile(g, 2) resets the counters and enables. */
 #define GPMA_PHASES 8
 int gpma_profile(gpma_t g, int enable);
+/* Restrict phase timing to the phases whose bit (1 << phase) is set in mask
+ * (default: all). Each timed phase adds two event records to the stream, so
+ * a caller timing one kernel group masks the others out. */
+int gpma_profile_phases(gpma_t g, uint32_t mask);
 int gpma_get_profile(gpma_t g, double *ms, uint64_t *calls, uint32_t n);
 
 /* Device-memory plumbing for callers without their own allocator (tests,
