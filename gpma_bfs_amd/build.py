@@ -21,7 +21,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]
-HIP_SOURCES = ["sort.hip", "update.hip", "bfs.hip", "msbfs.hip", "gpma.hip"]
+HIP_SOURCES = ["sort.hip", "msd.hip", "update.hip", "bfs.hip", "msbfs.hip", "gpma.hip"]
 CXX_SOURCES = ["gen.cpp"]
 
 

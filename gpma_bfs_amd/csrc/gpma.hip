@@ -102,11 +102,18 @@ enum Ctr : uint32_t {
     C_LIVE = 4,
     C_NEXT = 5,
     C_NRUNS = 6,
+    C_MSD_TICKET = 14,  // scan-block tickets of the bucket sort
+    C_MSD_MAXB = 15,    // largest bucket above 512 items in the bucket sort
     C_LVL = 8,  // C_LVL + L: pending inserts entering level L of the device-driven prefix
     C_TICKET = 16,  // C_TICKET + k: tile tickets of the k-th compaction of one update
     C_COUNT = 64
 };
 constexpr uint32_t N_TICKETS = C_COUNT - C_TICKET;
+// Bucket sort for batches up to this size (its count matrix is 8 bytes per
+// item); larger batches and row counts >= 2^28 use the LSD passes.
+constexpr uint64_t MSD_MAX_N = 4ull << 20;
+constexpr uint32_t MSD_BIG_BUCKET = 16384;
+constexpr uint32_t MSD_COOL_BATCHES = 64;
 // Levels 1..SPEC_LEVELS run without a host round trip; the host reads the
 // counters once after them and drives any higher level itself.
 constexpr uint32_t SPEC_LEVELS = 3;
@@ -129,6 +136,8 @@ struct gpma_handle {
     Buf pk0, pv0, pw0, pd0, pk1, pv1, pw1, pd1;
     Buf tmpk, tmpv, chunkcnt, chunkoff;
     Buf ccstat;              // look-back status words of the chained compaction
+    Buf msd_kp, msd_rk, msd_mat, msd_bstart, msd_status;  // bucket sort scratch
+    uint32_t msd_cool = 0;   // batches left on the LSD sort after a skewed batch
     uint32_t cc_epoch = 0;   // tag of the latest compaction call (never 0)
     uint32_t cc_ticket = 0;  // next ticket slot of this update
     // bfs scratch
@@ -365,6 +374,17 @@ int ensure_update_buffers(gpma_handle *g, uint64_t n) {
     return GPMA_OK;
 }
 
+// Next look-back tag; on wrap-around every status word is cleared so no
+// stale tag can match.
+uint32_t next_epoch(gpma_handle *g) {
+    if (++g->cc_epoch == 0) {
+        fill_u64(g->ccstat.as<uint64_t>(), 0, g->ccstat.bytes / 8, g->st);
+        if (g->msd_status.p) fill_u64(g->msd_status.as<uint64_t>(), 0, g->msd_status.bytes / 8, g->st);
+        g->cc_epoch = 1;
+    }
+    return g->cc_epoch;
+}
+
 // Stable compaction through the chained single-pass kernel; see
 // launch_compact_chained for the arguments.
 void compact_step(gpma_handle *g, const gkey *ik, const gval *iv, const uint32_t *iw, const uint8_t *idone,
@@ -375,15 +395,23 @@ void compact_step(gpma_handle *g, const gkey *ik, const gval *iv, const uint32_t
         fill_u32(ctr + C_TICKET, 0, N_TICKETS, g->st);
         g->cc_ticket = 0;
     }
-    if (++g->cc_epoch == 0) {
-        // the tag wrapped: clear every status word so no stale tag can match
-        fill_u64(g->ccstat.as<uint64_t>(), 0, g->ccstat.bytes / 8, g->st);
-        g->cc_epoch = 1;
-    }
+    const uint32_t epoch = next_epoch(g);
     launch_compact_chained(ik, iv, iw, idone, n_ub, d_n, ok, ov, ow, odone, mask,
-                           g->ccstat.as<unsigned long long>(), g->cc_epoch, ctr + C_TICKET + g->cc_ticket, d_out,
+                           g->ccstat.as<unsigned long long>(), epoch, ctr + C_TICKET + g->cc_ticket, d_out,
                            ctr + C_SORTERR, g->st);
     g->cc_ticket++;
+}
+
+int ensure_msd(gpma_handle *g, uint64_t n, const MsdPlan &pl) {
+    uint64_t *t = &g->dev_bytes;
+    TRY(g->msd_kp.ensure(n * 8, t));
+    TRY(g->msd_rk.ensure(n * 2, t));
+    TRY(g->msd_mat.ensure(2ull * pl.ntiles * pl.NB * 4, t));
+    TRY(g->msd_bstart.ensure(((uint64_t)pl.NB + 1) * 4, t));
+    void *before = g->msd_status.p;
+    TRY(g->msd_status.ensure((uint64_t)pl.nscan * 8, t));
+    if (g->msd_status.p != before) fill_u64(g->msd_status.as<uint64_t>(), 0, g->msd_status.bytes / 8, g->st);
+    return GPMA_OK;
 }
 
 int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint64_t n, int on_device,
@@ -413,16 +441,29 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
     for (uint32_t d = 0; d < dpw; d++) sh.s[npass++] = 8 * d;
     for (uint32_t d = 0; d < dpw; d++) sh.s[npass++] = 32 + 8 * d;
     phase_begin(g, P_SORT);
-    int cur = stage_and_sort(src_k, src_v, g->sk0.as<gkey>(), g->sv0.as<gval>(), g->sk1.as<gkey>(), g->sv1.as<gval>(),
-                             n, g->N, guards_ok, ctr + C_INVALID, sh, npass, g->sortwork.as<uint32_t>(),
-                             ctr + C_SORTERR, st);
-    phase_end(g, P_SORT, npass + 2);
+    int cur = 0;
+    const bool use_msd = msd_sort_fits(g->N) && n <= MSD_MAX_N && g->msd_cool == 0;
+    if (g->msd_cool) g->msd_cool--;
+    if (use_msd) {
+        const MsdPlan pl = msd_plan(n, g->N);
+        TRY(ensure_msd(g, n, pl));
+        MsdWork w{g->msd_kp.as<uint64_t>(), g->msd_rk.as<uint16_t>(), g->sk1.as<uint64_t>(), g->msd_mat.as<uint32_t>(),
+                  g->msd_bstart.as<uint32_t>(), g->msd_status.as<unsigned long long>()};
+        msd_sort(src_k, src_v, g->sk0.as<gkey>(), g->sv0.as<gval>(), pl, w, n, g->N, guards_ok, ctr + C_INVALID,
+                 next_epoch(g), ctr + C_MSD_TICKET, ctr + C_SORTERR, ctr + C_MSD_MAXB, st);
+        phase_end(g, P_SORT, 4);
+    } else {
+        cur = stage_and_sort(src_k, src_v, g->sk0.as<gkey>(), g->sv0.as<gval>(), g->sk1.as<gkey>(), g->sv1.as<gval>(), n,
+                             g->N, guards_ok, ctr + C_INVALID, sh, npass, g->sortwork.as<uint32_t>(), ctr + C_SORTERR,
+                             st);
+        phase_end(g, P_SORT, npass + 2);
+    }
     const gkey *ks = cur ? g->sk1.as<gkey>() : g->sk0.as<gkey>();
     const gval *vs = cur ? g->sv1.as<gval>() : g->sv0.as<gval>();
     // locate + in-place delete / modify
     phase_begin(g, P_LOCATE);
     launch_locate(ks, vs, n, g->keys.as<gkey>(), g->values.as<gval>(), g->seps.as<gkey>(), g->C / g->S, g->S,
-                  g->win.as<uint32_t>(), g->blkcnt.as<uint32_t>(), st);
+                  g->row_offset.as<uint32_t>(), !guards_ok, g->win.as<uint32_t>(), st);
     phase_end(g, P_LOCATE, 1);
     Buf *PK[2] = {&g->pk0, &g->pk1}, *PV[2] = {&g->pv0, &g->pv1}, *PW[2] = {&g->pw0, &g->pw1},
         *PD[2] = {&g->pd0, &g->pd1};
@@ -464,6 +505,9 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
     }
     TRY(sync_read_ctr(g));
     if (g->hctr[C_SORTERR]) return fail(GPMA_ERR_HIP, "sort or compaction look-back exceeded its spin limit");
+    // a batch with a bucket this large sorts faster with the LSD passes: use
+    // them for the next batches of this handle
+    if (use_msd && g->hctr[C_MSD_MAXB] > MSD_BIG_BUCKET) g->msd_cool = MSD_COOL_BATCHES;
     const uint64_t inval = g->hctr[C_INVALID];
     g->invalid += inval;
     g->applied += n - inval;
@@ -808,7 +852,8 @@ int gpma_destroy(gpma_t g) {
     hipSetDevice(g->device);
     if (g->st) hipStreamSynchronize(g->st);
     Buf *all[] = {&g->keys, &g->values, &g->seps, &g->row_offset, &g->in_k, &g->in_v, &g->sk0, &g->sv0,
-                  &g->sk1, &g->sv1, &g->win, &g->blkcnt, &g->blkoff, &g->sortwork, &g->pk0, &g->pv0,
+                  &g->sk1, &g->sv1, &g->win, &g->blkcnt, &g->blkoff, &g->sortwork, &g->msd_kp, &g->msd_rk,
+                  &g->msd_mat, &g->msd_bstart, &g->msd_status, &g->ccstat, &g->pk0, &g->pv0,
                   &g->pw0, &g->pd0, &g->pk1, &g->pv1, &g->pw1, &g->pd1, &g->tmpk, &g->tmpv, &g->chunkcnt,
                   &g->chunkoff, &g->lev, &g->fr0, &g->fr1, &g->fb0, &g->fb1, &g->vis, &g->bcnt, &g->bcntb,
                   &g->ms_seen, &g->ms_fr0, &g->ms_fr1, &g->ms_nx, &g->ms_lv16, &g->ms_q0, &g->ms_q1, &g->ms_qb0,

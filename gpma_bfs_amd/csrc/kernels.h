@@ -29,15 +29,45 @@ int stage_and_sort(const gkey *ik, const gval *iv, gkey *k0, gval *v0, gkey *k1,
                    bool guards_ok, uint32_t *invalid, const SortShifts &shifts, uint32_t npass, uint32_t *work,
                    uint32_t *d_err, hipStream_t st);
 
+// ---- msd.hip
+// Bucket sort for row_num < 2^28 (keys and values packed into one u64).
+bool msd_sort_fits(uint32_t N);
+struct MsdPlan {
+    uint32_t bs;      // bits per packed src / dst field
+    uint32_t D;       // bucket bits
+    uint32_t R;       // bits sorted inside a bucket
+    uint32_t bshift;  // shift of the bucket bits in a packed key
+    uint32_t ntiles;  // 4096-item tiles of the batch
+    uint32_t NB;      // buckets
+    uint32_t nscan;   // scan blocks (status words used)
+};
+MsdPlan msd_plan(uint64_t n, uint32_t N);
+struct MsdWork {
+    uint64_t *kp;               // n packed keys
+    uint16_t *rk;               // n ranks
+    uint64_t *kb;               // n bucketed packed keys
+    uint32_t *mat;              // 2 * ntiles * NB counts and offsets
+    uint32_t *bstart;           // NB + 1 bucket starts
+    unsigned long long *status; // nscan look-back words (zeroed once)
+};
+// Sorted (ok, ov) from (ik, iv); same contract as stage_and_sort. epoch:
+// non-zero tag distinct from earlier calls on the same status words;
+// *ticket zero before the call; *maxb raised to the largest bucket above 512
+// items.
+void msd_sort(const gkey *ik, const gval *iv, gkey *ok, gval *ov, const MsdPlan &pl, const MsdWork &w, uint64_t n,
+              uint32_t N, bool guards_ok, uint32_t *invalid, uint32_t epoch, uint32_t *ticket, uint32_t *err,
+              uint32_t *maxb, hipStream_t st);
+
 // ---- update.hip
 constexpr uint32_t CMP_BLOCK = 256;  // items per compaction block
 constexpr uint32_t WIN_MAX = 4096;   // widest window rebalanced by one workgroup in LDS
 constexpr uint32_t CHUNK = 4096;     // slots per workgroup in device-wide window passes
 
-// Locate + in-place delete/modify. win[i] = leaf base or SIZE_NONE; per-block
-// pending counts into blkcnt[ceil(n/CMP_BLOCK)].
+// Locate + in-place delete/modify. win[i] = leaf base or SIZE_NONE. With
+// row_hint the search starts from row_offset (valid guards in the array).
 void launch_locate(const gkey *ks, const gval *vs, uint64_t n, gkey *keys, gval *values, const gkey *seps,
-                   uint64_t nseg, uint32_t S, uint32_t *win, uint32_t *blkcnt, hipStream_t st);
+                   uint64_t nseg, uint32_t S, const uint32_t *row_offset, bool row_hint, uint32_t *win,
+                   hipStream_t st);
 // Exclusive scan of per-chunk counts (one workgroup); *total gets the sum.
 // The chunk count is nb, or ceil(*d_items / CMP_BLOCK) when d_items is given.
 void launch_scan_blocks(const uint32_t *cnt, uint64_t nb, uint32_t *off, uint32_t *total, hipStream_t st,

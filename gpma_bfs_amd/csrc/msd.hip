@@ -1,0 +1,350 @@
+// Bucket (MSD) sort of an update batch whose keys pack into 64 bits.
+//
+// Reference counterpart: thrust::sort_by_key(update_keys, update_values)
+// (gpma.cuh:977). Same result as the LSD sort in sort.hip (stable order by
+// the full key), by a different route that suits batches of 10^4..10^6:
+//
+//   pack      a valid key (src, dst) with value v becomes
+//             src << (bs + 8) | dst << 8 | v, bs = bit width of row_num, so
+//             one u64 carries key and value; a guard's dst (0xFFFFFFFF)
+//             packs as row_num and an out-of-range key as all ones, which
+//             keeps the order of the unpacked keys.
+//   rank      (k_msd_rank) each 4096-item tile ranks its items by bucket =
+//             the top D bits of the packed key (wave ballots), writes the
+//             packed keys, their rank inside (tile, bucket) and the tile's
+//             bucket counts;
+//   scan      (k_msd_scan) exclusive scan of the counts in (bucket, tile)
+//             order, chained across blocks of 64 buckets;
+//   scatter   (k_msd_scatter) every item to its bucket's range, stable;
+//   local     (k_msd_local) one workgroup per bucket sorts the bucket by the
+//             remaining bits: comparison ranking in LDS up to 512 items,
+//             else LSD passes with 8-bit digits over the bucket's range, and
+//             writes the unpacked keys and values.
+//
+// No pass waits on a chain of tiles longer than the 64 scan blocks, so the
+// whole sort is four launches whatever the key width.
+#include "common.h"
+#include "kernels.h"
+
+namespace gpma {
+
+namespace {
+
+constexpr uint32_t MT = 256;                 // threads per workgroup
+constexpr uint32_t MW = MT / WAVE;           // waves per workgroup
+constexpr uint32_t MIPT = 16;                // items per lane
+constexpr uint32_t MWI = WAVE * MIPT;        // items per wave
+constexpr uint32_t MTILE = MT * MIPT;        // items per tile
+constexpr uint32_t MAX_D = 12;               // bucket bits
+constexpr uint32_t MAX_NB = 1u << MAX_D;
+constexpr uint32_t SMALL = 512;              // buckets ranked by comparison in LDS
+constexpr uint32_t SCAN_B = WAVE;            // buckets per scan block
+
+struct PackInfo {
+    uint32_t N;          // row_num
+    uint32_t bs;         // bits per packed src / dst field
+    uint32_t guards_ok;  // dst == 0xFFFFFFFF accepted (guard keys)
+};
+
+__device__ __forceinline__ uint64_t pack_key(gkey k, gval v, const PackInfo &pi, bool &bad) {
+    const uint32_t src = (uint32_t)(k >> 32), dst = (uint32_t)k;
+    const bool guard = pi.guards_ok && dst == COL_NONE;
+    bad = src >= pi.N || (dst >= pi.N && !guard);
+    if (bad) return ~0ull >> (56 - 2 * pi.bs);
+    return ((uint64_t)src << (pi.bs + 8)) | ((uint64_t)(guard ? pi.N : dst) << 8) | v;
+}
+
+__device__ __forceinline__ void unpack_key(uint64_t p, const PackInfo &pi, gkey &k, gval &v) {
+    const uint32_t src = (uint32_t)(p >> (pi.bs + 8));
+    const uint32_t dst = (uint32_t)((p >> 8) & ((1ull << pi.bs) - 1));
+    v = (gval)p;
+    k = src >= pi.N ? KEY_NONE : (((uint64_t)src << 32) | (dst == pi.N ? COL_NONE : dst));
+}
+
+__global__ __launch_bounds__(MT) void k_msd_rank(const gkey *__restrict__ ik, const gval *__restrict__ iv, uint64_t n,
+                                                 PackInfo pi, uint32_t bshift, uint32_t D,
+                                                 uint64_t *__restrict__ kp, uint16_t *__restrict__ rk,
+                                                 uint32_t *__restrict__ counts, uint32_t *__restrict__ invalid) {
+    __shared__ uint16_t cnt[MW][MAX_NB];
+    const uint32_t NB = 1u << D;
+    for (uint32_t i = threadIdx.x; i < MW * MAX_NB / 2; i += MT) reinterpret_cast<uint32_t *>(&cnt[0][0])[i] = 0;
+    const uint32_t wid = threadIdx.x / WAVE, lane = lane_id();
+    const uint64_t wbase = (uint64_t)blockIdx.x * MTILE + (uint64_t)wid * MWI;
+    gkey k[MIPT];
+    gval v[MIPT];
+#pragma unroll
+    for (uint32_t j = 0; j < MIPT; j++) {
+        const uint64_t idx = wbase + (uint64_t)j * WAVE + lane;
+        k[j] = idx < n ? ik[idx] : 0;
+        v[j] = idx < n ? iv[idx] : 0;
+    }
+    uint64_t p[MIPT];
+    uint32_t b[MIPT];
+    uint32_t nbad = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < MIPT; j++) {
+        const uint64_t idx = wbase + (uint64_t)j * WAVE + lane;
+        bool bad = false;
+        p[j] = 0;
+        b[j] = NB;  // no item
+        if (idx < n) {
+            p[j] = pack_key(k[j], v[j], pi, bad);
+            b[j] = (uint32_t)(p[j] >> bshift) & (NB - 1);
+        }
+        nbad += (uint32_t)__popcll(ballot(bad));
+    }
+    if (lane == 0 && nbad) atomicAdd(invalid, nbad);
+    __syncthreads();  // counters zeroed
+    uint32_t r[MIPT];
+#pragma unroll
+    for (uint32_t j = 0; j < MIPT; j++) r[j] = wave_rank<MAX_D>(b[j], b[j] < NB, cnt[wid]);
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < NB; d += MT) {
+        uint32_t run = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < MW; w++) {
+            const uint32_t c = cnt[w][d];
+            cnt[w][d] = (uint16_t)run;
+            run += c;
+        }
+        counts[(uint64_t)blockIdx.x * NB + d] = run;
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t j = 0; j < MIPT; j++) {
+        const uint64_t idx = wbase + (uint64_t)j * WAVE + lane;
+        if (idx < n) {
+            kp[idx] = p[j];
+            rk[idx] = (uint16_t)(cnt[wid][b[j]] + r[j]);
+        }
+    }
+}
+
+// offs[t][b] = items in buckets < b + items of bucket b in tiles < t;
+// bstart[b] = offs[0][b], bstart[NB] = n. A block covers SCAN_B buckets with
+// four tile groups; blocks chain their bucket totals by look-back.
+__global__ __launch_bounds__(MT) void k_msd_scan(const uint32_t *__restrict__ counts, uint32_t ntiles, uint32_t NB,
+                                                 uint64_t n, uint32_t *__restrict__ offs,
+                                                 uint32_t *__restrict__ bstart, unsigned long long *status,
+                                                 uint32_t epoch, uint32_t *__restrict__ ticket,
+                                                 uint32_t *__restrict__ err) {
+    __shared__ uint32_t gs[MW][SCAN_B];
+    __shared__ uint32_t base_s[SCAN_B];
+    __shared__ uint32_t s_blk;
+    if (threadIdx.x == 0) s_blk = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t blk = s_blk;
+    const uint32_t g = threadIdx.x / SCAN_B, bl = threadIdx.x % SCAN_B;
+    const uint32_t bkt = blk * SCAN_B + bl;
+    const uint32_t tg = (ntiles + MW - 1) / MW;
+    const uint32_t t0 = min(ntiles, g * tg), t1 = min(ntiles, t0 + tg);
+    uint32_t sum = 0;
+    if (bkt < NB) {
+#pragma unroll 8
+        for (uint32_t t = t0; t < t1; t++) sum += counts[(uint64_t)t * NB + bkt];
+    }
+    gs[g][bl] = sum;
+    __syncthreads();
+    uint32_t gpre = 0, col = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < MW; q++) {
+        if (q < g) gpre += gs[q][bl];
+        col += gs[q][bl];
+    }
+    if (g == 0) {
+        // wave 0: bucket-exclusive prefix inside the block, block total, look-back
+        uint32_t v = col;
+#pragma unroll
+        for (uint32_t d = 1; d < WAVE; d <<= 1) {
+            const uint32_t y = __shfl_up(v, d, WAVE);
+            if (bl >= d) v += y;
+        }
+        const uint32_t agg = __shfl(v, WAVE - 1, WAVE);
+        const uint32_t excl = wave_lookback(status, blk, epoch, agg, err);
+        base_s[bl] = excl + v - col;
+    }
+    __syncthreads();
+    if (bkt < NB) {
+        uint32_t run = base_s[bl] + gpre;
+        if (g == 0) bstart[bkt] = base_s[bl];
+        for (uint32_t t = t0; t < t1; t++) {
+            const uint64_t at = (uint64_t)t * NB + bkt;
+            const uint32_t c = counts[at];
+            offs[at] = run;
+            run += c;
+        }
+    }
+    if (blk == 0 && threadIdx.x == 0) bstart[NB] = (uint32_t)n;
+}
+
+__global__ __launch_bounds__(MT) void k_msd_scatter(const uint64_t *__restrict__ kp, const uint16_t *__restrict__ rk,
+                                                    uint64_t n, uint32_t bshift, uint32_t NB,
+                                                    const uint32_t *__restrict__ offs, uint64_t *__restrict__ kb) {
+    const uint64_t base = (uint64_t)blockIdx.x * MTILE;
+    const uint32_t *o = offs + (uint64_t)blockIdx.x * NB;
+    uint64_t p[MIPT];
+    uint32_t r[MIPT];
+#pragma unroll
+    for (uint32_t j = 0; j < MIPT; j++) {
+        const uint64_t idx = base + (uint64_t)j * MT + threadIdx.x;
+        p[j] = idx < n ? kp[idx] : 0;
+        r[j] = idx < n ? rk[idx] : 0;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < MIPT; j++) {
+        const uint64_t idx = base + (uint64_t)j * MT + threadIdx.x;
+        if (idx < n) kb[(uint64_t)o[(uint32_t)(p[j] >> bshift) & (NB - 1)] + r[j]] = p[j];
+    }
+}
+
+// One workgroup per bucket: stable sort of kb[bstart[b], bstart[b+1]) by the
+// packed key's bits [8, 8 + R) (the bucket bits above them are equal), then
+// unpack into (ok, ov) at the same positions. tmp is scratch of the same
+// range. Buckets above SMALL items raise *maxb to their size.
+__global__ __launch_bounds__(MT) void k_msd_local(uint64_t *__restrict__ kb, uint64_t *__restrict__ tmp,
+                                                  const uint32_t *__restrict__ bstart, uint32_t R, PackInfo pi,
+                                                  gkey *__restrict__ ok, gval *__restrict__ ov,
+                                                  uint32_t *__restrict__ maxb) {
+    __shared__ uint64_t lk[SMALL];
+    __shared__ uint32_t cnt[MW][256];
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t tot[256];
+    __shared__ uint32_t scan_tmp[MW + 1];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t start = bstart[blockIdx.x], end = bstart[blockIdx.x + 1];
+    const uint32_t c = end - start;
+    if (c == 0) return;
+    if (c <= SMALL) {
+        for (uint32_t i = tid; i < c; i += MT) lk[i] = kb[(uint64_t)start + i];
+        __syncthreads();
+        for (uint32_t i = tid; i < c; i += MT) {
+            const uint64_t me = lk[i] >> 8;
+            uint32_t rank = 0;
+            for (uint32_t j = 0; j < c; j++) {
+                const uint64_t o = lk[j] >> 8;
+                rank += (o < me || (o == me && j < i)) ? 1u : 0u;
+            }
+            gkey k;
+            gval v;
+            unpack_key(lk[i], pi, k, v);
+            ok[(uint64_t)start + rank] = k;
+            ov[(uint64_t)start + rank] = v;
+        }
+        return;
+    }
+    if (tid == 0) atomicMax(maxb, c);
+    const uint32_t wid = tid / WAVE, lane = lane_id();
+    const uint32_t npass = (R + 7) / 8;
+    if (npass == 0) {
+        for (uint32_t i = tid; i < c; i += MT) {
+            gkey k;
+            gval v;
+            unpack_key(kb[(uint64_t)start + i], pi, k, v);
+            ok[(uint64_t)start + i] = k;
+            ov[(uint64_t)start + i] = v;
+        }
+        return;
+    }
+    uint64_t *src = kb, *dst = tmp;
+    for (uint32_t q = 0; q < npass; q++) {
+        const uint32_t shift = 8 + 8 * q;
+        const bool last = q + 1 == npass;
+        run[tid] = 0;
+        __syncthreads();
+        for (uint32_t i = tid; i < c; i += MT) atomicAdd(&run[(uint32_t)(src[(uint64_t)start + i] >> shift) & 0xFF], 1u);
+        __syncthreads();
+        const uint32_t h = run[tid];
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan(h, scan_tmp, &total);
+        run[tid] = start + ex;
+        __syncthreads();
+        for (uint32_t c0 = 0; c0 < c; c0 += MTILE) {
+            for (uint32_t i = tid; i < MW * 256; i += MT) (&cnt[0][0])[i] = 0;
+            __syncthreads();
+            const uint32_t wb = c0 + wid * MWI;
+            uint64_t p[MIPT];
+            uint32_t d[MIPT], r[MIPT];
+#pragma unroll
+            for (uint32_t j = 0; j < MIPT; j++) {
+                const uint32_t idx = wb + j * WAVE + lane;
+                const bool in = idx < c;
+                p[j] = in ? src[(uint64_t)start + idx] : 0;
+                d[j] = in ? ((uint32_t)(p[j] >> shift) & 0xFF) : 256u;
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < MIPT; j++) r[j] = wave_rank<8>(d[j], d[j] < 256, cnt[wid]);
+            __syncthreads();
+            {
+                uint32_t rw = 0;
+#pragma unroll
+                for (uint32_t w = 0; w < MW; w++) {
+                    const uint32_t cc = cnt[w][tid];
+                    cnt[w][tid] = rw;
+                    rw += cc;
+                }
+                tot[tid] = rw;
+            }
+            __syncthreads();
+#pragma unroll
+            for (uint32_t j = 0; j < MIPT; j++) {
+                if (d[j] < 256) {
+                    const uint64_t pos = (uint64_t)run[d[j]] + cnt[wid][d[j]] + r[j];
+                    if (last) {
+                        gkey k;
+                        gval v;
+                        unpack_key(p[j], pi, k, v);
+                        ok[pos] = k;
+                        ov[pos] = v;
+                    } else {
+                        dst[pos] = p[j];
+                    }
+                }
+            }
+            __syncthreads();
+            run[tid] += tot[tid];
+            __syncthreads();
+        }
+        uint64_t *t = src;
+        src = dst;
+        dst = t;
+    }
+}
+
+}  // namespace
+
+bool msd_sort_fits(uint32_t N) { return N > 0 && N < (1u << 28); }
+
+MsdPlan msd_plan(uint64_t n, uint32_t N) {
+    MsdPlan pl{};
+    uint32_t bs = 0;
+    while (bs < 32 && (N >> bs) != 0) bs++;  // bit width of N itself
+    pl.bs = bs;
+    const uint32_t B = 2 * bs;
+    uint32_t D = 1;
+    while (D < MAX_D && (n >> (D + 6)) != 0) D++;  // about 64 items per bucket
+    if (D > B) D = B;
+    pl.D = D;
+    pl.R = B - D;
+    pl.bshift = 8 + B - D;
+    pl.ntiles = (uint32_t)((n + MTILE - 1) / MTILE);
+    pl.NB = 1u << D;
+    pl.nscan = (pl.NB + SCAN_B - 1) / SCAN_B;
+    return pl;
+}
+
+void msd_sort(const gkey *ik, const gval *iv, gkey *ok, gval *ov, const MsdPlan &pl, const MsdWork &w, uint64_t n,
+              uint32_t N, bool guards_ok, uint32_t *invalid, uint32_t epoch, uint32_t *ticket, uint32_t *err,
+              uint32_t *maxb, hipStream_t st) {
+    if (n == 0) return;
+    const PackInfo pi{N, pl.bs, guards_ok ? 1u : 0u};
+    uint32_t *counts = w.mat;
+    uint32_t *offs = w.mat + (uint64_t)pl.ntiles * pl.NB;
+    hipLaunchKernelGGL(k_msd_rank, dim3(pl.ntiles), dim3(MT), 0, st, ik, iv, n, pi, pl.bshift, pl.D, w.kp, w.rk, counts,
+                       invalid);
+    hipLaunchKernelGGL(k_msd_scan, dim3(pl.nscan), dim3(MT), 0, st, counts, pl.ntiles, pl.NB, n, offs, w.bstart,
+                       w.status, epoch, ticket, err);
+    hipLaunchKernelGGL(k_msd_scatter, dim3(pl.ntiles), dim3(MT), 0, st, w.kp, w.rk, n, pl.bshift, pl.NB, offs, w.kb);
+    hipLaunchKernelGGL(k_msd_local, dim3(pl.NB), dim3(MT), 0, st, w.kb, w.kp, w.bstart, pl.R, pi, ok, ov, maxb);
+}
+
+}  // namespace gpma

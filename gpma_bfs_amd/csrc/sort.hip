@@ -29,7 +29,7 @@ constexpr uint32_t SORT_WAVES = SORT_THREADS / WAVE;
 constexpr uint32_t SORT_IPT = 16;                      // items per lane
 constexpr uint32_t WAVE_ITEMS = WAVE * SORT_IPT;       // 1024
 constexpr uint32_t TILE = SORT_THREADS * SORT_IPT;     // 4096
-constexpr uint32_t LOOKBACK = 16;                      // predecessors read per round trip
+constexpr uint32_t LOOKBACK = 32;                      // predecessors read per round trip
 constexpr uint32_t FLAG_AGG = 1u << 30;
 constexpr uint32_t FLAG_INC = 2u << 30;
 constexpr uint32_t VAL_MASK = (1u << 30) - 1;
@@ -37,37 +37,55 @@ constexpr uint32_t SPIN_LIMIT = 1u << 22;
 constexpr uint32_t HIST_GRID_MAX = 128;
 
 // Validate + copy the batch and build every pass's digit histogram at once;
-// also clears the look-back status words of the passes.
+// also clears the look-back status words of the passes. Each thread takes
+// HIST_IPT items per round with all loads issued before any is used; each
+// wave counts into its own LDS histograms (less same-address contention on
+// skewed digits), merged into the global histograms at the end.
+constexpr uint32_t HIST_IPT = 8;
 __global__ __launch_bounds__(SORT_THREADS) void k_stage_hist(const gkey *__restrict__ ik, const gval *__restrict__ iv,
                                                              gkey *__restrict__ ok, gval *__restrict__ ov,
                                                              uint64_t n, uint32_t N, uint32_t guards_ok,
                                                              uint32_t *__restrict__ invalid, SortShifts shifts,
                                                              uint32_t npass, uint32_t *__restrict__ hist,
                                                              uint32_t *__restrict__ status, uint64_t nstatus) {
-    __shared__ uint32_t h[8][256];
-    for (uint32_t i = threadIdx.x; i < 8 * 256; i += SORT_THREADS) (&h[0][0])[i] = 0;
+    __shared__ uint32_t h[SORT_WAVES][8][256];
+    for (uint32_t i = threadIdx.x; i < SORT_WAVES * 8 * 256; i += SORT_THREADS) (&h[0][0][0])[i] = 0;
     __syncthreads();
+    const uint32_t wid = threadIdx.x / WAVE;
     const uint64_t stride = (uint64_t)gridDim.x * SORT_THREADS;
     uint32_t nbad = 0;
-    for (uint64_t i0 = (uint64_t)blockIdx.x * SORT_THREADS; i0 < n; i0 += stride) {
-        const uint64_t i = i0 + threadIdx.x;
-        bool bad = false;
-        if (i < n) {
-            gkey k = ik[i];
-            const uint32_t dst = (uint32_t)k;
-            bad = (uint32_t)(k >> 32) >= N || (dst >= N && !(guards_ok && dst == COL_NONE));
-            if (bad) k = KEY_NONE;
-            ok[i] = k;
-            ov[i] = iv[i];
-            for (uint32_t p = 0; p < npass; p++) atomicAdd(&h[p][(uint32_t)(k >> shifts.s[p]) & 0xFF], 1u);
+    for (uint64_t r0 = (uint64_t)blockIdx.x * SORT_THREADS * HIST_IPT; r0 < n; r0 += stride * HIST_IPT) {
+        gkey k[HIST_IPT];
+        gval v[HIST_IPT];
+#pragma unroll
+        for (uint32_t j = 0; j < HIST_IPT; j++) {
+            const uint64_t i = r0 + (uint64_t)j * SORT_THREADS + threadIdx.x;
+            k[j] = i < n ? ik[i] : 0;
+            v[j] = i < n ? iv[i] : 0;
         }
-        nbad += (uint32_t)__popcll(ballot(bad));
+#pragma unroll
+        for (uint32_t j = 0; j < HIST_IPT; j++) {
+            const uint64_t i = r0 + (uint64_t)j * SORT_THREADS + threadIdx.x;
+            bool bad = false;
+            if (i < n) {
+                gkey kk = k[j];
+                const uint32_t dst = (uint32_t)kk;
+                bad = (uint32_t)(kk >> 32) >= N || (dst >= N && !(guards_ok && dst == COL_NONE));
+                if (bad) kk = KEY_NONE;
+                ok[i] = kk;
+                ov[i] = v[j];
+                for (uint32_t p = 0; p < npass; p++) atomicAdd(&h[wid][p][(uint32_t)(kk >> shifts.s[p]) & 0xFF], 1u);
+            }
+            nbad += (uint32_t)__popcll(ballot(bad));
+        }
     }
     if (lane_id() == 0 && nbad) atomicAdd(invalid, nbad);
     for (uint64_t i = (uint64_t)blockIdx.x * SORT_THREADS + threadIdx.x; i < nstatus; i += stride) status[i] = 0;
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < npass * 256; i += SORT_THREADS) {
-        uint32_t c = (&h[0][0])[i];
+        uint32_t c = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < SORT_WAVES; w++) c += (&h[w][0][0])[i];
         if (c) atomicAdd(&hist[i], c);
     }
 }
@@ -88,6 +106,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(const gkey *__restri
     const uint32_t lane = lane_id();
     for (uint32_t i = tid; i < SORT_WAVES * 256; i += SORT_THREADS) (&cnt[0][0])[i] = 0;
     if (tid == 0) s_tile = atomicAdd(ticket, 1u);
+    const uint32_t hist_d = hist[tid];  // SORT_THREADS == 256: thread d owns digit d
     __syncthreads();
     const uint32_t tile = s_tile;
     const uint64_t wbase = (uint64_t)tile * TILE + (uint64_t)wid * WAVE_ITEMS;
@@ -134,6 +153,10 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(const gkey *__restri
         cnt[w][d] = tot;
         tot += c;
     }
+    // Global digit base: exclusive scan of the pass histogram (before the
+    // look-back, so its latency overlaps the wait).
+    uint32_t htot;
+    const uint32_t hb = block_exclusive_scan(hist_d, scan_tmp, &htot);
     // Publish this tile's aggregate, then look back LOOKBACK tiles per round.
     uint32_t *my = status + (uint64_t)tile * 256 + d;
     __hip_atomic_store(my, (tile == 0 ? FLAG_INC : FLAG_AGG) | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -172,9 +195,6 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(const gkey *__restri
         }
         __hip_atomic_store(my, FLAG_INC | (excl + tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    // Global digit base: exclusive scan of the pass histogram.
-    uint32_t htot;
-    uint32_t hb = block_exclusive_scan(hist[d], scan_tmp, &htot);
     gofs[d] = hb + excl;
     __syncthreads();
 #pragma unroll
@@ -208,7 +228,7 @@ int stage_and_sort(const gkey *ik, const gval *iv, gkey *k0, gval *v0, gkey *k1,
     const uint64_t tiles = (n + TILE - 1) / TILE;
     const uint64_t nstatus = tiles * 256 * npass;
     fill_u32(hist, 0, npass * 256 + npass, st);
-    uint64_t want = (std::max<uint64_t>(n, nstatus) + SORT_THREADS * 8 - 1) / (SORT_THREADS * 8);
+    uint64_t want = (std::max<uint64_t>(n, nstatus) + SORT_THREADS * HIST_IPT - 1) / (SORT_THREADS * HIST_IPT);
     uint32_t hb = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(HIST_GRID_MAX, want));
     hipLaunchKernelGGL(k_stage_hist, dim3(hb), dim3(SORT_THREADS), 0, st, ik, iv, k0, v0, n, N, guards_ok ? 1u : 0u,
                        invalid, shifts, npass, hist, status, nstatus);

@@ -44,48 +44,121 @@ __device__ __forceinline__ void wave_count(uint32_t *ctr, bool p) {
     if (m && lane_id() == (uint32_t)__ffsll((unsigned long long)m) - 1) atomicAdd(ctr, (uint32_t)__popcll(m));
 }
 
+// A leaf's S keys and values with 16-byte loads (leaves are S-slot aligned,
+// so keys + w is 16 * S / 2 byte aligned and values + w is S byte aligned).
+template <uint32_t S>
+__device__ __forceinline__ void load_leaf_keys(const gkey *__restrict__ keys, uint64_t w, gkey (&k)[S]) {
+    const uint4 *q = reinterpret_cast<const uint4 *>(keys + w);
+    uint4 t[S / 2];
+#pragma unroll
+    for (uint32_t j = 0; j < S / 2; j++) t[j] = q[j];
+#pragma unroll
+    for (uint32_t j = 0; j < S / 2; j++) {
+        k[2 * j] = (uint64_t)t[j].x | ((uint64_t)t[j].y << 32);
+        k[2 * j + 1] = (uint64_t)t[j].z | ((uint64_t)t[j].w << 32);
+    }
+}
+template <uint32_t S>
+__device__ __forceinline__ void load_leaf_values(const gval *__restrict__ values, uint64_t w, gval (&v)[S]) {
+    if constexpr (S >= 16) {
+        const uint4 *q = reinterpret_cast<const uint4 *>(values + w);
+#pragma unroll
+        for (uint32_t j = 0; j < S / 16; j++) {
+            const uint4 t = q[j];
+            const uint32_t wd[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+            for (uint32_t b = 0; b < 16; b++) v[16 * j + b] = (gval)(wd[b / 4] >> (8 * (b % 4)));
+        }
+    } else if constexpr (S == 8) {
+        const uint2 t = *reinterpret_cast<const uint2 *>(values + w);
+#pragma unroll
+        for (uint32_t b = 0; b < 8; b++) v[b] = (gval)((b < 4 ? t.x : t.y) >> (8 * (b % 4)));
+    } else if constexpr (S == 4) {
+        const uint32_t t = *reinterpret_cast<const uint32_t *>(values + w);
+#pragma unroll
+        for (uint32_t b = 0; b < 4; b++) v[b] = (gval)(t >> (8 * b));
+    } else {
+        const uint16_t t = *reinterpret_cast<const uint16_t *>(values + w);
+        v[0] = (gval)t;
+        v[1] = (gval)(t >> 8);
+    }
+}
+
+// Segment index the bitwise descending search of gpma.cuh:207-215 returns
+// for key k: bit b of the result (b = nseg/2 .. 1) is set when
+// seps[prefix | b] <= k. Path-dependent on non-monotone separators (the first
+// key of an empty segment is KEY_NONE), so the exact walk is kept.
+__device__ __forceinline__ uint64_t bitwise_search(const gkey *__restrict__ seps, uint64_t nseg, gkey k) {
+    uint64_t p = 0;
+    for (uint64_t b = nseg >> 1; b > 0; b >>= 1)
+        if (seps[p | b] <= k) p |= b;
+    return p;
+}
+
+// True when the bitwise search for k would return p: every step's
+// comparison along p's path agrees with p's bit. The loads are independent,
+// so they cost one round trip instead of log2(nseg) dependent ones.
+__device__ __forceinline__ bool search_path_matches(const gkey *__restrict__ seps, uint64_t nseg, gkey k,
+                                                    uint64_t p) {
+    bool ok = p < nseg;
+    for (uint64_t b = nseg >> 1; b > 0; b >>= 1) {
+        const uint64_t pos = (p & ~((b << 1) - 1)) | b;
+        const bool le = seps[pos] <= k;
+        ok &= le == ((p & b) != 0);
+    }
+    return ok;
+}
+
 // One update per thread (gpma.cuh:204-226). Sorted input, so the last of
 // equal keys is the one applied (earlier duplicates are skipped).
+// With row_hint, the segment is first guessed from row_offset (the key lies
+// between the slots of guards src-1 and src) and the guess is checked against
+// the full bitwise search path; a failed check falls back to the walk, so the
+// result is always the reference search's.
+template <uint32_t S>
 __global__ __launch_bounds__(CMP_BLOCK) void k_locate(const gkey *__restrict__ ks, const gval *__restrict__ vs,
                                                       uint64_t n, gkey *__restrict__ keys, gval *__restrict__ values,
-                                                      const gkey *__restrict__ seps, uint64_t nseg, uint32_t S,
-                                                      uint32_t *__restrict__ win, uint32_t *__restrict__ blkcnt) {
-    __shared__ uint32_t wcnt[CMP_BLOCK / WAVE];
+                                                      const gkey *__restrict__ seps, uint64_t nseg,
+                                                      const uint32_t *__restrict__ row_offset, uint32_t row_hint,
+                                                      uint32_t *__restrict__ win) {
     const uint64_t i = (uint64_t)blockIdx.x * CMP_BLOCK + threadIdx.x;
-    bool pend = false;
-    if (i < n) {
-        const gkey k = ks[i];
-        const gval v = vs[i];
-        uint32_t res = SIZE_NONE;
-        const bool skip = (k == KEY_NONE) || (i + 1 < n && ks[i + 1] == k);
-        if (!skip) {
-            uint64_t p = 0;
-            for (uint64_t b = nseg >> 1; b > 0; b >>= 1)
-                if (seps[p | b] <= k) p |= b;
-            const uint64_t base = p * S;
-            bool found = false;
-            for (uint32_t j = 0; j < S; j++) {
-                if (keys[base + j] == k) {
-                    values[base + j] = v;
-                    found = true;
-                    break;
-                }
+    if (i >= n) return;
+    const gkey k = ks[i];
+    const gval v = vs[i];
+    uint32_t res = SIZE_NONE;
+    const bool skip = (k == KEY_NONE) || (i + 1 < n && ks[i + 1] == k);
+    if (!skip) {
+        uint64_t p = 0;
+        bool have = false;
+        if (row_hint) {
+            const uint32_t src = (uint32_t)(k >> 32);
+            uint64_t lo = row_offset[src] / S, hi = row_offset[src + 1] / S;
+            if (hi >= nseg) hi = nseg - 1;
+            if (lo > hi) lo = hi;
+            // last segment in [lo, hi] whose first key is <= k
+            while (hi - lo > 4) {
+                const uint64_t mid = (lo + hi + 1) >> 1;
+                if (seps[mid] <= k) lo = mid;
+                else hi = mid - 1;
             }
-            if (!found && v != 0) {
-                res = (uint32_t)base;
-                pend = true;
-            }
+            p = lo;
+            for (uint64_t j = lo + 1; j <= hi; j++)
+                if (seps[j] <= k) p = j;
+            have = search_path_matches(seps, nseg, k, p);
         }
-        win[i] = res;
+        if (!have) p = bitwise_search(seps, nseg, k);
+        const uint64_t base = p * S;
+        // handle_del_mod (gpma.cuh:188-199): the first slot holding the key
+        gkey lk[S];
+        load_leaf_keys<S>(keys, base, lk);
+        uint32_t hit = S;
+#pragma unroll
+        for (uint32_t j = S; j-- > 0;)
+            if (lk[j] == k) hit = j;
+        if (hit < S) values[base + hit] = v;
+        else if (v != 0) res = (uint32_t)base;
     }
-    uint64_t m = ballot(pend);
-    if (lane_id() == 0) wcnt[threadIdx.x / WAVE] = (uint32_t)__popcll(m);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t s = 0;
-        for (uint32_t w = 0; w < CMP_BLOCK / WAVE; w++) s += wcnt[w];
-        blkcnt[blockIdx.x] = s;
-    }
+    win[i] = res;
 }
 
 // Exclusive scan of per-chunk counts (one workgroup). The number of chunks
@@ -169,8 +242,6 @@ constexpr uint32_t CC_THREADS = 256;
 constexpr uint32_t CC_ROWS = 8;                        // 64-entry rows per wave
 constexpr uint32_t CC_WAVE_ITEMS = WAVE * CC_ROWS;     // 512
 constexpr uint32_t CC_TILE = CC_THREADS * CC_ROWS;     // 2048
-constexpr uint64_t CC_AGG = 1ull << 30, CC_INC = 2ull << 30, CC_CNT = (1ull << 30) - 1;
-constexpr uint32_t CC_SPIN_LIMIT = 1u << 22;
 
 __global__ __launch_bounds__(CC_THREADS) void k_compact_chained(
     const gkey *__restrict__ ik, const gval *__restrict__ iv, const uint32_t *__restrict__ iw,
@@ -210,49 +281,7 @@ __global__ __launch_bounds__(CC_THREADS) void k_compact_chained(
         tile_total += wtot[w];
     }
     if (wid == 0) {
-        const uint64_t tag = (uint64_t)epoch << 32;
-        if (lane == 0)
-            __hip_atomic_store(status + tile, tag | (tile == 0 ? CC_INC : CC_AGG) | tile_total, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        uint32_t excl = 0;
-        if (tile > 0) {
-            int64_t t = (int64_t)tile - 1;  // next predecessor not yet added
-            uint32_t spins = 0;
-            while (true) {
-                const int64_t q = t - (int64_t)lane;
-                unsigned long long w = 0;
-                if (q >= 0)
-                    w = __hip_atomic_load(status + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const bool mine = q >= 0 && (w >> 32) == epoch;
-                const uint64_t fl = w & ~(CC_CNT | (0xFFFFFFFFull << 32));
-                const bool ready = q < 0 || (mine && fl != 0);
-                const bool inc = q < 0 || (mine && fl == CC_INC);
-                const uint64_t notready = ballot(!ready);
-                const uint64_t incm = ballot(inc);
-                // lanes before the first not-ready lane are usable; stop at the first INC among them
-                const uint64_t usable = notready ? ((notready & (~notready + 1)) - 1) : ~0ull;
-                const uint64_t stop = incm & usable;
-                const uint64_t take = stop ? (((stop & (~stop + 1)) << 1) - 1) : usable;
-                const uint32_t c = ((take >> lane) & 1ull) && q >= 0 ? (uint32_t)(w & CC_CNT) : 0u;
-                // wave sum of c
-                uint32_t sum = c;
-                for (uint32_t off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, WAVE);
-                excl += sum;
-                if (stop) break;
-                const uint32_t used = (uint32_t)__popcll(take);
-                t -= used;
-                if (used == 0) {
-                    if (++spins > CC_SPIN_LIMIT) {
-                        if (lane == 0) atomicOr(err, 1u);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                }
-            }
-            if (lane == 0)
-                __hip_atomic_store(status + tile, tag | CC_INC | (excl + tile_total), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-        }
+        const uint32_t excl = wave_lookback(status, tile, epoch, tile_total, err);
         if (lane == 0) {
             s_excl = excl;
             if (base + CC_TILE >= n) *d_out = excl + tile_total;
@@ -277,6 +306,8 @@ __global__ __launch_bounds__(CC_THREADS) void k_compact_chained(
 // Level 0 (window = one leaf of S slots), one thread per run of pending
 // inserts that share a leaf. With f = W/S = 1 the redistribution packs the
 // merged items at the front of the leaf (gpma.cuh:636-654 with f = 1).
+// The leaf is read with 16-byte loads, merged into a per-thread LDS row and
+// written back with 16-byte stores.
 template <uint32_t S>
 __global__ __launch_bounds__(256) void k_leaf(const gkey *__restrict__ pk, const gval *__restrict__ pv,
                                               const uint32_t *__restrict__ pw, uint8_t *__restrict__ pd,
@@ -284,6 +315,9 @@ __global__ __launch_bounds__(256) void k_leaf(const gkey *__restrict__ pk, const
                                               gval *__restrict__ values, gkey *__restrict__ seps,
                                               uint32_t *__restrict__ row_offset, uint32_t lower, uint32_t upper,
                                               uint32_t bulk_threshold) {
+    constexpr uint32_t RK = S + 1;  // padded LDS row (spreads the rows over the banks)
+    __shared__ gkey rowk[256 * RK];
+    __shared__ gval rowv[256 * S];
     const uint64_t np = *d_np;
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     // a batch with this many pending inserts takes the bulk path instead
@@ -295,20 +329,26 @@ __global__ __launch_bounds__(256) void k_leaf(const gkey *__restrict__ pk, const
     if (r > upper) return;
     gkey k[S];
     gval v[S];
+    load_leaf_keys<S>(keys, w, k);
+    load_leaf_values<S>(values, w, v);
     bool live[S];
     uint32_t shift[S];
     uint32_t c = 0;
 #pragma unroll
     for (uint32_t j = 0; j < S; j++) {
-        k[j] = keys[(uint64_t)w + j];
-        v[j] = values[(uint64_t)w + j];
         live[j] = is_live(k[j], v[j]);
         c += live[j] ? 1u : 0u;
         shift[j] = 0;
     }
     const uint32_t m = c + r;
     if (m < lower || m > upper) return;
-    gkey first = KEY_NONE;
+    gkey *ok = rowk + threadIdx.x * RK;
+    gval *ov = rowv + threadIdx.x * S;
+#pragma unroll
+    for (uint32_t j = 0; j < S; j++) {
+        ok[j] = KEY_NONE;
+        ov[j] = 0;
+    }
     for (uint32_t u = 0; u < r; u++) {
         const gkey ik = pk[i + u];
         const gval iv = pv[i + u];
@@ -321,10 +361,9 @@ __global__ __launch_bounds__(256) void k_leaf(const gkey *__restrict__ pk, const
             }
         }
         const uint32_t pos = u + lb;
-        keys[(uint64_t)w + pos] = ik;
-        values[(uint64_t)w + pos] = iv;
+        ok[pos] = ik;
+        ov[pos] = iv;
         if (is_guard(ik)) row_offset[(ik >> 32) + 1] = w + pos;
-        if (pos == 0) first = ik;
         pd[i + u] = 1;
     }
     uint32_t lr = 0;
@@ -332,21 +371,33 @@ __global__ __launch_bounds__(256) void k_leaf(const gkey *__restrict__ pk, const
     for (uint32_t j = 0; j < S; j++) {
         if (live[j]) {
             const uint32_t pos = lr + shift[j];
-            keys[(uint64_t)w + pos] = k[j];
-            values[(uint64_t)w + pos] = v[j];
+            ok[pos] = k[j];
+            ov[pos] = v[j];
             if (is_guard(k[j])) row_offset[(k[j] >> 32) + 1] = w + pos;
-            if (pos == 0) first = k[j];
             lr++;
         }
     }
+    // the row is private to this thread: no barrier needed before reading it back
+    uint4 *qk = reinterpret_cast<uint4 *>(keys + w);
 #pragma unroll
-    for (uint32_t j = 0; j < S; j++) {
-        if (j >= m) {
-            keys[(uint64_t)w + j] = KEY_NONE;
-            values[(uint64_t)w + j] = 0;
-        }
+    for (uint32_t j = 0; j < S / 2; j++) {
+        const gkey a = ok[2 * j], b = ok[2 * j + 1];
+        qk[j] = make_uint4((uint32_t)a, (uint32_t)(a >> 32), (uint32_t)b, (uint32_t)(b >> 32));
     }
-    seps[w / S] = first;
+    if constexpr (S >= 16) {
+        uint4 *qv = reinterpret_cast<uint4 *>(values + w);
+#pragma unroll
+        for (uint32_t j = 0; j < S / 16; j++) {
+            uint32_t wd[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (uint32_t b = 0; b < 16; b++) wd[b / 4] |= (uint32_t)ov[16 * j + b] << (8 * (b % 4));
+            qv[j] = make_uint4(wd[0], wd[1], wd[2], wd[3]);
+        }
+    } else {
+#pragma unroll
+        for (uint32_t j = 0; j < S; j++) values[(uint64_t)w + j] = ov[j];
+    }
+    seps[w / S] = ok[0];
 }
 
 // number of entries of a sorted array a[0..n) strictly below x
@@ -616,11 +667,21 @@ void fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t st) {
 }
 
 void launch_locate(const gkey *ks, const gval *vs, uint64_t n, gkey *keys, gval *values, const gkey *seps,
-                   uint64_t nseg, uint32_t S, uint32_t *win, uint32_t *blkcnt, hipStream_t st) {
+                   uint64_t nseg, uint32_t S, const uint32_t *row_offset, bool row_hint, uint32_t *win,
+                   hipStream_t st) {
     if (!n) return;
-    uint64_t nb = (n + CMP_BLOCK - 1) / CMP_BLOCK;
-    hipLaunchKernelGGL(k_locate, dim3((uint32_t)nb), dim3(CMP_BLOCK), 0, st, ks, vs, n, keys, values, seps, nseg, S,
-                       win, blkcnt);
+    dim3 g((uint32_t)((n + CMP_BLOCK - 1) / CMP_BLOCK)), b(CMP_BLOCK);
+    const uint32_t rh = row_hint ? 1u : 0u;
+#define GPMA_LOCATE(SS) \
+    hipLaunchKernelGGL(k_locate<SS>, g, b, 0, st, ks, vs, n, keys, values, seps, nseg, row_offset, rh, win)
+    switch (S) {
+    case 2: GPMA_LOCATE(2); break;
+    case 4: GPMA_LOCATE(4); break;
+    case 8: GPMA_LOCATE(8); break;
+    case 16: GPMA_LOCATE(16); break;
+    default: GPMA_LOCATE(32); break;
+    }
+#undef GPMA_LOCATE
 }
 
 void launch_scan_blocks(const uint32_t *cnt, uint64_t nb, uint32_t *off, uint32_t *total, hipStream_t st,

@@ -266,3 +266,30 @@ def test_fused_batches_equal_sequential():
     assert keys[half] not in set(la.tolist()) and keys[0] in set(la.tolist())
     for s in (0, 1, 100):
         assert np.array_equal(G.gpma_bfs(a, s)[0], G.gpma_bfs(b, s)[0])
+
+
+def test_skewed_batches_sort_paths():
+    """Batches whose updates crowd into few sort buckets (one hub source with
+    thousands of edges, then a batch over two sources only): every sort path
+    (comparison ranking, in-bucket digit passes, and the switch to the full
+    LSD passes after a bucket above 16384 items) must leave the same live
+    set as the oracle, layout included."""
+    N = 40000
+    rng = np.random.default_rng(7)
+    g = G.GPMA(N)
+    o = O.OracleGPMA(N)
+    # hub 5 gets 2000 edges, then 20000 edges spread over sources 7 and 8
+    hub = (np.uint64(5) << np.uint64(32)) | rng.permutation(N)[:2000].astype(np.uint64)
+    two = np.unique((rng.integers(7, 9, 30000).astype(np.uint64) << np.uint64(32)) |
+                    rng.integers(0, N, 30000).astype(np.uint64))[:20000]
+    rng.shuffle(two)
+    mixed = (rng.integers(0, N, 5000).astype(np.uint64) << np.uint64(32)) | rng.integers(0, N, 5000).astype(np.uint64)
+    batches = [(hub, np.ones(len(hub), np.uint8)), (two, np.ones(len(two), np.uint8)),
+               (mixed, np.ones(len(mixed), np.uint8)),
+               (np.concatenate([hub[:700], two[:9000]]), np.zeros(9700, np.uint8))]
+    for i, (k, v) in enumerate(batches):
+        G.update_gpma(g, k, v)
+        o.update(k, v)
+        assert np.array_equal(g.export_live(), o.live()), f"batch {i}"
+        assert_layout_equal(g, o, f"batch {i}")
+        assert_invariants(g)
