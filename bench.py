@@ -31,6 +31,7 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "edge-updates/sec + BFS GTEPS on 1 GPU, % HBM peak; batched vs plain; p99 latency"
 S_REF = 16  # reference segment length at every capacity in [2^15, 2^30]
 
 
@@ -69,12 +70,24 @@ def ms_bfs_bytes(N, outdeg, lv_all, nsrc):
     level at which any source reaches it (16 B of offsets and masks, then 9 B
     slot + 8 B neighbour mask per edge), plus the u32 level outputs. Also
     returns the aggregate edges traversed over all sources (for TEPS)."""
-    srt = np.sort(np.ascontiguousarray(lv_all.T), axis=1)  # [N][nsrc]
-    distinct = srt != 0
-    distinct[:, 1:] &= srt[:, 1:] != srt[:, :-1]
-    D = distinct.sum(axis=1).astype(np.int64)
+    reached = np.zeros(N, np.int64)
+    if int(lv_all.max(initial=0)) < 64:
+        mask = np.zeros(N, np.uint64)
+        for row in lv_all:
+            nz = row != 0
+            mask[nz] |= np.left_shift(np.uint64(1), row[nz].astype(np.uint64))
+            reached += nz
+        D = np.zeros(N, np.int64)
+        for b in range(64):
+            D += ((mask >> np.uint64(b)) & np.uint64(1)).astype(np.int64)
+    else:
+        srt = np.sort(np.ascontiguousarray(lv_all.T), axis=1)  # [N][nsrc]
+        distinct = srt != 0
+        distinct[:, 1:] &= srt[:, 1:] != srt[:, :-1]
+        D = distinct.sum(axis=1).astype(np.int64)
+        reached = (lv_all != 0).sum(axis=0).astype(np.int64)
     b = int((D * (16 + 17 * outdeg)).sum()) + nsrc * N * 4
-    teps_edges = int(((lv_all != 0).sum(axis=0).astype(np.int64) * outdeg).sum())
+    teps_edges = int((reached * outdeg).sum())
     return b, teps_edges
 
 
@@ -187,6 +200,119 @@ def run_batched(G, N, keys, half, steps, warmup, nsrc=64, fused=8, check=True):
     return out
 
 
+def live_checksum(keys):
+    """(count, wrapping sum, xor) of a u64 key set: a size-independent parity
+    check that needs no sort."""
+    k = np.asarray(keys, np.uint64)
+    with np.errstate(over="ignore"):
+        return int(len(k)), int(k.sum(dtype=np.uint64)), int(np.bitwise_xor.reduce(k)) if len(k) else 0
+
+
+def run_large(G, args):
+    """Large single-GPU configuration: R-MAT scale 25 (2^25 nodes), 2^29 draws
+    (a,b,c = .57,.19,.19, seed 3), deduplicated; the first half is loaded in
+    bulk, then slides of 2^20 inserts + 2^20 deletes (one batch each), with a
+    64-source BFS (sources seed 3) after the timed slides."""
+    scale, draws = 25, 1 << 29
+    N = 1 << scale
+    t0 = time.time()
+    src, dst = G.gen_rmat(scale, draws, seed=3)
+    keys = G.make_keys(src, dst)
+    del src, dst
+    E = len(keys)
+    half = E // 2
+    step = 1 << 20
+    log(f"large: {E} unique edges over {N} nodes in {time.time() - t0:.1f}s; half={half}")
+    g = G.GPMA(N)
+    t = time.time()
+    G.update_gpma(g, keys[:half], np.ones(half, np.uint8))
+    g.sync()
+    t_load = time.time() - t
+    st = g.stats()
+    log(f"large: bulk window of {half} edges in {t_load:.2f}s -> C={st['capacity']} S={st['segment_length']} "
+        f"h={st['tree_height']}")
+    W, K = 2, args.large_steps
+    nsl = W + K
+    if half + nsl * step > E:
+        raise RuntimeError("not enough edges for the requested slides")
+    nb = 2 * step
+    batches = np.empty(nsl * nb, np.uint64)
+    for i in range(nsl):
+        batches[i * nb: i * nb + step] = keys[half + i * step: half + (i + 1) * step]
+        batches[i * nb + step: (i + 1) * nb] = keys[i * step: (i + 1) * step]
+    vals = np.tile(np.concatenate([np.ones(step, np.uint8), np.zeros(step, np.uint8)]), nsl)
+    dk = G.DeviceBuffer.from_numpy(batches)
+    dv = G.DeviceBuffer.from_numpy(vals)
+    del batches, vals
+    for i in range(W):
+        G.update_gpma_device(g, dk.ptr + i * nb * 8, dv.ptr + i * nb, nb)
+    g.sync()
+    per = []
+    t_start = time.perf_counter()
+    for i in range(W, nsl):
+        a = time.perf_counter()
+        G.update_gpma_device(g, dk.ptr + i * nb * 8, dv.ptr + i * nb, nb)
+        per.append(time.perf_counter() - a)
+    g.sync()
+    t_total = time.perf_counter() - t_start
+    ups = K * nb / t_total
+    dk.free()
+    dv.free()
+    win = keys[nsl * step: half + nsl * step]
+    # parity: live set == window + guards + 2 KEY_MAX, by count / sum / xor
+    guards = (np.arange(N, dtype=np.uint64) << np.uint64(32)) | np.uint64(0xFFFFFFFF)
+    live = g.export_live()
+    c_live = live_checksum(live)
+    del live
+    ce = [live_checksum(win), live_checksum(guards), live_checksum(np.array([G.KEY_MAX, G.KEY_MAX], np.uint64))]
+    expect = (sum(c[0] for c in ce), sum(c[1] for c in ce) % (1 << 64), ce[0][2] ^ ce[1][2] ^ ce[2][2])
+    ok_live = c_live == expect
+    outdeg = np.bincount((win >> np.uint64(32)).astype(np.int64), minlength=N)
+    # single-source BFS from node 0 and a 64-source BFS
+    lv, reach = G.gpma_bfs(g, 0)
+    bfs_b, e_trav = bfs_bytes(N, outdeg, lv)
+    bt = []
+    for _ in range(3):
+        a = time.perf_counter()
+        G.gpma_bfs_count(g, 0)
+        bt.append(time.perf_counter() - a)
+    bfs_s = float(np.median(bt))
+    starts = np.random.default_rng(3).integers(0, N, 64).astype(np.uint32)
+    lv_all, mreach = G.gpma_bfs_multi(g, starts)
+    ok_ms = bool(np.array_equal(lv_all[0], G.gpma_bfs(g, int(starts[0]))[0]))
+    mb, me = ms_bfs_bytes(N, outdeg, lv_all, 64)
+    del lv_all
+    dlev = G.DeviceBuffer(64 * N * 4)
+    mt = []
+    for _ in range(3):
+        a = time.perf_counter()
+        G.gpma_bfs_multi(g, starts, dlev.ptr)
+        mt.append(time.perf_counter() - a)
+    dlev.free()
+    ms_s = float(np.median(mt))
+    per_ms = np.array(per) * 1e3
+    st = g.stats()
+    g.close()
+    ub = update_bytes(step, step)
+    return {
+        "workload": f"R-MAT scale 25 (N=2^25), 2^29 draws (a,b,c=.57,.19,.19, seed 3), dedup -> {E} edges; "
+                    f"window = first {half}; step = 2^20 inserts + 2^20 deletes",
+        "nodes": N, "edges": E, "bulk_load_s": t_load, "steps": K, "warmup": W,
+        "edge_updates_per_s": ups, "ms_per_step": t_total / K * 1e3,
+        "latency_ms": {"p50": float(np.percentile(per_ms, 50)), "p99": float(np.percentile(per_ms, 99))},
+        "step_roofline": {"algorithmic_bytes": ub["total"], "achieved": ub["total"] / (t_total / K) / 1e9,
+                          "frac": ub["total"] / (t_total / K) / 1e9 / HBM_PEAK_GBS, "unit": "GB/s"},
+        "bfs": {"reachable": int(reach), "edges_traversed": e_trav, "ms": bfs_s * 1e3,
+                "gteps": e_trav / bfs_s / 1e9, "achieved_gbs": bfs_b / bfs_s / 1e9,
+                "frac": bfs_b / bfs_s / 1e9 / HBM_PEAK_GBS},
+        "msbfs": {"sources": 64, "ms": ms_s * 1e3, "edges": me, "gteps": me / ms_s / 1e9,
+                  "algorithmic_bytes": mb, "achieved_gbs": mb / ms_s / 1e9, "frac": mb / ms_s / 1e9 / HBM_PEAK_GBS},
+        "layout": {"capacity": st["capacity"], "segment_length": st["segment_length"],
+                   "tree_height": st["tree_height"], "device_bytes": st.get("device_bytes")},
+        "parity": {"live_set_count_sum_xor": ok_live, "msbfs_row0_equals_bfs": ok_ms},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,6 +325,9 @@ def main():
     ap.add_argument("--breakdown-slides", type=int, default=10,
                     help="slides before the timed ones with every phase timed (phase_ms_per_step)")
     ap.add_argument("--batched-steps", type=int, default=10, help="batched-form steps (0 = skip)")
+    ap.add_argument("--large", action="store_true",
+                    help="R-MAT 2^25 nodes / 2^29 draws instead (bulk load, 2^21-update slides, 64-source BFS)")
+    ap.add_argument("--large-steps", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -209,6 +338,19 @@ def main():
         sys.exit(2)
 
     import gpma_bfs_amd as G
+
+    if args.large:
+        r = run_large(G, args)
+        log(f"large: {r['edge_updates_per_s'] / 1e9:.3f} G edge-updates/s, p99 {r['latency_ms']['p99']:.2f} ms, "
+            f"BFS {r['bfs']['gteps']:.1f} GTEPS, 64-source BFS {r['msbfs']['gteps']:.1f} GTEPS, parity {r['parity']}")
+        out = {"metric": METRIC, "value": r["edge_updates_per_s"], "unit": "edge-updates/s", "n_gpus": 1,
+               "steps": r["steps"], "warmup": r["warmup"], "ms_per_step": r["ms_per_step"], "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+               "config": {"workload": r["workload"], "form": "plain, large", "batch": 1 << 21,
+                          "nodes": r["nodes"], "edges": r["edges"]},
+               "large": r}
+        print(json.dumps(out), flush=True)
+        return
 
     t0 = time.time()
     if args.small:
@@ -360,7 +502,7 @@ def main():
             log(f"cpu_baseline skipped: {e}")
 
     out = {
-        "metric": "edge-updates/sec + BFS GTEPS on 1 GPU, % HBM peak; batched vs plain; p99 latency",
+        "metric": METRIC,
         "value": ups, "unit": "edge-updates/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
         "ms_per_step": step_s * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u64", "data": "synthetic",

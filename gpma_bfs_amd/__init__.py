@@ -286,12 +286,11 @@ def gpma_csr_bfs(keys_ptr, values_ptr, row_offsets_ptr, node_size, edge_size, st
 def gen_rmat(scale, ndraws, a=0.57, b=0.19, c=0.19, seed=1):
     """Seeded R-MAT edge list: unique, no self-loops, shuffled (src, dst uint32)."""
     n = ctypes.c_uint64()
-    check(lib.gpma_gen_rmat(scale, ndraws, a, b, c, seed, None, None, 0, ctypes.byref(n)), "gpma_gen_rmat")
-    src = np.empty(n.value, np.uint32)
-    dst = np.empty(n.value, np.uint32)
-    check(lib.gpma_gen_rmat(scale, ndraws, a, b, c, seed, src.ctypes.data, dst.ctypes.data, n.value,
+    src = np.empty(ndraws, np.uint32)  # the unique edges are at most the draws
+    dst = np.empty(ndraws, np.uint32)
+    check(lib.gpma_gen_rmat(scale, ndraws, a, b, c, seed, src.ctypes.data, dst.ctypes.data, ndraws,
                             ctypes.byref(n)), "gpma_gen_rmat")
-    return src, dst
+    return src[:n.value].copy(), dst[:n.value].copy()
 
 
 def gen_powerlaw(N, E, alpha=2.1, soft_cap=2800.0, p1=0.24, seed=0):

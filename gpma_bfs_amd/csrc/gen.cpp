@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <atomic>
 #include <thread>
 #include <vector>
 
@@ -38,6 +39,8 @@ uint64_t mix(uint64_t a, uint64_t b) {
     m.next();
     return m.next();
 }
+
+constexpr uint64_t RMAT_BINNED_SHUFFLE = 1ull << 26;
 
 unsigned nthreads() {
     unsigned t = std::thread::hardware_concurrency();
@@ -72,6 +75,126 @@ void shuffle_edges(uint32_t *src, uint32_t *dst, uint64_t n, uint64_t seed) {
     }
 }
 
+// Sorted unique keys, identical to std::sort + std::unique; large inputs
+// are bucketed by their top bits and the buckets sorted in parallel.
+void sort_unique(std::vector<uint64_t> &keys, uint32_t key_bits) {
+    const uint64_t n = keys.size();
+    if (n < (1ull << 22)) {
+        std::sort(keys.begin(), keys.end());
+        keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+        return;
+    }
+    const uint32_t bb = 16;
+    const uint32_t shift = key_bits > bb ? key_bits - bb : 0;
+    const uint64_t NBK = 1ull << bb;
+    auto bucket = [&](uint64_t k) { return std::min<uint64_t>(k >> shift, NBK - 1); };
+    const unsigned T = nthreads();
+    const uint64_t chunk = (n + T - 1) / T;
+    std::vector<uint64_t> cnt((uint64_t)T * NBK, 0);
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < T; t++)
+        th.emplace_back([&, t] {
+            const uint64_t a = t * chunk, b = std::min(n, a + chunk);
+            uint64_t *c = cnt.data() + (uint64_t)t * NBK;
+            for (uint64_t i = a; i < b; i++) c[bucket(keys[i])]++;
+        });
+    for (auto &x : th) x.join();
+    th.clear();
+    std::vector<uint64_t> bstart(NBK + 1, 0);
+    uint64_t run = 0;
+    for (uint64_t b = 0; b < NBK; b++) {
+        bstart[b] = run;
+        for (unsigned t = 0; t < T; t++) {
+            const uint64_t c = cnt[(uint64_t)t * NBK + b];
+            cnt[(uint64_t)t * NBK + b] = run;
+            run += c;
+        }
+    }
+    bstart[NBK] = run;
+    std::vector<uint64_t> tmp(n);
+    for (unsigned t = 0; t < T; t++)
+        th.emplace_back([&, t] {
+            const uint64_t a = t * chunk, b = std::min(n, a + chunk);
+            uint64_t *c = cnt.data() + (uint64_t)t * NBK;
+            for (uint64_t i = a; i < b; i++) tmp[c[bucket(keys[i])]++] = keys[i];
+        });
+    for (auto &x : th) x.join();
+    th.clear();
+    std::vector<uint64_t> ucount(NBK, 0);
+    std::atomic<uint64_t> next{0};
+    for (unsigned t = 0; t < T; t++)
+        th.emplace_back([&] {
+            for (uint64_t b; (b = next.fetch_add(1)) < NBK;) {
+                auto lo = tmp.begin() + bstart[b], hi = tmp.begin() + bstart[b + 1];
+                std::sort(lo, hi);
+                ucount[b] = (uint64_t)(std::unique(lo, hi) - lo);
+            }
+        });
+    for (auto &x : th) x.join();
+    th.clear();
+    std::vector<uint64_t> uoff(NBK + 1, 0);
+    for (uint64_t b = 0; b < NBK; b++) uoff[b + 1] = uoff[b] + ucount[b];
+    parallel_for(NBK, [&](uint64_t lo, uint64_t hi) {
+        for (uint64_t b = lo; b < hi; b++)
+            std::copy(tmp.begin() + bstart[b], tmp.begin() + bstart[b] + ucount[b], keys.begin() + uoff[b]);
+    });
+    keys.resize(uoff[NBK]);
+}
+
+// Uniform shuffle of a large key list: every key goes to one of NBIN bins
+// drawn from its own stream (index-keyed, so the result is independent of
+// the thread count), keys keep their order inside a bin, each bin is then
+// Fisher-Yates shuffled with its own stream, and the bins are concatenated.
+// I.i.d. uniform bins followed by uniform in-bin shuffles give a uniform
+// permutation (Sanders 1998, "Random permutations on distributed, external
+// and hierarchical memory").
+void shuffle_binned(std::vector<uint64_t> &keys, uint64_t seed) {
+    const uint64_t n = keys.size();
+    const uint64_t NBIN = 4096;
+    const unsigned T = nthreads();
+    const uint64_t chunk = (n + T - 1) / T;
+    const uint64_t bseed = mix(seed, 0x42494E53ull);
+    auto bin = [&](uint64_t i) { return (uint64_t)(((unsigned __int128)mix(bseed, i) * NBIN) >> 64); };
+    std::vector<uint64_t> cnt((uint64_t)T * NBIN, 0);
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < T; t++)
+        th.emplace_back([&, t] {
+            const uint64_t a = t * chunk, b = std::min(n, a + chunk);
+            for (uint64_t i = a; i < b; i++) cnt[(uint64_t)t * NBIN + bin(i)]++;
+        });
+    for (auto &x : th) x.join();
+    th.clear();
+    std::vector<uint64_t> bstart(NBIN + 1, 0);
+    uint64_t run = 0;
+    for (uint64_t b = 0; b < NBIN; b++) {
+        bstart[b] = run;
+        for (unsigned t = 0; t < T; t++) {
+            const uint64_t c = cnt[(uint64_t)t * NBIN + b];
+            cnt[(uint64_t)t * NBIN + b] = run;
+            run += c;
+        }
+    }
+    bstart[NBIN] = run;
+    std::vector<uint64_t> tmp(n);
+    for (unsigned t = 0; t < T; t++)
+        th.emplace_back([&, t] {
+            const uint64_t a = t * chunk, b = std::min(n, a + chunk);
+            uint64_t *c = cnt.data() + (uint64_t)t * NBIN;
+            for (uint64_t i = a; i < b; i++) tmp[c[bin(i)]++] = keys[i];
+        });
+    for (auto &x : th) x.join();
+    th.clear();
+    const uint64_t sseed = mix(seed, 0x5348554646ull);
+    parallel_for(NBIN, [&](uint64_t lo, uint64_t hi) {
+        for (uint64_t b = lo; b < hi; b++) {
+            SplitMix64 r(mix(sseed, b));
+            uint64_t *base = tmp.data() + bstart[b];
+            for (uint64_t i = bstart[b + 1] - bstart[b]; i > 1; i--) std::swap(base[i - 1], base[r.below(i)]);
+        }
+    });
+    keys.swap(tmp);
+}
+
 }  // namespace
 
 extern "C" {
@@ -80,8 +203,9 @@ extern "C" {
 // draws descends `scale` levels, MSB first, picking quadrant (0,0) with
 // probability a, (0,1) b, (1,0) c and (1,1) d = 1-a-b-c from one uniform
 // double per level. Self-loops and duplicate edges are removed, the unique
-// edges are taken in (src, dst) order and then shuffled. Writes at most `cap`
-// edges and returns the number of unique edges in *n_out.
+// edges are taken in (src, dst) order and then shuffled (Fisher-Yates below
+// 2^26 edges, the binned parallel shuffle above). Writes at most `cap` edges
+// and returns the number of unique edges in *n_out.
 int gpma_gen_rmat(uint32_t scale, uint64_t ndraws, double a, double b, double c, uint64_t seed, uint32_t *src,
                   uint32_t *dst, uint64_t cap, uint64_t *n_out) {
     if (scale == 0 || scale > 32 || !n_out) return GPMA_ERR_ARG;
@@ -109,18 +233,27 @@ int gpma_gen_rmat(uint32_t scale, uint64_t ndraws, double a, double b, double c,
             keys[k] = (s == d) ? ~0ull : ((s << 32) | d);
         }
     });
-    std::sort(keys.begin(), keys.end());
-    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    sort_unique(keys, 32 + scale);
     if (!keys.empty() && keys.back() == ~0ull) keys.pop_back();
     uint64_t n = keys.size();
     *n_out = n;
     if (!src || !dst) return 0;
     if (cap < n) return GPMA_ERR_CAPACITY;
-    for (uint64_t i = 0; i < n; i++) {
-        src[i] = (uint32_t)(keys[i] >> 32);
-        dst[i] = (uint32_t)keys[i];
+    if (n < RMAT_BINNED_SHUFFLE) {
+        for (uint64_t i = 0; i < n; i++) {
+            src[i] = (uint32_t)(keys[i] >> 32);
+            dst[i] = (uint32_t)keys[i];
+        }
+        shuffle_edges(src, dst, n, seed);
+        return 0;
     }
-    shuffle_edges(src, dst, n, seed);
+    shuffle_binned(keys, seed);
+    parallel_for(n, [&](uint64_t lo, uint64_t hi) {
+        for (uint64_t i = lo; i < hi; i++) {
+            src[i] = (uint32_t)(keys[i] >> 32);
+            dst[i] = (uint32_t)keys[i];
+        }
+    });
     return 0;
 }
 
