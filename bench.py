@@ -313,6 +313,129 @@ def run_large(G, args):
     }
 
 
+def run_serve(G, args):
+    """Serving: R-MAT scale 24 (2^24 nodes), 2^28 draws (seed 4), first half
+    loaded; then an open-loop Poisson stream of requests through the request
+    server: 70% update batches (log-uniform 1..1e5 records, half inserts of
+    new edges, half deletes of the oldest, a circular sliding window over the
+    edge list, staged in HBM beforehand), 30% single-source BFS from uniform
+    sources. Reports p50/p99 latency per request class and sustained rates."""
+    import threading
+
+    scale, draws = 24, 1 << 28
+    N = 1 << scale
+    t0 = time.time()
+    src, dst = G.gen_rmat(scale, draws, seed=4)
+    keys = G.make_keys(src, dst)
+    del src, dst
+    E = len(keys)
+    half = E // 2
+    log(f"serve: {E} edges over {N} nodes in {time.time() - t0:.1f}s")
+    g = G.GPMA(N)
+    G.update_gpma(g, keys[:half], np.ones(half, np.uint8))
+    g.sync()
+    rng = np.random.default_rng(4)
+    R = args.serve_requests
+    is_upd = rng.random(R) < 0.7
+    sizes = np.where(is_upd, np.exp(rng.uniform(0, np.log(1e5), R)).astype(np.int64), 0)
+    sizes = (sizes // 2) * 2
+    sizes[is_upd & (sizes == 0)] = 2
+    starts = rng.integers(0, N, R).astype(np.uint32)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, R))
+    total = int(sizes.sum())
+    # update stream: request j inserts edges[ins : ins + h] and deletes edges[dele : dele + h] (indices mod E)
+    stream = np.empty(total, np.uint64)
+    vals = np.empty(total, np.uint8)
+    offs = np.zeros(R + 1, np.int64)
+    offs[1:] = np.cumsum(sizes)
+    def circ(a, h):  # keys[a .. a + h) over the circular edge list
+        a %= E
+        return keys[a: a + h] if a + h <= E else np.concatenate([keys[a:], keys[: a + h - E]])
+
+    ins, dele = half, 0
+    for j in np.nonzero(is_upd)[0]:
+        h = int(sizes[j] // 2)
+        a = int(offs[j])
+        stream[a: a + h] = circ(ins, h)
+        stream[a + h: a + 2 * h] = circ(dele, h)
+        vals[a: a + h] = 1
+        vals[a + h: a + 2 * h] = 0
+        ins += h
+        dele += h
+    dk = G.DeviceBuffer.from_numpy(stream)
+    dv = G.DeviceBuffer.from_numpy(vals)
+    del stream, vals
+    log(f"serve: {R} requests ({int(is_upd.sum())} updates, {total} records staged in HBM), "
+        f"open loop at {args.serve_rate:.0f} req/s")
+    tickets = [None] * R
+    results = [None] * R
+    srv = G.GpmaServer(g, order=args.serve_order)
+    done_evt = threading.Event()
+
+    def waiter():
+        for j in range(R):
+            while tickets[j] is None:
+                time.sleep(0.0002)
+            results[j] = srv.wait(tickets[j])
+        done_evt.set()
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    lag = np.zeros(R)
+    t_start = time.perf_counter()
+    last_log = t_start
+    for j in range(R):
+        now = time.perf_counter() - t_start
+        if arrivals[j] > now:
+            time.sleep(arrivals[j] - now)
+        lag[j] = time.perf_counter() - t_start - arrivals[j]
+        if is_upd[j]:
+            tickets[j] = srv.submit_update(dk.ptr + int(offs[j]) * 8, dv.ptr + int(offs[j]), int(sizes[j]))
+        else:
+            tickets[j] = srv.submit_bfs(int(starts[j]), levels=False)
+        if time.perf_counter() - last_log > 20:
+            last_log = time.perf_counter()
+            log(f"serve: {j + 1}/{R} submitted")
+    th.join()
+    t_total = time.perf_counter() - t_start
+    sst = srv.stats()
+    srv.close()
+    dk.free()
+    dv.free()
+    lat = np.array([r["latency_us"] for r in results]) / 1e3
+    st_ok = all(r["status"] == G.GPMA_OK for r in results)
+    # parity: live set == circular window edges[dele .. ins) + guards + 2 KEY_MAX (count / sum / xor)
+    win = circ(dele, ins - dele)
+    guards = (np.arange(N, dtype=np.uint64) << np.uint64(32)) | np.uint64(0xFFFFFFFF)
+    ce = [live_checksum(win), live_checksum(guards), live_checksum(np.array([G.KEY_MAX, G.KEY_MAX], np.uint64))]
+    expect = (sum(c[0] for c in ce), sum(c[1] for c in ce) % (1 << 64), ce[0][2] ^ ce[1][2] ^ ce[2][2])
+    ok_live = live_checksum(g.export_live()) == expect
+    # spot check: BFS requests after the last update see the final graph
+    tail = [j for j in range(R) if not is_upd[j] and not is_upd[j:].any()]
+    ok_bfs = all(G.gpma_bfs(g, int(starts[j]))[1] == results[j]["reachable"] for j in tail[:8]) if tail else None
+    g.close()
+
+    def pct(x, q):
+        return float(np.percentile(x, q)) if len(x) else None
+
+    upd_lat, bfs_lat = lat[is_upd], lat[~is_upd]
+    return {
+        "workload": f"R-MAT scale 24 (N=2^24), 2^28 draws (seed 4) -> {E} edges, first half loaded; {R} requests, "
+                    f"Poisson arrivals at {args.serve_rate:.0f}/s: 70% update batches log-uniform 1..1e5 records "
+                    "(1:1 insert:delete, circular window, staged in HBM), 30% single-source BFS (uniform sources); "
+                    f"server order {args.serve_order}",
+        "requests": R, "seconds": t_total, "offered_rate": args.serve_rate, "sustained_req_per_s": R / t_total,
+        "update_records_per_s": total / t_total, "bfs_per_s": int((~is_upd).sum()) / t_total,
+        "latency_ms": {"update_p50": pct(upd_lat, 50), "update_p99": pct(upd_lat, 99),
+                       "bfs_p50": pct(bfs_lat, 50), "bfs_p99": pct(bfs_lat, 99), "all_p99": pct(lat, 99)},
+        "submit_lag_ms": {"p50": pct(lag * 1e3, 50), "p99": pct(lag * 1e3, 99)},
+        "server": sst,
+        "mean_update_fusion": sst["update_records"] / max(sst["update_launches"], 1),
+        "mean_bfs_group": sst["bfs_requests"] / max(sst["bfs_launches"], 1),
+        "parity": {"all_ok": st_ok, "live_set_count_sum_xor": ok_live, "last_bfs_reach": ok_bfs},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -328,6 +451,11 @@ def main():
     ap.add_argument("--large", action="store_true",
                     help="R-MAT 2^25 nodes / 2^29 draws instead (bulk load, 2^21-update slides, 64-source BFS)")
     ap.add_argument("--large-steps", type=int, default=10)
+    ap.add_argument("--serve", action="store_true",
+                    help="request server under an open-loop Poisson stream on R-MAT 2^24 nodes / 2^28 draws")
+    ap.add_argument("--serve-requests", type=int, default=100000)
+    ap.add_argument("--serve-rate", type=float, default=5000.0)
+    ap.add_argument("--serve-order", default="snapshot", choices=("strict", "snapshot"))
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -339,6 +467,17 @@ def main():
 
     import gpma_bfs_amd as G
 
+    if args.serve:
+        r = run_serve(G, args)
+        log(f"serve: {r['sustained_req_per_s']:.0f} req/s sustained, update p50/p99 "
+            f"{r['latency_ms']['update_p50']:.2f}/{r['latency_ms']['update_p99']:.2f} ms, BFS p50/p99 "
+            f"{r['latency_ms']['bfs_p50']:.2f}/{r['latency_ms']['bfs_p99']:.2f} ms, parity {r['parity']}")
+        out = {"metric": METRIC, "value": r["update_records_per_s"], "unit": "edge-updates/s", "n_gpus": 1,
+               "steps": r["requests"], "warmup": 0, "ms_per_step": r["seconds"] / r["requests"] * 1e3,
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+               "data": "synthetic", "config": {"workload": r["workload"], "form": "serving"}, "serve": r}
+        print(json.dumps(out), flush=True)
+        return
     if args.large:
         r = run_large(G, args)
         log(f"large: {r['edge_updates_per_s'] / 1e9:.3f} G edge-updates/s, p99 {r['latency_ms']['p99']:.2f} ms, "

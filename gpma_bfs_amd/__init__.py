@@ -25,6 +25,9 @@ from ._lib import (  # noqa: F401
     GpmaConfig,
     GpmaCsr,
     GpmaError,
+    GpmaRequestResult,
+    GpmaServerConfig,
+    GpmaServerStats,
     GpmaStats,
     check,
     last_error,
@@ -304,3 +307,82 @@ def gen_powerlaw(N, E, alpha=2.1, soft_cap=2800.0, p1=0.24, seed=0):
 
 def make_keys(src, dst):
     return (src.astype(np.uint64) << np.uint64(32)) | dst.astype(np.uint64)
+
+
+class GpmaServer:
+    """Request server over one GPMA (include/gpma_serve.h): updates and BFS
+    requests from any thread take effect in ticket order; runs of queued
+    updates are fused into one launch, runs of BFS requests share one
+    multi-source traversal. The GPMA must not be used directly meanwhile."""
+
+    def __init__(self, gpma, max_bfs_group=0, hbm_budget_bytes=0, max_fused_records=0, max_queued_records=0,
+                 order="strict"):
+        """order: "strict" (every request sees exactly the requests ticketed
+        before it) or "snapshot" (a BFS may also see updates ticketed after
+        it, which lets queued BFS requests share one traversal)."""
+        cfg = GpmaServerConfig(ctypes.sizeof(GpmaServerConfig), max_bfs_group, hbm_budget_bytes, max_fused_records,
+                               max_queued_records, {"strict": 0, "snapshot": 1}[order])
+        h = ctypes.c_void_p()
+        check(lib.gpma_server_create(gpma.handle, ctypes.byref(cfg), ctypes.byref(h)), "gpma_server_create")
+        self._h = h
+        self.gpma = gpma
+        self._keep = {}  # ticket -> arrays the request reads or writes
+
+    def submit_update(self, keys, values, n=None):
+        """keys/values: host numpy arrays (copied at submission), or raw device
+        pointers (ints) with n records - those must stay valid until the
+        request completes."""
+        if isinstance(keys, int):
+            if not isinstance(values, int) or n is None:
+                raise GpmaError(GPMA_ERR_ARG, "device pointers need a values pointer and n")
+            kp, vp, dev = keys, values, 1
+        else:
+            keys = np.ascontiguousarray(keys, np.uint64)
+            values = np.ascontiguousarray(values, np.uint8)
+            if len(keys) != len(values):
+                raise GpmaError(GPMA_ERR_ARG, "keys and values differ in length")
+            kp, vp, n, dev = keys.ctypes.data, values.ctypes.data, len(keys), 0
+        t = ctypes.c_uint64()
+        check(lib.gpma_server_submit_update(self._h, kp, vp, n, dev, ctypes.byref(t)), "gpma_server_submit_update")
+        return t.value
+
+    def submit_bfs(self, start, levels=True):
+        """levels=True: a uint32[row_num] array is filled and returned by wait()."""
+        out = np.empty(self.gpma.row_num, np.uint32) if levels else None
+        t = ctypes.c_uint64()
+        check(lib.gpma_server_submit_bfs(self._h, start, out.ctypes.data if out is not None else None,
+                                         ctypes.byref(t)), "gpma_server_submit_bfs")
+        if out is not None:
+            self._keep[t.value] = out
+        return t.value
+
+    def wait(self, ticket, timeout_s=None):
+        """Result dict of a request (status, kind, reachable, group, queue_us,
+        service_us, latency_us, state_ticket, and levels for a BFS submitted
+        with levels)."""
+        r = GpmaRequestResult()
+        us = -1 if timeout_s is None else int(timeout_s * 1e6)
+        check(lib.gpma_server_wait(self._h, ticket, us, ctypes.byref(r)), "gpma_server_wait")
+        out = {name: getattr(r, name) for name, _ in GpmaRequestResult._fields_}
+        out["kind"] = "update" if r.kind == 0 else "bfs"
+        lv = self._keep.pop(ticket, None)
+        if lv is not None:
+            out["levels"] = lv
+        return out
+
+    def stats(self):
+        s = GpmaServerStats()
+        check(lib.gpma_server_get_stats(self._h, ctypes.byref(s)), "gpma_server_get_stats")
+        return {name: getattr(s, name) for name, _ in GpmaServerStats._fields_ if name != "reserved"}
+
+    def close(self):
+        """Serve everything submitted, then stop."""
+        if self._h:
+            check(lib.gpma_server_destroy(self._h), "gpma_server_destroy")
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -16,6 +16,10 @@ GPMA_ERR_OOM = 3
 GPMA_ERR_CAPACITY = 4
 GPMA_ERR_KEY_RANGE = 5
 GPMA_ERR_NO_DEVICE = 6
+GPMA_ERR_TIMEOUT = 7
+GPMA_ERR_STOPPED = 8
+GPMA_SERVE_STRICT = 0
+GPMA_SERVE_SNAPSHOT = 1
 
 ERROR_NAMES = {
     GPMA_ERR_ARG: "GPMA_ERR_ARG",
@@ -24,6 +28,8 @@ ERROR_NAMES = {
     GPMA_ERR_CAPACITY: "GPMA_ERR_CAPACITY",
     GPMA_ERR_KEY_RANGE: "GPMA_ERR_KEY_RANGE",
     GPMA_ERR_NO_DEVICE: "GPMA_ERR_NO_DEVICE",
+    GPMA_ERR_TIMEOUT: "GPMA_ERR_TIMEOUT",
+    GPMA_ERR_STOPPED: "GPMA_ERR_STOPPED",
 }
 
 KEY_NONE = 0xFFFFFFFFFFFFFFFF
@@ -76,7 +82,47 @@ class GpmaStats(ctypes.Structure):
     ]
 
 
-# name -> (restype, argtypes); mirrors include/gpma.h one to one
+class GpmaServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32),
+        ("max_bfs_group", ctypes.c_uint32),
+        ("hbm_budget_bytes", ctypes.c_uint64),
+        ("max_fused_records", ctypes.c_uint64),
+        ("max_queued_records", ctypes.c_uint64),
+        ("order", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 3),
+    ]
+
+
+class GpmaRequestResult(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int32),
+        ("kind", ctypes.c_uint32),
+        ("reachable", ctypes.c_uint32),
+        ("group", ctypes.c_uint32),
+        ("queue_us", ctypes.c_double),
+        ("service_us", ctypes.c_double),
+        ("latency_us", ctypes.c_double),
+        ("state_ticket", ctypes.c_uint64),
+    ]
+
+
+class GpmaServerStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_uint64),
+        ("completed", ctypes.c_uint64),
+        ("update_launches", ctypes.c_uint64),
+        ("update_records", ctypes.c_uint64),
+        ("bfs_launches", ctypes.c_uint64),
+        ("bfs_requests", ctypes.c_uint64),
+        ("rejected", ctypes.c_uint64),
+        ("max_fused_records", ctypes.c_uint64),
+        ("max_bfs_group", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+# name -> (restype, argtypes); mirrors include/gpma.h and include/gpma_serve.h one to one
 _P = ctypes.c_void_p
 _U32 = ctypes.c_uint32
 _U64 = ctypes.c_uint64
@@ -103,6 +149,15 @@ SIGNATURES = {
     "gpma_device_malloc": (_I, [_P, _U64]),
     "gpma_device_free": (_I, [_P]),
     "gpma_memcpy": (_I, [_P, _P, _U64, _I]),
+    "gpma_mem_info": (_I, [ctypes.c_int32, _P, _P]),
+    "gpma_copy_async": (_I, [_P, _P, _P, _U64, _I]),
+    "gpma_device_of": (_I, [_P]),
+    "gpma_server_create": (_I, [_P, _P, _P]),
+    "gpma_server_destroy": (_I, [_P]),
+    "gpma_server_submit_update": (_I, [_P, _P, _P, _U64, _I, _P]),
+    "gpma_server_submit_bfs": (_I, [_P, _U32, _P, _P]),
+    "gpma_server_wait": (_I, [_P, _U64, ctypes.c_int64, _P]),
+    "gpma_server_get_stats": (_I, [_P, _P]),
     "gpma_gen_rmat": (_I, [_U32, _U64, ctypes.c_double, ctypes.c_double, ctypes.c_double, _U64, _P, _P, _U64, _P]),
     "gpma_gen_powerlaw": (_I, [_U32, _U64, ctypes.c_double, ctypes.c_double, ctypes.c_double, _U64, _P, _P]),
 }

@@ -1,7 +1,7 @@
 """Build recipe for the gfx950 library (libgpma.so, in this directory).
 
 hipcc compiles each .hip translation unit for gfx950 only, with packed FP32
-instructions disabled; gen.cpp is plain host C++ (g++). Objects go to
+instructions disabled; gen.cpp and serve.cpp are plain host C++ (g++). Objects go to
 _build/ beside this file and the shared library lands next to the package.
 """
 import concurrent.futures
@@ -22,7 +22,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function",
 ]
 HIP_SOURCES = ["sort.hip", "msd.hip", "update.hip", "bfs.hip", "msbfs.hip", "gpma.hip"]
-CXX_SOURCES = ["gen.cpp"]
+CXX_SOURCES = ["gen.cpp", "serve.cpp"]
 
 
 def _run(cmd):
@@ -43,7 +43,7 @@ def build(verbose=False):
     for src in CXX_SOURCES:
         obj = os.path.join(BUILD, src + ".o")
         objs.append(obj)
-        jobs.append(["g++", "-O3", "-std=c++17", "-fPIC", "-c", os.path.join(CSRC, src), "-o", obj])
+        jobs.append(["g++", "-O3", "-std=c++17", "-fPIC", "-pthread", "-Wall", "-c", os.path.join(CSRC, src), "-o", obj])
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         for out in ex.map(_run, jobs):
             if verbose and out.strip():

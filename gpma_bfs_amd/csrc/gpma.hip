@@ -780,6 +780,20 @@ int gpma_device_malloc(void **ptr, uint64_t bytes) {
     return GPMA_OK;
 }
 
+int gpma_mem_info(int32_t device, uint64_t *free_bytes, uint64_t *total_bytes) {
+    if (!free_bytes || !total_bytes) return fail(GPMA_ERR_ARG, "gpma_mem_info: null output");
+    int old = 0;
+    HIP_TRY(hipGetDevice(&old));
+    if (device >= 0) HIP_TRY(hipSetDevice(device));
+    size_t f = 0, t = 0;
+    hipError_t e = hipMemGetInfo(&f, &t);
+    if (device >= 0) (void)hipSetDevice(old);
+    HIP_TRY(e);
+    *free_bytes = f;
+    *total_bytes = t;
+    return GPMA_OK;
+}
+
 int gpma_device_free(void *ptr) {
     if (ptr) HIP_TRY(hipFree(ptr));
     return GPMA_OK;
@@ -1038,5 +1052,17 @@ int gpma_sync(gpma_t g) {
 }
 
 void *gpma_stream(gpma_t g) { return g ? (void *)g->st : nullptr; }
+
+int gpma_copy_async(gpma_t g, void *dst, const void *src, uint64_t bytes, int kind) {
+    if (!g || (!dst && bytes) || (!src && bytes)) return fail(GPMA_ERR_ARG, "gpma_copy_async: null argument");
+    if (kind < 0 || kind > 2) return fail(GPMA_ERR_ARG, "gpma_copy_async: kind must be 0, 1 or 2");
+    if (!bytes) return GPMA_OK;
+    HIP_TRY(hipSetDevice(g->device));
+    const hipMemcpyKind k[3] = {hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice};
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, k[kind], g->st));
+    return GPMA_OK;
+}
+
+int gpma_device_of(gpma_t g) { return g ? g->device : -1; }
 
 }  // extern "C"

@@ -31,6 +31,8 @@ extern "C" {
 #define GPMA_ERR_CAPACITY 4  /* caller buffer too small, or u32 slot index would overflow */
 #define GPMA_ERR_KEY_RANGE 5 /* some update keys had src or dst >= row_num; they were skipped */
 #define GPMA_ERR_NO_DEVICE 6 /* no HIP device visible */
+#define GPMA_ERR_TIMEOUT 7   /* a wait ended before the request completed (gpma_serve.h) */
+#define GPMA_ERR_STOPPED 8   /* the server is shutting down (gpma_serve.h) */
 
 #define GPMA_KEY_NONE 0xFFFFFFFFFFFFFFFFull
 #define GPMA_KEY_MAX 0xFFFFFFFFFFFFFFFEull
@@ -143,6 +145,13 @@ int gpma_get_stats(gpma_t g, gpma_stats *out);
 /* Wait for all work queued on the handle's stream. */
 int gpma_sync(gpma_t g);
 
+/* Queue a copy on the handle's stream (kind as gpma_memcpy); ordered before
+ * the handle's next launches. Pageable host memory makes it synchronous. */
+int gpma_copy_async(gpma_t g, void *dst, const void *src, uint64_t bytes, int kind);
+
+/* HIP device ordinal the handle lives on (-1 for a null handle). */
+int gpma_device_of(gpma_t g);
+
 /* The HIP stream the handle launches on (as void*, a hipStream_t). */
 void *gpma_stream(gpma_t g);
 
@@ -164,6 +173,8 @@ int gpma_get_profile(gpma_t g, double *ms, uint64_t *calls, uint32_t n);
  * benchmarks): plain hipMalloc / hipFree / hipMemcpy on the current device.
  * kind: 0 host->device, 1 device->host, 2 device->device. */
 int gpma_device_malloc(void **ptr, uint64_t bytes);
+/* Free and total bytes of device memory on `device` (-1 = current). */
+int gpma_mem_info(int32_t device, uint64_t *free_bytes, uint64_t *total_bytes);
 int gpma_device_free(void *ptr);
 int gpma_memcpy(void *dst, const void *src, uint64_t bytes, int kind);
 

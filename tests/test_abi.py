@@ -13,13 +13,21 @@ import gpma_bfs_amd as G
 from gpma_bfs_amd import _lib
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "gpma.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("gpma.h", "gpma_serve.h")]
 
 
 def declared_functions():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:int|void\s*\*|const char\s*\*)\s*(gpma_\w+)\s*\(", text, flags=re.M)))
+    names = set()
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"^\s*(?:int|void\s*\*|const char\s*\*)\s*(gpma_\w+)\s*\(", text, flags=re.M))
+    return sorted(names)
+
+
+def test_every_c_header_is_checked():
+    found = sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h"))
+    assert found == sorted(os.path.basename(h) for h in HEADERS)
 
 
 def test_header_declares_expected_surface():
@@ -43,6 +51,9 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(_lib.GpmaConfig) == 32
     assert ctypes.sizeof(_lib.GpmaCsr) == 3 * 8 + 8 + 4 * 4
     assert ctypes.sizeof(_lib.GpmaStats) == 8 + 4 * 4 + 6 * 8
+    assert ctypes.sizeof(_lib.GpmaServerConfig) == 4 + 4 + 3 * 8 + 4 * 4
+    assert ctypes.sizeof(_lib.GpmaRequestResult) == 4 * 4 + 4 * 8
+    assert ctypes.sizeof(_lib.GpmaServerStats) == 8 * 8 + 2 * 4
 
 
 def test_no_device_paths_fail_loudly():
