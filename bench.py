@@ -35,6 +35,24 @@ METRIC = "edge-updates/sec + BFS GTEPS on 1 GPU, % HBM peak; batched vs plain; p
 S_REF = 16  # reference segment length at every capacity in [2^15, 2^30]
 
 
+TRAFFIC_FILE = "profiles/c2_update_hbm_traffic.md"
+
+
+def measured_traffic(phase):
+    """HBM bytes per batch of a phase's kernels on the power-law 1.63M-node
+    workload, from the calibrated FETCH_SIZE + WRITE_SIZE summary committed
+    beside this file (None if absent)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TRAFFIC_FILE)
+    key = {"sort": "sort_group", "locate": "locate", "leaf": "leaf"}[phase] + "_traffic_bytes_per_slide:"
+    try:
+        for line in open(path):
+            if line.startswith(key):
+                return int(line.split(":")[1])
+    except OSError:
+        return None
+    return None
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -451,6 +469,8 @@ def main():
     ap.add_argument("--large", action="store_true",
                     help="R-MAT 2^25 nodes / 2^29 draws instead (bulk load, 2^21-update slides, 64-source BFS)")
     ap.add_argument("--large-steps", type=int, default=10)
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="end with one device-to-device copy of this many MiB (counter calibration)")
     ap.add_argument("--serve", action="store_true",
                     help="request server under an open-loop Poisson stream on R-MAT 2^24 nodes / 2^28 draws")
     ap.add_argument("--serve-requests", type=int, default=100000)
@@ -593,14 +613,19 @@ def main():
     dom_ms = prof[dom]["ms"] / K
     dom_s = dom_ms / 1e3
     dom_launches = prof[dom]["launches"] / K
+    traffic = measured_traffic(dom) if not args.small else None
     roofline = {
-        "bound": "hbm", "kernel": {"sort": "k_sort_hist + k_sort_pass (radix sort of the batch)",
-                                   "leaf": "k_leaf<16> (leaf-level rebalance)",
-                                   "locate": "k_locate (batched leaf locate)"}[dom],
+        "bound": "hbm",
+        "kernel": {"sort": "batch sort group: k_msd_rank + k_msd_scan + k_msd_scatter + k_msd_local, "
+                           "one launch each per batch (k_stage_hist + k_sort_pass for batches that take the LSD sort)",
+                   "leaf": "k_leaf<16> (leaf-level rebalance)",
+                   "locate": "k_locate<16> (locate + in-place delete / modify)"}[dom],
         "achieved": attr[dom] / dom_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": attr[dom] / dom_s / 1e9 / HBM_PEAK_GBS, "traffic": None,
-        "algorithmic_bytes_per_launch": attr[dom] / max(dom_launches, 1), "launches_per_step": dom_launches,
-        "kernel_ms_per_step": dom_ms,
+        "frac": attr[dom] / dom_s / 1e9 / HBM_PEAK_GBS,
+        "traffic": traffic,
+        "traffic_source": TRAFFIC_FILE if traffic is not None else None,
+        "algorithmic_bytes_per_batch": attr[dom], "launches_per_batch": dom_launches,
+        "kernel_ms_per_batch": dom_ms,
     }
     step_roofline = {"algorithmic_bytes": ub["total"], "achieved": ub["total"] / step_s / 1e9,
                      "frac": ub["total"] / step_s / 1e9 / HBM_PEAK_GBS, "unit": "GB/s"}
@@ -663,6 +688,12 @@ def main():
     dk.free()
     dv.free()
     g.close()
+    if args.calib_copy_mb:
+        nbytes = args.calib_copy_mb << 20
+        a, b = G.DeviceBuffer(nbytes), G.DeviceBuffer(nbytes)
+        G.check(G.lib.gpma_memcpy(b.ptr, a.ptr, nbytes, 2), "gpma_memcpy")
+        a.free()
+        b.free()
 
 
 if __name__ == "__main__":
