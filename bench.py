@@ -36,6 +36,7 @@ S_REF = 16  # reference segment length at every capacity in [2^15, 2^30]
 
 
 TRAFFIC_FILE = "profiles/c2_update_hbm_traffic.md"
+TUNED_FILE = "gpma_bfs_amd/tuned/mi355x_powerlaw_1p6m.json"
 
 
 def measured_traffic(phase):
@@ -121,7 +122,12 @@ def run_batched(G, N, keys, half, steps, warmup, nsrc=64, fused=8, check=True):
     need = int(sizes.sum() // 2)
     if half + need > len(keys):
         raise RuntimeError("not enough edges for the requested batched steps")
-    g = G.GPMA(N)
+    knobs = {}
+    tuned = os.path.join(os.path.dirname(os.path.abspath(__file__)), TUNED_FILE)
+    if os.path.exists(tuned):
+        from gpma_bfs_amd import tune
+        knobs = tune.load(tuned)
+    g = G.GPMA(N, tuning=knobs)
     G.update_gpma(g, keys[:half], np.ones(half, np.uint8))
     # stage every step's fused batches in HBM
     ins, dele = half, 0
@@ -179,7 +185,8 @@ def run_batched(G, N, keys, half, steps, warmup, nsrc=64, fused=8, check=True):
            "msbfs_ms_per_step": bfs_t / steps * 1e3, "step_ms": t_all / steps * 1e3,
            "step_p50_ms": float(np.percentile(np.array(per) * 1e3, 50)),
            "step_p99_ms": float(np.percentile(np.array(per) * 1e3, 99)),
-           "phase_ms_per_step": brk, "phase_breakdown_steps": nprof}
+           "phase_ms_per_step": brk, "phase_breakdown_steps": nprof,
+           "tuning": {"file": TUNED_FILE if knobs else None, "knobs": knobs}}
     # algorithmic bytes and TEPS of the last step's BFS, and parity spot checks
     win = keys[dele: ins]
     outdeg = np.bincount((win >> np.uint64(32)).astype(np.int64), minlength=N)
@@ -457,14 +464,16 @@ def run_serve(G, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=85)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--bfs-reps", type=int, default=5)
     ap.add_argument("--cpu-slides", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--small", action="store_true", help="R-MAT 2^16/2^20 instead (quick check)")
-    ap.add_argument("--breakdown-slides", type=int, default=10,
+    ap.add_argument("--breakdown-slides", type=int, default=5,
                     help="slides before the timed ones with every phase timed (phase_ms_per_step)")
+    ap.add_argument("--event-every", type=int, default=3,
+                    help="time the dominant kernel group with events on every k-th timed slide")
     ap.add_argument("--batched-steps", type=int, default=10, help="batched-form steps (0 = skip)")
     ap.add_argument("--large", action="store_true",
                     help="R-MAT 2^25 nodes / 2^29 draws instead (bulk load, 2^21-update slides, 64-source BFS)")
@@ -553,28 +562,37 @@ def main():
         G.update_gpma_device(g, dk.ptr + i * nb * 8, dv.ptr + i * nb, nb)
     g.sync()
     # phase breakdown on separate slides (every phase timed with events)
-    brk = None
+    brk = brk_launches = None
     if nbrk:
         g.profile(True, reset=True)
         for i in range(args.warmup, args.warmup + nbrk):
             G.update_gpma_device(g, dk.ptr + i * nb * 8, dv.ptr + i * nb, nb)
         g.profile(False)
-        brk = {k: v["ms"] / nbrk for k, v in g.profile_report().items() if k != "bfs_expand"}
+        rep = g.profile_report()
+        brk = {k: v["ms"] / nbrk for k, v in rep.items() if k != "bfs_expand"}
+        brk_launches = {k: v["launches"] / nbrk for k, v in rep.items() if k != "bfs_expand"}
     attr_names = ("sort", "leaf", "locate")
     dom = max(attr_names, key=lambda k: brk[k]) if brk else "sort"
-    # timed slides: only the dominant kernel group carries events
+    # timed slides: the dominant kernel group carries events on every
+    # --event-every-th slide (each event pair costs a few microseconds of stream time)
     g.profile_phases([dom])
-    g.profile(True, reset=True)
+    g.profile(False, reset=True)
     per = []
+    sampled = 0
     g.sync()
     t_start = time.perf_counter()
-    for i in range(args.warmup + nbrk, nslides):
+    for j, i in enumerate(range(args.warmup + nbrk, nslides)):
+        ev = j % args.event_every == 0
+        if ev:
+            g.profile(True)
+            sampled += 1
         a = time.perf_counter()
         G.update_gpma_device(g, dk.ptr + i * nb * 8, dv.ptr + i * nb, nb)
         per.append(time.perf_counter() - a)
+        if ev:
+            g.profile(False)
     g.sync()
     t_total = time.perf_counter() - t_start
-    g.profile(False)
     g.profile_phases(None)
     prof = g.profile_report()
     K = args.steps
@@ -626,9 +644,9 @@ def main():
     step_s = t_total / K
     # attribute algorithmic bytes to the kernels that move them
     attr = {"sort": ub["sort"], "leaf": ub["leaf"], "locate": ub["locate"]}
-    dom_ms = prof[dom]["ms"] / K
+    dom_ms = prof[dom]["ms"] / max(sampled, 1)
     dom_s = dom_ms / 1e3
-    dom_launches = prof[dom]["launches"] / K
+    dom_launches = prof[dom]["launches"] / max(sampled, 1)
     traffic = measured_traffic(dom) if not args.small else None
     roofline = {
         "bound": "hbm",
@@ -641,6 +659,7 @@ def main():
         "traffic": traffic,
         "traffic_source": TRAFFIC_FILE if traffic is not None else None,
         "algorithmic_bytes_per_batch": attr[dom], "launches_per_batch": dom_launches,
+        "event_sampled_batches": sampled,
         "kernel_ms_per_batch": dom_ms,
     }
     step_roofline = {"algorithmic_bytes": ub["total"], "achieved": ub["total"] / step_s / 1e9,
@@ -649,6 +668,33 @@ def main():
                 "gteps": e_trav / bfs_s / 1e9, "kernel_ms": bfs_kernel_s * 1e3,
                 "algorithmic_bytes": bfs_b, "achieved_gbs": bfs_b / bfs_kernel_s / 1e9 if bfs_kernel_s else None,
                 "frac": (bfs_b / bfs_kernel_s / 1e9 / HBM_PEAK_GBS) if bfs_kernel_s else None}
+
+    # the same timed slides on a handle with the autotuned knobs (same live set)
+    tuned_plain = None
+    tuned_path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TUNED_FILE)
+    if not args.small and os.path.exists(tuned_path):
+        from gpma_bfs_amd import tune
+        knobs = tune.load(tuned_path)
+        gt = G.GPMA(N, tuning=knobs)
+        G.update_gpma(gt, keys[:half], np.ones(half, np.uint8))
+        first = args.warmup + nbrk
+        for i in range(first):
+            G.update_gpma_device(gt, dk.ptr + i * nb * 8, dv.ptr + i * nb, nb)
+        gt.sync()
+        a = time.perf_counter()
+        for i in range(first, nslides):
+            G.update_gpma_device(gt, dk.ptr + i * nb * 8, dv.ptr + i * nb, nb)
+        gt.sync()
+        t_t = time.perf_counter() - a
+        guards = (np.arange(N, dtype=np.uint64) << np.uint64(32)) | np.uint64(0xFFFFFFFF)
+        exp = np.sort(np.concatenate([keys[nslides * step: half + nslides * step], guards,
+                                      np.array([G.KEY_MAX, G.KEY_MAX], np.uint64)]))
+        tuned_plain = {"file": TUNED_FILE, "knobs": knobs, "edge_updates_per_s": K * nb / t_t,
+                       "ms_per_step": t_t / K * 1e3, "segment_length": gt.segment_length,
+                       "live_set_closed_form": bool(np.array_equal(gt.export_live(), exp))}
+        gt.close()
+        log(f"tuned knobs {knobs}: {tuned_plain['edge_updates_per_s'] / 1e9:.3f} G edge-updates/s on the same "
+            f"slides, live set ok {tuned_plain['live_set_closed_form']}")
 
     batched = None
     if args.batched_steps > 0 and not args.small:
@@ -693,11 +739,13 @@ def main():
         "step_roofline": step_roofline,
         "cpu_baseline": cpu,
         "bfs": bfs_info,
+        "tuned_plain": tuned_plain,
         "batched": batched,
         "latency_ms": {"p50": float(np.percentile(per_ms, 50)), "p99": float(np.percentile(per_ms, 99))},
         "host_buffer_edge_updates_per_s": host_ups,
         "phase_ms_per_step": brk,
         "phase_breakdown_slides": nbrk,
+        "phase_launches_per_step": brk_launches,
         "layout": {k: st[k] for k in ("capacity", "segment_length", "tree_height")},
         "init": {"guards_s": t_guard, "initial_window_s": t_load},
     }

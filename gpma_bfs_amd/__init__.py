@@ -54,8 +54,20 @@ def _ptr(a, dtype):
 class GPMA:
     """One device-resident GPMA (reference: class GPMA, gpma.cuh:31-102)."""
 
-    def __init__(self, row_num, device=-1, bulk_threshold=0, pull_permille=0):
-        cfg = GpmaConfig(ctypes.sizeof(GpmaConfig), device, bulk_threshold, pull_permille)
+    def __init__(self, row_num, device=-1, bulk_threshold=0, pull_permille=0, segment_length=0, device_levels=0,
+                 sort_bucket_bits=0, tuning=None):
+        """tuning: a dict of the knobs below (as written by gpma_bfs_amd.tune); explicit
+        arguments win over it. None of them changes the live keys or BFS results."""
+        t = dict(tuning or {})
+        kw = {"pull_permille": pull_permille, "segment_length": segment_length, "device_levels": device_levels,
+              "sort_bucket_bits": sort_bucket_bits}
+        for k, v in kw.items():
+            if v:
+                t[k] = v
+        self.tuning = {k: int(t.get(k, 0)) for k in kw}
+        cfg = GpmaConfig(ctypes.sizeof(GpmaConfig), device, bulk_threshold, self.tuning["pull_permille"],
+                         self.tuning["segment_length"], self.tuning["device_levels"],
+                         self.tuning["sort_bucket_bits"])
         h = ctypes.c_void_p()
         check(lib.gpma_create(row_num, ctypes.byref(cfg), ctypes.byref(h)), "gpma_create")
         self._h = h
@@ -126,7 +138,9 @@ class GPMA:
 
     def profile(self, enable=True, reset=False):
         """Per-phase HIP-event timing on the handle's stream."""
-        check(lib.gpma_profile(self.handle, 2 if (enable and reset) else int(bool(enable))), "gpma_profile")
+        if reset:
+            check(lib.gpma_profile(self.handle, 2), "gpma_profile")
+        check(lib.gpma_profile(self.handle, int(bool(enable))), "gpma_profile")
 
     def profile_phases(self, names=None):
         """Time only the named phases (None: all of them)."""

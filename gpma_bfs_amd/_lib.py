@@ -49,7 +49,10 @@ class GpmaConfig(ctypes.Structure):
         ("device", ctypes.c_int32),
         ("bulk_threshold", ctypes.c_uint32),
         ("pull_permille", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 4),
+        ("segment_length", ctypes.c_uint32),
+        ("device_levels", ctypes.c_uint32),
+        ("sort_bucket_bits", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 

@@ -114,8 +114,12 @@ constexpr uint32_t N_TICKETS = C_COUNT - C_TICKET;
 constexpr uint64_t MSD_MAX_N = 4ull << 20;
 constexpr uint32_t MSD_BIG_BUCKET = 16384;
 constexpr uint32_t MSD_COOL_BATCHES = 64;
-// Levels 1..SPEC_LEVELS run without a host round trip; the host reads the
-// counters once after them and drives any higher level itself.
+// Levels 1..spec_levels (default SPEC_LEVELS) run without a host round trip;
+// the host reads the counters once after them and drives any higher level
+// itself. Each device level costs two launches on every batch, each host
+// level a round trip on the batches that reach it; over the 100 slides of the
+// power-law 1.63M-node protocol (rebalance depth grows from 1-2 to 4-5) three
+// device levels is the best default. device_levels tunes it.
 constexpr uint32_t SPEC_LEVELS = 3;
 
 }  // namespace
@@ -128,6 +132,9 @@ struct gpma_handle {
     uint32_t S = 0, h = 0;
     std::vector<uint32_t> lower, upper;
     uint32_t bulk_threshold = 5u * 1000u * 1000u;
+    uint32_t seg_override = 0;  // segment length after a resize (0: the reference formula)
+    uint32_t spec_levels = SPEC_LEVELS;
+    uint32_t sort_bits = 0;     // bucket bits of the batch sort (0: from the batch size)
     uint64_t dev_bytes = 0;
     // structure
     Buf keys, values, seps, row_offset;
@@ -258,6 +265,16 @@ int ensure_tmp(gpma_handle *g, uint64_t n) {
     return GPMA_OK;
 }
 
+// S, h and the level bounds for a T-slot array: gpma.cuh:917-920 (S =
+// 2^(fls(fls T) - 1), h = fls(T / S) - 1), or the configured segment length
+// (capped at T / 2).
+void size_tree(gpma_handle *g, uint64_t T) {
+    g->S = 1u << (fls32(fls32((uint32_t)T)) - 1);
+    if (g->seg_override) g->S = (uint32_t)std::min<uint64_t>(g->seg_override, T / 2);
+    g->h = fls32((uint32_t)(T / g->S)) - 1;
+    recalc_density(g);
+}
+
 // Live count of window [w, w+W) (one host sync), leaving chunk offsets for
 // a following compact_live.
 int count_live(gpma_handle *g, uint64_t w, uint64_t W, uint64_t *c) {
@@ -288,9 +305,7 @@ int rebuild(gpma_handle *g, const gkey *pk, const gval *pv, uint64_t np, bool ch
     uint64_t T = 4;
     while (std::floor(0.84 * (double)T) < (double)merge) T *= 2;
     if (T > (1ull << 31)) return fail(GPMA_ERR_CAPACITY, "packed array would exceed 2^31 slots");
-    g->S = 1u << (fls32(fls32((uint32_t)T)) - 1);
-    g->h = fls32((uint32_t)(T / g->S)) - 1;
-    recalc_density(g);
+    size_tree(g, T);
     if (check_bounds) {
         // The reference checks nothing here and loses the pending inserts when
         // the trimmed root bound is below live + pending (gpma.cuh:1034-1044);
@@ -298,9 +313,7 @@ int rebuild(gpma_handle *g, const gkey *pk, const gval *pv, uint64_t np, bool ch
         while (merge > g->upper[g->h]) {
             T *= 2;
             if (T > (1ull << 31)) return fail(GPMA_ERR_CAPACITY, "packed array would exceed 2^31 slots");
-            g->S = 1u << (fls32(fls32((uint32_t)T)) - 1);
-            g->h = fls32((uint32_t)(T / g->S)) - 1;
-            recalc_density(g);
+            size_tree(g, T);
         }
     }
     if (T != g->C) {
@@ -445,7 +458,7 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
     const bool use_msd = msd_sort_fits(g->N) && n <= MSD_MAX_N && g->msd_cool == 0;
     if (g->msd_cool) g->msd_cool--;
     if (use_msd) {
-        const MsdPlan pl = msd_plan(n, g->N);
+        const MsdPlan pl = msd_plan(n, g->N, g->sort_bits);
         TRY(ensure_msd(g, n, pl));
         MsdWork w{g->msd_kp.as<uint64_t>(), g->msd_rk.as<uint16_t>(), g->sk1.as<uint64_t>(), g->msd_mat.as<uint32_t>(),
                   g->msd_bstart.as<uint32_t>(), g->msd_status.as<unsigned long long>()};
@@ -491,7 +504,7 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
     phase_end(g, P_LEAF, 1);
     compact_pending(0, n, 0, ctr + C_LVL, ctr + C_LVL + 1);
     uint32_t spec = 0;
-    while (spec < SPEC_LEVELS && spec + 1 <= g->h && ((uint64_t)g->S << (spec + 1)) <= WIN_MAX) {
+    while (spec < g->spec_levels && spec + 1 <= g->h && ((uint64_t)g->S << (spec + 1)) <= WIN_MAX) {
         const uint32_t L = spec + 1;
         const int pc = L & 1;
         phase_begin(g, P_UPPER);
@@ -824,6 +837,19 @@ int gpma_create(uint32_t row_num, const gpma_config *cfg, gpma_t *out) {
         else hipGetDevice(&g->device);
         if (cfg->bulk_threshold) g->bulk_threshold = cfg->bulk_threshold;
         if (cfg->pull_permille) g->pull_fraction = cfg->pull_permille / 1000.0;
+        const uint32_t sl = cfg->segment_length;
+        if (sl && (sl < 4 || sl > 32 || (sl & (sl - 1)))) {
+            delete g;
+            return fail(GPMA_ERR_ARG, "gpma_create: segment_length must be 4, 8, 16 or 32");
+        }
+        if (cfg->device_levels > 8 || (cfg->sort_bucket_bits && (cfg->sort_bucket_bits < 4 ||
+                                                                    cfg->sort_bucket_bits > 12))) {
+            delete g;
+            return fail(GPMA_ERR_ARG, "gpma_create: device_levels must be <= 8, sort_bucket_bits 4..12");
+        }
+        g->seg_override = sl;
+        if (cfg->device_levels) g->spec_levels = cfg->device_levels;
+        g->sort_bits = cfg->sort_bucket_bits;
     } else {
         hipGetDevice(&g->device);
     }

@@ -41,7 +41,7 @@ struct MsdPlan {
     uint32_t NB;      // buckets
     uint32_t nscan;   // scan blocks (status words used)
 };
-MsdPlan msd_plan(uint64_t n, uint32_t N);
+MsdPlan msd_plan(uint64_t n, uint32_t N, uint32_t bucket_bits);  // bucket_bits 0: from n
 struct MsdWork {
     uint64_t *kp;               // n packed keys
     uint16_t *rk;               // n ranks

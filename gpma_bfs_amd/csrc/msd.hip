@@ -314,7 +314,7 @@ __global__ __launch_bounds__(MT) void k_msd_local(uint64_t *__restrict__ kb, uin
 
 bool msd_sort_fits(uint32_t N) { return N > 0 && N < (1u << 28); }
 
-MsdPlan msd_plan(uint64_t n, uint32_t N) {
+MsdPlan msd_plan(uint64_t n, uint32_t N, uint32_t bucket_bits) {
     MsdPlan pl{};
     uint32_t bs = 0;
     while (bs < 32 && (N >> bs) != 0) bs++;  // bit width of N itself
@@ -322,6 +322,7 @@ MsdPlan msd_plan(uint64_t n, uint32_t N) {
     const uint32_t B = 2 * bs;
     uint32_t D = 1;
     while (D < MAX_D && (n >> (D + 6)) != 0) D++;  // about 64 items per bucket
+    if (bucket_bits) D = std::min(bucket_bits, MAX_D);
     if (D > B) D = B;
     pl.D = D;
     pl.R = B - D;

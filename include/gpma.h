@@ -47,7 +47,13 @@ typedef struct gpma_config {
     uint32_t bulk_threshold;   /* pending inserts that take the bulk path (reference: 5,000,000) */
     uint32_t pull_permille;    /* BFS streams the whole array when the frontier's slots exceed
                                   this many thousandths of the capacity (0 = 250) */
-    uint32_t reserved[4];
+    /* Tuning knobs (0 = default). None changes the live-key set or BFS results;
+     * segment_length changes the physical layout (the reference's formula is
+     * S = 2^(fls(fls C) - 1)). */
+    uint32_t segment_length;   /* slots per leaf after a resize / bulk load: 4, 8, 16 or 32 */
+    uint32_t device_levels;    /* rebalance levels 1..k queued without a host round trip (1..8, 0 = 3) */
+    uint32_t sort_bucket_bits; /* bucket bits of the batch sort (4..12, 0 = from the batch size) */
+    uint32_t reserved;
 } gpma_config;
 
 /* State summary. Pointers are device pointers owned by the handle and are

@@ -110,3 +110,4 @@ def test_bench_rejects_multi_gpu():
                        stderr=subprocess.PIPE, text=True, timeout=120)
     assert p.returncode == 2
     assert "one GPU" in p.stderr
+

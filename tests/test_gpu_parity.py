@@ -293,3 +293,41 @@ def test_skewed_batches_sort_paths():
         assert np.array_equal(g.export_live(), o.live()), f"batch {i}"
         assert_layout_equal(g, o, f"batch {i}")
         assert_invariants(g)
+
+
+@pytest.mark.parametrize("knobs", [{"segment_length": 8}, {"segment_length": 32}, {"segment_length": 4},
+                                   {"device_levels": 1}, {"device_levels": 8},
+                                   {"sort_bucket_bits": 4}, {"sort_bucket_bits": 12}])
+def test_tuning_knobs_keep_results(knobs):
+    """Tuning knobs change timing (and, for segment_length, the layout) but
+    never the live-key set or BFS levels: 20 protocol slides on an R-MAT
+    graph with the closed-form live set after every slide, and BFS equal to
+    a textbook BFS over the final window."""
+    src, dst = G.gen_rmat(14, 1 << 17, seed=5)
+    N = 1 << 14
+    keys = G.make_keys(src, dst)
+    g = G.GPMA(N, tuning=knobs)
+    half = len(keys) // 2
+    G.update_gpma(g, keys[:half], np.ones(half, np.uint8))
+    assert np.array_equal(g.export_live(), closed_form_live(keys, N, 0, num_slide=20)), "window load"
+    assert_invariants(g)
+    for i, b, v in protocol_batches(keys, 20, num_slide=20):
+        G.update_gpma(g, b, v)
+        assert np.array_equal(g.export_live(), closed_form_live(keys, N, i + 1, num_slide=20)), f"slide {i}"
+    assert_invariants(g)
+    if "segment_length" in knobs:
+        assert g.segment_length == knobs["segment_length"]
+    step = half // 20
+    win = keys[20 * step: half + 20 * step]
+    ws, wd = (win >> np.uint64(32)).astype(np.uint32), (win & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    for s0 in (0, 7, 1234):
+        lv, reach = G.gpma_bfs(g, s0)
+        elv, ereach = O.bfs_edges(N, ws, wd, s0)
+        assert np.array_equal(lv, elv) and reach == ereach, f"start {s0}"
+
+
+def test_unsupported_tuning_is_rejected():
+    for bad in ({"segment_length": 2}, {"segment_length": 12}, {"device_levels": 9}, {"sort_bucket_bits": 3}):
+        with pytest.raises(G.GpmaError) as e:
+            G.GPMA(64, tuning=bad)
+        assert e.value.code == G.GPMA_ERR_ARG
