@@ -242,6 +242,7 @@ constexpr uint32_t CC_THREADS = 256;
 constexpr uint32_t CC_ROWS = 8;                        // 64-entry rows per wave
 constexpr uint32_t CC_WAVE_ITEMS = WAVE * CC_ROWS;     // 512
 constexpr uint32_t CC_TILE = CC_THREADS * CC_ROWS;     // 2048
+constexpr uint32_t CC_GRID_MAX = 256;                  // workgroups; each loops over tiles
 
 __global__ __launch_bounds__(CC_THREADS) void k_compact_chained(
     const gkey *__restrict__ ik, const gval *__restrict__ iv, const uint32_t *__restrict__ iw,
@@ -252,6 +253,9 @@ __global__ __launch_bounds__(CC_THREADS) void k_compact_chained(
     __shared__ uint32_t s_tile, s_excl;
     __shared__ uint32_t wtot[CC_THREADS / WAVE];
     const uint64_t n = d_n ? std::min<uint64_t>(*d_n, n_ub) : n_ub;
+    // a bounded grid takes tiles by ticket until the list is exhausted
+    for (;;) {
+    __syncthreads();  // s_tile, wtot and s_excl of the previous tile are consumed
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
@@ -300,6 +304,7 @@ __global__ __launch_bounds__(CC_THREADS) void k_compact_chained(
             odone[d] = 0;
         }
         o += (uint32_t)__popcll(km[j]);
+    }
     }
 }
 
@@ -493,6 +498,7 @@ __device__ void window_one(uint32_t S, uint32_t W, const gkey *__restrict__ pk, 
 
 
 // Workgroups stride over the pending list; each run head is one window.
+constexpr uint64_t WINDOW_GRID = 1024;
 __global__ __launch_bounds__(256) void k_window(uint32_t S, uint32_t W, const gkey *__restrict__ pk,
                                                 const gval *__restrict__ pv, const uint32_t *__restrict__ pw,
                                                 uint8_t *__restrict__ pd, const uint32_t *__restrict__ d_np,
@@ -704,7 +710,7 @@ void launch_compact_chained(const gkey *ik, const gval *iv, const uint32_t *iw, 
                             const uint32_t *d_n, gkey *ok, gval *ov, uint32_t *ow, uint8_t *odone, uint32_t mask,
                             unsigned long long *status, uint32_t epoch, uint32_t *ticket, uint32_t *d_out,
                             uint32_t *err, hipStream_t st) {
-    const uint64_t nt = std::max<uint64_t>(1, (n_ub + CC_TILE - 1) / CC_TILE);
+    const uint64_t nt = std::max<uint64_t>(1, std::min<uint64_t>((n_ub + CC_TILE - 1) / CC_TILE, CC_GRID_MAX));
     hipLaunchKernelGGL(k_compact_chained, dim3((uint32_t)nt), dim3(CC_THREADS), 0, st, ik, iv, iw, idone, n_ub, d_n,
                        ok, ov, ow, odone, mask, status, epoch, ticket, d_out, err);
 }
@@ -740,7 +746,7 @@ void launch_window(uint32_t S, uint32_t W, const gkey *pk, const gval *pv, const
                    uint32_t *row_offset, uint32_t lower, uint32_t upper, const uint32_t *d_gate, uint32_t gate,
                    hipStream_t st) {
     if (!n_ub) return;
-    hipLaunchKernelGGL(k_window, dim3((uint32_t)std::min<uint64_t>(n_ub, 1024)), dim3(256), 0, st, S, W, pk, pv, pw,
+    hipLaunchKernelGGL(k_window, dim3((uint32_t)std::min<uint64_t>(n_ub, WINDOW_GRID)), dim3(256), 0, st, S, W, pk, pv, pw,
                        pd, d_np, keys, values, seps, row_offset, lower, upper, d_gate, gate);
 }
 
