@@ -7,7 +7,8 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libgpma.so")
+# GPMA_LIB selects another build of the library (e.g. a variant under test)
+LIB_PATH = os.environ.get("GPMA_LIB") or os.path.join(HERE, "libgpma.so")
 
 GPMA_OK = 0
 GPMA_ERR_ARG = 1
