@@ -42,8 +42,11 @@ __device__ __forceinline__ uint32_t rlen(const uint32_t *ro, uint32_t v) {
     return e > b ? e - b : 0;
 }
 
+// Masks start empty. The [N][64] level staging is not cleared: the output
+// reads an entry only where the source's bit is set in seen[v], and every
+// such entry was written when that bit was set.
 __global__ void k_ms_init(uint64_t *__restrict__ seen, uint64_t *__restrict__ fr0, uint64_t *__restrict__ fr1,
-                          uint64_t *__restrict__ nx, uint16_t *__restrict__ lv16, uint32_t N) {
+                          uint64_t *__restrict__ nx, uint32_t N) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     for (uint64_t i = t; i < N; i += stride) {
@@ -52,39 +55,44 @@ __global__ void k_ms_init(uint64_t *__restrict__ seen, uint64_t *__restrict__ fr
         fr1[i] = 0;
         nx[i] = 0;
     }
-    uint4 *l = (uint4 *)lv16;
-    const uint64_t nl = (uint64_t)N * 64 * 2 / 16;
-    for (uint64_t i = t; i < nl; i += stride) l[i] = make_uint4(0, 0, 0, 0);
 }
-
-// Sources: bit i for starts[i]; the frontier list gets each distinct start
-// once (the first index holding it queues it).
+// One lane per source: mark its bit, stage level 1, and queue the first
+// occurrence of each start vertex (short or long range list).
 __global__ void k_ms_sources(const uint32_t *__restrict__ starts, uint32_t nsrc, uint64_t *__restrict__ seen,
                              uint64_t *__restrict__ fr, uint16_t *__restrict__ lv16, uint32_t *__restrict__ q,
                              uint32_t *__restrict__ qbig, unsigned long long *__restrict__ cnt,
                              unsigned long long *__restrict__ cnt_big, uint32_t *__restrict__ tcnt,
                              const uint32_t *__restrict__ ro) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    cnt[0] = cnt_big[0] = 0;
-    tcnt[0] = tcnt[1] = 0;
-    unsigned long long c = 0, cb = 0;
-    for (uint32_t i = 0; i < nsrc; i++) {
-        const uint32_t s = starts[i];
-        const uint64_t bit = 1ull << i;
-        bool first = true;
-        for (uint32_t j = 0; j < i; j++)
-            if (starts[j] == s) first = false;
-        seen[s] |= bit;
-        fr[s] |= bit;
-        lv16[(uint64_t)s * 64 + i] = 1;
-        if (first) {
-            const unsigned long long len = rlen(ro, s);
-            if (len > MS_BIG_RANGE) qbig[(uint32_t)cb] = s, cb += (len << 32) | 1ull;
-            else q[(uint32_t)c] = s, c += (len << 32) | 1ull;
-        }
+    __shared__ uint32_t ss[WAVE];
+    const uint32_t lane = threadIdx.x;  // one wave
+    const bool valid = lane < nsrc;
+    const uint32_t s = valid ? starts[lane] : 0;
+    ss[lane] = s;
+    __syncthreads();
+    bool first = valid;
+    for (uint32_t j = 0; j < lane; j++)
+        if (ss[j] == s) first = false;
+    if (valid) {
+        atomicOr((unsigned long long *)(seen + s), 1ull << lane);
+        atomicOr((unsigned long long *)(fr + s), 1ull << lane);
+        lv16[(uint64_t)s * 64 + lane] = 1;
     }
-    cnt[1] = c;
-    cnt_big[1] = cb;
+    const uint64_t len = first ? rlen(ro, s) : 0;
+    const bool big = first && len > MS_BIG_RANGE, small = first && !big;
+    const uint64_t ms = ballot(small), mb = ballot(big);
+    if (small) q[__popcll(ms & lanemask_lt())] = s;
+    if (big) qbig[__popcll(mb & lanemask_lt())] = s;
+    uint64_t ws = small ? len : 0, wb = big ? len : 0;
+    for (uint32_t d = WAVE / 2; d > 0; d >>= 1) {
+        ws += __shfl_xor(ws, d, WAVE);
+        wb += __shfl_xor(wb, d, WAVE);
+    }
+    if (lane == 0) {
+        cnt[0] = cnt_big[0] = 0;
+        tcnt[0] = tcnt[1] = 0;
+        cnt[1] = (ws << 32) | (unsigned long long)__popcll(ms);
+        cnt_big[1] = (wb << 32) | (unsigned long long)__popcll(mb);
+    }
 }
 
 struct MsQueue {  // LDS staging of touched vertices
@@ -374,16 +382,22 @@ __global__ __launch_bounds__(T) void k_ms_finalize(const uint32_t *__restrict__ 
     }
 }
 
-// Transpose the vertex-major u16 levels into source-major u32 arrays and
-// count reachable vertices per source (partials per workgroup).
-__global__ __launch_bounds__(T) void k_ms_output(const uint16_t *__restrict__ lv16, uint32_t N, uint32_t nsrc,
-                                                 uint32_t *__restrict__ out, uint32_t *__restrict__ partial) {
+// Transpose the vertex-major u16 levels into source-major u32 arrays (an
+// entry counts only where the source's bit is set in seen[v]) and count
+// reachable vertices per source: wave w writes the rows of sources w, w+4,
+// ... as coalesced 64-vertex runs and counts them with a ballot; partials per
+// workgroup.
+__global__ __launch_bounds__(T) void k_ms_output(const uint16_t *__restrict__ lv16, const uint64_t *__restrict__ seen,
+                                                 uint32_t N, uint32_t nsrc, uint32_t *__restrict__ out,
+                                                 uint32_t *__restrict__ partial) {
+    constexpr uint32_t NW = T / WAVE;
     __shared__ uint16_t tile[64][64 + 2];
-    __shared__ uint32_t reach[64];
-    if (threadIdx.x < 64) reach[threadIdx.x] = 0;
-    __syncthreads();
+    __shared__ uint64_t sm[64];
+    const uint32_t wid = threadIdx.x / WAVE, lane = lane_id();
+    uint32_t acc[64 / NW];
+#pragma unroll
+    for (uint32_t k = 0; k < 64 / NW; k++) acc[k] = 0;
     for (uint64_t v0 = (uint64_t)blockIdx.x * 64; v0 < N; v0 += (uint64_t)gridDim.x * 64) {
-        // load 64 vertices x 64 sources (each vertex's 128-byte line)
         for (uint32_t i = threadIdx.x; i < 64 * 32; i += T) {
             const uint32_t vv = i / 32, pair = i % 32;
             uint32_t x = 0;
@@ -391,27 +405,40 @@ __global__ __launch_bounds__(T) void k_ms_output(const uint16_t *__restrict__ lv
             tile[vv][2 * pair] = (uint16_t)x;
             tile[vv][2 * pair + 1] = (uint16_t)(x >> 16);
         }
+        if (threadIdx.x < 64) sm[threadIdx.x] = v0 + threadIdx.x < N ? seen[v0 + threadIdx.x] : 0;
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < 64 * 64; i += T) {
-            const uint32_t s = i / 64, vv = i % 64;
-            if (s < nsrc && v0 + vv < N) {
-                const uint32_t l = tile[vv][s];
-                out[(uint64_t)s * N + v0 + vv] = l;
-                if (l) atomicAdd(&reach[s], 1u);
+        const bool in = v0 + lane < N;
+        const uint64_t mask = sm[lane];
+#pragma unroll
+        for (uint32_t k = 0; k < 64 / NW; k++) {
+            const uint32_t s = wid + NW * k;
+            if (s < nsrc) {
+                const uint32_t l = ((mask >> s) & 1ull) ? tile[lane][s] : 0u;
+                if (in) out[(uint64_t)s * N + v0 + lane] = l;
+                acc[k] += (uint32_t)__popcll(ballot(l != 0));
             }
         }
         __syncthreads();
     }
-    if (threadIdx.x < nsrc) partial[(uint64_t)blockIdx.x * 64 + threadIdx.x] = reach[threadIdx.x];
+    if (lane == 0) {
+#pragma unroll
+        for (uint32_t k = 0; k < 64 / NW; k++) {
+            const uint32_t s = wid + NW * k;
+            if (s < nsrc) partial[(uint64_t)blockIdx.x * 64 + s] = acc[k];
+        }
+    }
 }
 
-__global__ void k_ms_reach(const uint32_t *__restrict__ partial, uint32_t nblocks, uint32_t nsrc,
-                           uint32_t *__restrict__ reach) {
-    const uint32_t s = threadIdx.x;
-    if (s >= nsrc) return;
+// reach[s] = sum of the workgroups' partials (one workgroup per source).
+__global__ __launch_bounds__(T) void k_ms_reach(const uint32_t *__restrict__ partial, uint32_t nblocks,
+                                                uint32_t *__restrict__ reach) {
+    __shared__ uint32_t scratch[T / WAVE + 1];
+    const uint32_t s = blockIdx.x;
     uint32_t r = 0;
-    for (uint32_t b = 0; b < nblocks; b++) r += partial[(uint64_t)b * 64 + s];
-    reach[s] = r;
+    for (uint32_t b = threadIdx.x; b < nblocks; b += T) r += partial[(uint64_t)b * 64 + s];
+    uint32_t tot;
+    (void)block_exclusive_scan(r, scratch, &tot);
+    if (threadIdx.x == 0) reach[s] = tot;
 }
 
 __global__ void k_ms_clear(unsigned long long *__restrict__ cnt, unsigned long long *__restrict__ cnt_big,
@@ -427,7 +454,7 @@ __global__ void k_ms_clear(unsigned long long *__restrict__ cnt, unsigned long l
 
 void ms_init(const MsState &m, uint32_t N, const uint32_t *starts, uint32_t nsrc, const uint32_t *ro,
              hipStream_t st) {
-    hipLaunchKernelGGL(k_ms_init, dim3(2048), dim3(256), 0, st, m.seen, m.fr[0], m.fr[1], m.nx, m.lv16, N);
+    hipLaunchKernelGGL(k_ms_init, dim3(2048), dim3(256), 0, st, m.seen, m.fr[0], m.fr[1], m.nx, N);
     hipLaunchKernelGGL(k_ms_sources, dim3(1), dim3(64), 0, st, starts, nsrc, m.seen, m.fr[1], m.lv16, m.q[1],
                        m.qbig[1], m.cnt, m.cnt_big, m.tcnt, ro);
 }
@@ -451,8 +478,8 @@ void ms_level(const gkey *keys, const gval *values, uint64_t C, uint32_t N, cons
 
 void ms_output(const MsState &m, uint32_t N, uint32_t nsrc, uint32_t *out, uint32_t *reach, hipStream_t st) {
     const uint32_t nb = (uint32_t)std::min<uint64_t>(2048, ((uint64_t)N + 63) / 64);
-    hipLaunchKernelGGL(k_ms_output, dim3(std::max(nb, 1u)), dim3(T), 0, st, m.lv16, N, nsrc, out, m.partial);
-    hipLaunchKernelGGL(k_ms_reach, dim3(1), dim3(64), 0, st, m.partial, std::max(nb, 1u), nsrc, reach);
+    hipLaunchKernelGGL(k_ms_output, dim3(std::max(nb, 1u)), dim3(T), 0, st, m.lv16, m.seen, N, nsrc, out, m.partial);
+    hipLaunchKernelGGL(k_ms_reach, dim3(nsrc), dim3(T), 0, st, m.partial, std::max(nb, 1u), reach);
 }
 
 uint32_t ms_output_blocks(uint32_t N) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(2048, ((uint64_t)N + 63) / 64)); }
