@@ -208,8 +208,8 @@ def run_batched(G, N, keys, half, steps, warmup, nsrc=64, fused=8, check=True):
     out.update({"update_algorithmic_bytes_per_step": ub / steps,
                 "update_achieved_gbs": ub / upd_t / 1e9, "update_frac": ub / upd_t / 1e9 / HBM_PEAK_GBS,
                 "step_frac": (ub / steps + mb) / (t_all / steps) / 1e9 / HBM_PEAK_GBS})
+    live = g.export_live() if check else None
     if check:
-        live = g.export_live()
         guards = (np.arange(N, dtype=np.uint64) << np.uint64(32)) | np.uint64(0xFFFFFFFF)
         expect = np.sort(np.concatenate([win, guards, np.array([G.KEY_MAX, G.KEY_MAX], np.uint64)]))
         ok_live = bool(np.array_equal(live, expect))
@@ -218,10 +218,66 @@ def run_batched(G, N, keys, half, steps, warmup, nsrc=64, fused=8, check=True):
             one, r1 = G.gpma_bfs(g, int(starts_all[total - 1][i]))
             ok_bfs &= bool(np.array_equal(one, lv_all[i]) and r1 == reach[i])
         out["parity"] = {"live_set_closed_form": ok_live, "sources_equal_plain_bfs": ok_bfs}
+    g.close()
+
+    # The same steps as captured HIP graphs (gpma_step), on fresh handles:
+    # update only, then update + BFS.
+    max_records = max(int(o[-1]) for o in step_offs)
+
+    def graph_pass(with_bfs):
+        h = G.GPMA(N, tuning=knobs)
+        G.update_gpma(h, keys[:half], np.ones(half, np.uint8))
+        ns = nsrc if with_bfs else 0
+        lp = dlev.ptr if with_bfs else None
+        # capture every size class before the timed steps; the warm-up steps
+        # set the BFS depth the graphs hold, so capture again after them
+        G.gpma_step_prepare(h, max_records, ns, lp)
+        for i in range(warmup):
+            G.gpma_step(h, dk[i].ptr, dv[i].ptr, step_offs[i], starts_all[i] if with_bfs else (),
+                        levels_device_ptr=lp)
+        G.gpma_step_prepare(h, max_records, ns, lp)
+        h.sync()
+        cap0 = G.gpma_step_stats(h)["captures"]
+        ts = []
+        t0 = time.perf_counter()
+        for i in range(warmup, warmup + steps):
+            a = time.perf_counter()
+            G.gpma_step(h, dk[i].ptr, dv[i].ptr, step_offs[i], starts_all[i] if with_bfs else (),
+                        levels_device_ptr=dlev.ptr if with_bfs else None)
+            ts.append(time.perf_counter() - a)
+        t_run = time.perf_counter() - t0
+        timed_captures.append(G.gpma_step_stats(h)["captures"] - cap0)
+        for i in range(warmup + steps, total):
+            G.gpma_step(h, dk[i].ptr, dv[i].ptr, step_offs[i], starts_all[i] if with_bfs else (),
+                        levels_device_ptr=dlev.ptr if with_bfs else None)
+        h.sync()
+        return h, t_run, np.array(ts) * 1e3
+
+    timed_captures = []
+    hu, tu, _ = graph_pass(False)
+    su = G.gpma_step_stats(hu)
+    ok_u = bool(np.array_equal(hu.export_live(), live)) if check else None
+    hu.close()
+    hs, ts_all, per_ms = graph_pass(True)
+    ss = G.gpma_step_stats(hs)
+    gout = {"sample": "same steps and sources as the host path, each through gpma_step on a fresh handle",
+            "edge_updates_per_s": n_upd / tu, "update_ms_per_step": tu / steps * 1e3,
+            "step_ms": ts_all / steps * 1e3, "step_p50_ms": float(np.percentile(per_ms, 50)),
+            "step_p99_ms": float(np.percentile(per_ms, 99)),
+            "update_only_stats": su, "step_stats": ss, "captures_in_timed_steps": timed_captures,
+            "max_step_records": max_records}
+    gout["step_frac"] = (ub / steps + mb) / (ts_all / steps) / 1e9 / HBM_PEAK_GBS
+    gout["update_frac"] = ub / tu / 1e9 / HBM_PEAK_GBS
+    if check:
+        lv_g, reach_g = G.gpma_bfs_multi(hs, starts_all[total - 1])
+        gout["parity"] = {"live_set_equals_host_path": ok_u and bool(np.array_equal(hs.export_live(), live)),
+                          "levels_equal_host_path": bool(np.array_equal(lv_g, lv_all) and
+                                                         np.array_equal(reach_g, reach))}
+    hs.close()
+    out["graph"] = gout
     for b in dk + dv:
         b.free()
     dlev.free()
-    g.close()
     return out
 
 
@@ -702,10 +758,16 @@ def main():
         log("batched form: 8 fused uneven batches + 64-source BFS per step")
         batched = run_batched(G, N, keys, half, args.batched_steps, 2)
         batched["vs_plain_update_throughput"] = batched["edge_updates_per_s"] / ups
-        log(f"batched: {batched['edge_updates_per_s'] / 1e9:.3f} G edge-updates/s "
+        gr = batched["graph"]
+        gr["vs_plain_update_throughput"] = gr["edge_updates_per_s"] / ups
+        log(f"batched (host path): {batched['edge_updates_per_s'] / 1e9:.3f} G edge-updates/s "
             f"({batched['vs_plain_update_throughput']:.2f}x plain), 64-source BFS "
             f"{batched['msbfs_ms_per_step']:.2f} ms ({batched['msbfs_gteps']:.1f} GTEPS), "
             f"step p99 {batched['step_p99_ms']:.2f} ms, parity {batched.get('parity')}")
+        log(f"batched (graph): {gr['edge_updates_per_s'] / 1e9:.3f} G edge-updates/s "
+            f"({gr['vs_plain_update_throughput']:.2f}x plain), step {gr['step_ms']:.2f} ms "
+            f"(p99 {gr['step_p99_ms']:.2f}), replays {gr['step_stats']['replays']}, "
+            f"captures {gr['step_stats']['captures']}, parity {gr.get('parity')}")
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -29,6 +29,7 @@ from ._lib import (  # noqa: F401
     GpmaServerConfig,
     GpmaServerStats,
     GpmaStats,
+    GpmaStepStats,
     check,
     last_error,
     lib,
@@ -282,6 +283,52 @@ def gpma_bfs_multi(gpma, starts, levels_device_ptr=None, return_levels=True):
     else:
         check(lib.gpma_bfs_multi(gpma.handle, starts.ctypes.data, ns, None, 0, reach.ctypes.data), "gpma_bfs_multi")
     return out, reach
+
+
+def gpma_step(gpma, keys, values, offsets, starts=(), levels_device_ptr=None, return_levels=False, strict=True):
+    """One batched step through a captured HIP graph: the fused update
+    batches (records [offsets[b], offsets[b+1]) of keys/values, host numpy
+    arrays or device pointers) then a BFS from each of `starts` (<= 64).
+    Same results as update_gpma_batches followed by gpma_bfs_multi. Returns
+    (status, levels, reachable); levels is None unless return_levels (host
+    copy) is set; with levels_device_ptr the levels stay in that buffer."""
+    offsets = np.ascontiguousarray(offsets, np.uint64)
+    starts = np.ascontiguousarray(starts, np.uint32)
+    ns = len(starts)
+    if isinstance(keys, np.ndarray):
+        keys = np.ascontiguousarray(keys, np.uint64)
+        values = np.ascontiguousarray(values, np.uint8)
+        if len(keys) != len(values) or len(keys) < int(offsets[-1]):
+            raise GpmaError(GPMA_ERR_ARG, "keys and values must cover offsets[-1] records")
+        kp, vp, dev = keys.ctypes.data, values.ctypes.data, 0
+    else:
+        kp, vp, dev = int(keys), int(values), 1
+    reach = np.empty(max(ns, 1), np.uint32)
+    out = None
+    lp, ldev = None, 0
+    if levels_device_ptr is not None:
+        lp, ldev = levels_device_ptr, 1
+    elif return_levels and ns:
+        out = np.empty((ns, gpma.row_num), np.uint32)
+        lp = out.ctypes.data
+    rc = lib.gpma_step(gpma.handle, kp, vp, offsets.ctypes.data, len(offsets) - 1, dev,
+                       starts.ctypes.data if ns else None, ns, lp, ldev, reach.ctypes.data if ns else None)
+    if strict:
+        check(rc, "gpma_step")
+    return rc, out, reach[:ns]
+
+
+def gpma_step_prepare(gpma, max_records, nsrc=0, levels_device_ptr=None, bfs_levels=0):
+    """Capture gpma_step's graphs for every size class up to max_records
+    now (see gpma_step_prepare in include/gpma.h)."""
+    check(lib.gpma_step_prepare(gpma.handle, max_records, nsrc, levels_device_ptr, bfs_levels), "gpma_step_prepare")
+
+
+def gpma_step_stats(gpma):
+    """Counters of gpma_step: replays, captures, host-path steps and tails."""
+    st = GpmaStepStats()
+    check(lib.gpma_get_step_stats(gpma.handle, ctypes.byref(st)), "gpma_get_step_stats")
+    return {k: getattr(st, k) for k, _ in st._fields_}
 
 
 def gpma_bfs_count(gpma, start_node, levels_device_ptr=None):

@@ -86,6 +86,19 @@ class GpmaStats(ctypes.Structure):
     ]
 
 
+class GpmaStepStats(ctypes.Structure):
+    _fields_ = [
+        ("steps", ctypes.c_uint64),
+        ("replays", ctypes.c_uint64),
+        ("captures", ctypes.c_uint64),
+        ("host_steps", ctypes.c_uint64),
+        ("update_tails", ctypes.c_uint64),
+        ("bfs_tails", ctypes.c_uint64),
+        ("graphs", ctypes.c_uint32),
+        ("bfs_levels", ctypes.c_uint32),
+    ]
+
+
 class GpmaServerConfig(ctypes.Structure):
     _fields_ = [
         ("struct_size", ctypes.c_uint32),
@@ -138,6 +151,9 @@ SIGNATURES = {
     "gpma_update_batches": (_I, [_P, _P, _P, _P, _U32, _I]),
     "gpma_bfs": (_I, [_P, _U32, _P, _I, _P]),
     "gpma_bfs_multi": (_I, [_P, _P, _U32, _P, _I, _P]),
+    "gpma_step": (_I, [_P, _P, _P, _P, _U32, _I, _P, _U32, _P, _I, _P]),
+    "gpma_get_step_stats": (_I, [_P, _P]),
+    "gpma_step_prepare": (_I, [_P, _U64, _U32, _P, _U32]),
     "gpma_csr_bfs": (_I, [_P, _P, _P, _U32, _U32, _U32, _P]),
     "gpma_get_csr": (_I, [_P, _P]),
     "gpma_copy_layout": (_I, [_P, _P, _P, _P]),

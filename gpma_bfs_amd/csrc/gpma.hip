@@ -102,13 +102,15 @@ enum Ctr : uint32_t {
     C_LIVE = 4,
     C_NEXT = 5,
     C_NRUNS = 6,
-    C_MSD_TICKET = 14,  // scan-block tickets of the bucket sort
-    C_MSD_MAXB = 15,    // largest bucket above 512 items in the bucket sort
-    C_LVL = 8,  // C_LVL + L: pending inserts entering level L of the device-driven prefix
+    C_GATE = 7,         // a captured step's update left work for the host
+    C_MSD_TICKET = 8,   // scan-block tickets of the bucket sort
+    C_MSD_MAXB = 9,     // largest bucket above 1024 items in the bucket sort
     C_TICKET = 16,  // C_TICKET + k: tile tickets of the k-th compaction of one update
-    C_COUNT = 64
+    C_LVL = 64,     // C_LVL + L: pending inserts entering level L of the device-driven prefix
+    C_COUNT = 96
 };
-constexpr uint32_t N_TICKETS = C_COUNT - C_TICKET;
+constexpr uint32_t N_TICKETS = C_LVL - C_TICKET;
+constexpr uint32_t C_LVL_SLOTS = C_COUNT - C_LVL;
 // Bucket sort for batches up to this size (its count matrix is 8 bytes per
 // item); larger batches and row counts >= 2^28 use the LSD passes.
 constexpr uint64_t MSD_MAX_N = 4ull << 20;
@@ -173,6 +175,22 @@ struct gpma_handle {
     hipEvent_t ev_open[GPMA_PHASES] = {};
     double ph_ms[GPMA_PHASES] = {};
     uint64_t ph_calls[GPMA_PHASES] = {};
+    // captured steps (gpma_step): one executable graph per (size class,
+    // sources, output) and layout; sig lists every pointer and size the
+    // graph's launches bake in
+    struct StepGraph {
+        std::vector<uint64_t> sig;
+        hipGraphExec_t exec = nullptr;
+        uint64_t used = 0;
+        uint32_t spec = 0;  // device-driven update levels it holds
+    };
+    std::vector<StepGraph> graphs;
+    bool capturing = false;
+    uint32_t step_blevels = 8;  // BFS levels a new capture holds (grows to the depth seen)
+    uint64_t step_clock = 0;
+    Buf step_n;                    // device u32: records of the step being replayed
+    unsigned long long *step_h = nullptr;  // pinned: BFS counts and reachable counts read back
+    gpma_step_stats sst{};
 };
 
 namespace {
@@ -210,7 +228,7 @@ hipEvent_t pool_event(gpma_handle *g) {
     g->ev_pool.pop_back();
     return e;
 }
-bool phase_on(const gpma_handle *g, int p) { return g->prof && ((g->prof_mask >> p) & 1u); }
+bool phase_on(const gpma_handle *g, int p) { return g->prof && !g->capturing && ((g->prof_mask >> p) & 1u); }
 void phase_begin(gpma_handle *g, int p) {
     if (!phase_on(g, p)) return;
     if (g->ev_open[p]) g->ev_pool.push_back(g->ev_open[p]);
@@ -389,8 +407,11 @@ int ensure_update_buffers(gpma_handle *g, uint64_t n) {
 
 // Next look-back tag; on wrap-around every status word is cleared so no
 // stale tag can match.
+void drop_graphs(gpma_handle *g);
+
 uint32_t next_epoch(gpma_handle *g) {
     if (++g->cc_epoch == 0) {
+        drop_graphs(g);  // their tags could now repeat
         fill_u64(g->ccstat.as<uint64_t>(), 0, g->ccstat.bytes / 8, g->st);
         if (g->msd_status.p) fill_u64(g->msd_status.as<uint64_t>(), 0, g->msd_status.bytes / 8, g->st);
         g->cc_epoch = 1;
@@ -427,66 +448,54 @@ int ensure_msd(gpma_handle *g, uint64_t n, const MsdPlan &pl) {
     return GPMA_OK;
 }
 
-int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint64_t n, int on_device,
-              bool guards_ok = false) {
-    if (n == 0) return GPMA_OK;
-    if (!keys || !values) return fail(GPMA_ERR_ARG, "gpma_update: null keys or values");
-    if (n >= (1ull << 30)) return fail(GPMA_ERR_CAPACITY, "gpma_update: batch larger than 2^30 records");
-    HIP_TRY(hipSetDevice(g->device));
-    TRY(ensure_update_buffers(g, n));
+// Device part of one update: sort, locate, the level-0 compaction, the leaf
+// pass and levels 1..dev_levels, each reading its own count on the device,
+// so nothing here waits for the host. The record count is n_ub, or *d_n
+// (<= n_ub) when d_n is given: launch shapes then cover n_ub and the kernels
+// past the batch exit at once, which lets a captured graph serve batches of
+// any size up to n_ub. pl: bucket sort plan for n_ub (nullptr: LSD passes).
+// Returns the number of device-driven levels above the leaves.
+uint32_t enqueue_update(gpma_handle *g, const gkey *src_k, const gval *src_v, uint64_t n_ub, const uint32_t *d_n,
+                        bool guards_ok, const MsdPlan *pl, uint32_t dev_levels) {
     hipStream_t st = g->st;
     uint32_t *ctr = g->ctr.as<uint32_t>();
-    fill_u32(ctr, 0, C_COUNT, st);
-    g->cc_ticket = 0;
-    const gkey *src_k = keys;
-    const gval *src_v = values;
-    if (!on_device) {
-        HIP_TRY(hipMemcpyAsync(g->in_k.p, keys, n * 8, hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(g->in_v.p, values, n, hipMemcpyHostToDevice, st));
-        src_k = g->in_k.as<gkey>();
-        src_v = g->in_v.as<gval>();
-    }
-    // digits: bits of N-1 plus one (so KEY_NONE sorts after every valid key)
-    uint32_t nb = std::max<uint32_t>(1, fls32(g->N ? g->N - 1 : 0)) + 1;
-    uint32_t dpw = std::min<uint32_t>(4, (nb + 7) / 8);
-    SortShifts sh{};
-    uint32_t npass = 0;
-    for (uint32_t d = 0; d < dpw; d++) sh.s[npass++] = 8 * d;
-    for (uint32_t d = 0; d < dpw; d++) sh.s[npass++] = 32 + 8 * d;
     phase_begin(g, P_SORT);
     int cur = 0;
-    const bool use_msd = msd_sort_fits(g->N) && n <= MSD_MAX_N && g->msd_cool == 0;
-    if (g->msd_cool) g->msd_cool--;
-    if (use_msd) {
-        const MsdPlan pl = msd_plan(n, g->N, g->sort_bits);
-        TRY(ensure_msd(g, n, pl));
+    if (pl) {
         MsdWork w{g->msd_kp.as<uint64_t>(), g->msd_rk.as<uint16_t>(), g->sk1.as<uint64_t>(), g->msd_mat.as<uint32_t>(),
                   g->msd_bstart.as<uint32_t>(), g->msd_status.as<unsigned long long>()};
-        msd_sort(src_k, src_v, g->sk0.as<gkey>(), g->sv0.as<gval>(), pl, w, n, g->N, guards_ok, ctr + C_INVALID,
-                 next_epoch(g), ctr + C_MSD_TICKET, ctr + C_SORTERR, ctr + C_MSD_MAXB, st);
+        msd_sort(src_k, src_v, g->sk0.as<gkey>(), g->sv0.as<gval>(), *pl, w, n_ub, d_n, g->N, guards_ok,
+                 ctr + C_INVALID, next_epoch(g), ctr + C_MSD_TICKET, ctr + C_SORTERR, ctr + C_MSD_MAXB, st);
         phase_end(g, P_SORT, 4);
     } else {
-        cur = stage_and_sort(src_k, src_v, g->sk0.as<gkey>(), g->sv0.as<gval>(), g->sk1.as<gkey>(), g->sv1.as<gval>(), n,
-                             g->N, guards_ok, ctr + C_INVALID, sh, npass, g->sortwork.as<uint32_t>(), ctr + C_SORTERR,
-                             st);
+        // digits: bits of N-1 plus one (so KEY_NONE sorts after every valid key)
+        const uint32_t nb = std::max<uint32_t>(1, fls32(g->N ? g->N - 1 : 0)) + 1;
+        const uint32_t dpw = std::min<uint32_t>(4, (nb + 7) / 8);
+        SortShifts sh{};
+        uint32_t npass = 0;
+        for (uint32_t d = 0; d < dpw; d++) sh.s[npass++] = 8 * d;
+        for (uint32_t d = 0; d < dpw; d++) sh.s[npass++] = 32 + 8 * d;
+        cur = stage_and_sort(src_k, src_v, g->sk0.as<gkey>(), g->sv0.as<gval>(), g->sk1.as<gkey>(), g->sv1.as<gval>(),
+                             n_ub, g->N, guards_ok, ctr + C_INVALID, sh, npass, g->sortwork.as<uint32_t>(),
+                             ctr + C_SORTERR, st);
         phase_end(g, P_SORT, npass + 2);
     }
     const gkey *ks = cur ? g->sk1.as<gkey>() : g->sk0.as<gkey>();
     const gval *vs = cur ? g->sv1.as<gval>() : g->sv0.as<gval>();
     // locate + in-place delete / modify
     phase_begin(g, P_LOCATE);
-    launch_locate(ks, vs, n, g->keys.as<gkey>(), g->values.as<gval>(), g->seps.as<gkey>(), g->C / g->S, g->S,
+    launch_locate(ks, vs, n_ub, d_n, g->keys.as<gkey>(), g->values.as<gval>(), g->seps.as<gkey>(), g->C / g->S, g->S,
                   g->row_offset.as<uint32_t>(), !guards_ok, g->win.as<uint32_t>(), st);
     phase_end(g, P_LOCATE, 1);
     Buf *PK[2] = {&g->pk0, &g->pk1}, *PV[2] = {&g->pv0, &g->pv1}, *PW[2] = {&g->pw0, &g->pw1},
         *PD[2] = {&g->pd0, &g->pd1};
     phase_begin(g, P_COMPACT);
-    compact_step(g, ks, vs, g->win.as<uint32_t>(), nullptr, n, nullptr, g->pk0.as<gkey>(), g->pv0.as<gval>(),
+    compact_step(g, ks, vs, g->win.as<uint32_t>(), nullptr, n_ub, d_n, g->pk0.as<gkey>(), g->pv0.as<gval>(),
                  g->pw0.as<uint32_t>(), g->pd0.as<uint8_t>(), 0xFFFFFFFFu, ctr + C_LVL);
     phase_end(g, P_COMPACT, 1);
     // Drop placed inserts of list `pc` (count in *d_in), move the rest up to
     // level+1 into the other list, count in *d_out.
-    auto compact_pending = [&](int pc, uint64_t n_ub, uint32_t level, const uint32_t *d_in, uint32_t *d_out) {
+    auto compact_pending = [&](int pc, uint32_t level, const uint32_t *d_in, uint32_t *d_out) {
         const uint32_t mask = ~(uint32_t)(((uint64_t)g->S << (level + 1)) - 1);
         phase_begin(g, P_COMPACT);
         compact_step(g, PK[pc]->as<gkey>(), PV[pc]->as<gval>(), PW[pc]->as<uint32_t>(), PD[pc]->as<uint8_t>(), n_ub,
@@ -494,33 +503,41 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
                      PD[pc ^ 1]->as<uint8_t>(), mask, d_out);
         phase_end(g, P_COMPACT, 1);
     };
-    // Device-driven prefix: level 0 and levels 1..spec, launched before the
-    // host knows any pending count; each kernel reads its count on the device,
-    // and all of them stand down if the batch turns out to be a bulk load.
+    // level 0, then levels 1..spec; every kernel stands down if the batch
+    // turns out to be a bulk load
     phase_begin(g, P_LEAF);
-    launch_leaf(g->S, g->pk0.as<gkey>(), g->pv0.as<gval>(), g->pw0.as<uint32_t>(), g->pd0.as<uint8_t>(), n,
+    launch_leaf(g->S, g->pk0.as<gkey>(), g->pv0.as<gval>(), g->pw0.as<uint32_t>(), g->pd0.as<uint8_t>(), n_ub,
                 ctr + C_LVL, g->keys.as<gkey>(), g->values.as<gval>(), g->seps.as<gkey>(),
                 g->row_offset.as<uint32_t>(), g->lower[0], g->upper[0], g->bulk_threshold, st);
     phase_end(g, P_LEAF, 1);
-    compact_pending(0, n, 0, ctr + C_LVL, ctr + C_LVL + 1);
+    compact_pending(0, 0, ctr + C_LVL, ctr + C_LVL + 1);
     uint32_t spec = 0;
-    while (spec < g->spec_levels && spec + 1 <= g->h && ((uint64_t)g->S << (spec + 1)) <= WIN_MAX) {
+    while (spec < dev_levels && spec + 1 <= g->h && ((uint64_t)g->S << (spec + 1)) <= WIN_MAX &&
+           spec + 2 < C_LVL_SLOTS) {
         const uint32_t L = spec + 1;
         const int pc = L & 1;
         phase_begin(g, P_UPPER);
         launch_window(g->S, g->S << L, PK[pc]->as<gkey>(), PV[pc]->as<gval>(), PW[pc]->as<uint32_t>(),
-                      PD[pc]->as<uint8_t>(), n, ctr + C_LVL + L, g->keys.as<gkey>(), g->values.as<gval>(),
+                      PD[pc]->as<uint8_t>(), n_ub, ctr + C_LVL + L, g->keys.as<gkey>(), g->values.as<gval>(),
                       g->seps.as<gkey>(), g->row_offset.as<uint32_t>(), g->lower[L], g->upper[L], ctr + C_LVL,
                       g->bulk_threshold, st);
         phase_end(g, P_UPPER, 1);
-        compact_pending(pc, n, L, ctr + C_LVL + L, ctr + C_LVL + L + 1);
+        compact_pending(pc, L, ctr + C_LVL + L, ctr + C_LVL + L + 1);
         spec++;
     }
-    TRY(sync_read_ctr(g));
+    return spec;
+}
+
+// Host part of one update of n records, after the counters of its device
+// part (enqueue_update, `spec` device levels) are in hctr: errors, counts,
+// the bulk path, and any level above the device-driven ones.
+int finish_update(gpma_handle *g, uint64_t n, uint32_t spec, bool used_msd) {
+    hipStream_t st = g->st;
+    uint32_t *ctr = g->ctr.as<uint32_t>();
     if (g->hctr[C_SORTERR]) return fail(GPMA_ERR_HIP, "sort or compaction look-back exceeded its spin limit");
     // a batch with a bucket this large sorts faster with the LSD passes: use
     // them for the next batches of this handle
-    if (use_msd && g->hctr[C_MSD_MAXB] > MSD_BIG_BUCKET) g->msd_cool = MSD_COOL_BATCHES;
+    if (used_msd && g->hctr[C_MSD_MAXB] > MSD_BIG_BUCKET) g->msd_cool = MSD_COOL_BATCHES;
     const uint64_t inval = g->hctr[C_INVALID];
     g->invalid += inval;
     g->applied += n - inval;
@@ -536,6 +553,8 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
     g->last_levels = 0;
     for (uint32_t L = 0; L <= spec; L++)
         if (g->hctr[C_LVL + L]) g->last_levels = L + 1;
+    Buf *PK[2] = {&g->pk0, &g->pk1}, *PV[2] = {&g->pv0, &g->pv1}, *PW[2] = {&g->pw0, &g->pw1},
+        *PD[2] = {&g->pd0, &g->pd1};
     // host-driven levels above the prefix (rare: wide windows, growth)
     uint32_t level = spec + 1;
     int pc = level & 1;
@@ -587,7 +606,11 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
                 TRY(stream_sync(g));
             }
         }
-        compact_pending(pc, np, level, d_np, ctr + npslot[pc ^ 1]);
+        const uint32_t mask = ~(uint32_t)(((uint64_t)g->S << (level + 1)) - 1);
+        phase_begin(g, P_COMPACT);
+        compact_step(g, pk, pv, pw, pd, np, d_np, PK[pc ^ 1]->as<gkey>(), PV[pc ^ 1]->as<gval>(),
+                     PW[pc ^ 1]->as<uint32_t>(), PD[pc ^ 1]->as<uint8_t>(), mask, ctr + npslot[pc ^ 1]);
+        phase_end(g, P_COMPACT, 1);
         TRY(sync_read_ctr(g));
         if (g->hctr[C_SORTERR]) return fail(GPMA_ERR_HIP, "compaction look-back exceeded its spin limit");
         pc ^= 1;
@@ -602,6 +625,41 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
     HIP_TRY(hipGetLastError());
     if (inval) return fail(GPMA_ERR_KEY_RANGE, std::to_string(inval) + " keys with src/dst >= row_num skipped");
     return GPMA_OK;
+}
+
+bool msd_usable(const gpma_handle *g, uint64_t n) {
+    return msd_sort_fits(g->N) && n <= MSD_MAX_N && g->msd_cool == 0;
+}
+
+int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint64_t n, int on_device,
+              bool guards_ok = false) {
+    if (n == 0) return GPMA_OK;
+    if (!keys || !values) return fail(GPMA_ERR_ARG, "gpma_update: null keys or values");
+    if (n >= (1ull << 30)) return fail(GPMA_ERR_CAPACITY, "gpma_update: batch larger than 2^30 records");
+    HIP_TRY(hipSetDevice(g->device));
+    TRY(ensure_update_buffers(g, n));
+    hipStream_t st = g->st;
+    fill_u32(g->ctr.as<uint32_t>(), 0, C_COUNT, st);
+    g->cc_ticket = 0;
+    const gkey *src_k = keys;
+    const gval *src_v = values;
+    if (!on_device) {
+        HIP_TRY(hipMemcpyAsync(g->in_k.p, keys, n * 8, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(g->in_v.p, values, n, hipMemcpyHostToDevice, st));
+        src_k = g->in_k.as<gkey>();
+        src_v = g->in_v.as<gval>();
+    }
+    const bool use_msd = msd_usable(g, n);
+    if (g->msd_cool) g->msd_cool--;
+    MsdPlan pl{};
+    if (use_msd) {
+        pl = msd_plan(n, g->N, g->sort_bits);
+        TRY(ensure_msd(g, n, pl));
+    }
+    const uint32_t spec = enqueue_update(g, src_k, src_v, n, nullptr, guards_ok, use_msd ? &pl : nullptr,
+                                         g->spec_levels);
+    TRY(sync_read_ctr(g));
+    return finish_update(g, n, spec, use_msd);
 }
 
 int ensure_bfs(gpma_handle *g) {
@@ -712,16 +770,17 @@ MsState ms_state(gpma_handle *g) {
     return m;
 }
 
-// Multi-source level loop: chunks of levels without a host round trip, like
-// run_bfs; levels land in out[nsrc][N] (device), counts in reach[nsrc] (device).
-int run_msbfs(gpma_handle *g, const uint32_t *d_starts, uint32_t nsrc, uint32_t *out, uint32_t *reach) {
-    const MsState m = ms_state(g);
+// Multi-source levels from `level` on, in chunks without a host round trip
+// per level (like run_bfs), until a level adds no vertex. cnt[level] must
+// hold the level's frontier counts and the touched count of `level` must be
+// clear. *depth: the last level run.
+int msbfs_levels(gpma_handle *g, const MsState &m, uint32_t level, uint32_t *depth) {
     hipStream_t st = g->st;
     const uint32_t N = g->N;
     const unsigned long long thr = (unsigned long long)(g->pull_fraction * (double)g->C);
-    ms_init(m, N, d_starts, nsrc, g->row_offset.as<uint32_t>(), st);
-    uint32_t level = 1, chunk = 4;
+    uint32_t chunk = 4;
     bool done = false;
+    *depth = level;
     while (!done) {
         if ((uint64_t)level + chunk + 1 > (uint64_t)N + BFS_MAX_CHUNK) chunk = 1;
         // counts of the chunk's levels (touched counts are indexed by the
@@ -737,18 +796,262 @@ int run_msbfs(gpma_handle *g, const uint32_t *d_starts, uint32_t nsrc, uint32_t 
         HIP_TRY(hipMemcpyAsync(g->ms_hcnt + BFS_MAX_CHUNK + 1, m.cnt_big + level + 1, chunk * 8, hipMemcpyDeviceToHost,
                                st));
         TRY(stream_sync(g));
-        for (uint32_t j = 0; j < chunk && !done; j++)
+        for (uint32_t j = 0; j < chunk && !done; j++) {
+            *depth = level + j;
             if ((uint32_t)g->ms_hcnt[j] == 0 && (uint32_t)g->ms_hcnt[BFS_MAX_CHUNK + 1 + j] == 0) done = true;
+        }
         level += chunk;
         chunk = std::min<uint32_t>(chunk * 2, BFS_MAX_CHUNK);
         if (level >= N + 1) done = true;
     }
+    return GPMA_OK;
+}
+
+// Multi-source BFS: levels land in out[nsrc][N] (device), counts in
+// reach[nsrc] (device).
+int run_msbfs(gpma_handle *g, const uint32_t *d_starts, uint32_t nsrc, uint32_t *out, uint32_t *reach,
+              uint32_t *depth = nullptr) {
+    const MsState m = ms_state(g);
+    ms_init(m, g->N, d_starts, nsrc, g->row_offset.as<uint32_t>(), g->st);
+    uint32_t d = 0;
+    TRY(msbfs_levels(g, m, 1, &d));
+    if (depth) *depth = d;
     phase_begin(g, P_BFS);
-    ms_output(m, N, nsrc, out, reach, st);
+    ms_output(m, g->N, nsrc, out, reach, g->st);
     phase_end(g, P_BFS, 2);
     TRY(stream_sync(g));
     HIP_TRY(hipGetLastError());
     return GPMA_OK;
+}
+
+// ---- captured steps
+
+constexpr uint32_t STEP_MIN_CLASS = 16;   // smallest size class: 2^16 records
+constexpr uint32_t STEP_MAX_GRAPHS = 16;  // graphs kept per handle (least recently used go first)
+constexpr uint32_t STEP_MAX_BLEVELS = 64; // BFS levels a graph may hold
+
+void drop_graphs(gpma_handle *g) {
+    for (auto &sg : g->graphs)
+        if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
+    g->graphs.clear();
+}
+
+// Everything a captured step's launches bake in: buffers, layout, limits.
+std::vector<uint64_t> step_signature(const gpma_handle *g, uint32_t cls, uint32_t nsrc, const uint32_t *out,
+                                     uint32_t ulevels, uint32_t blevels) {
+    std::vector<uint64_t> v = {cls, nsrc, (uint64_t)(uintptr_t)out, ulevels, blevels, g->N, g->C, g->S, g->h,
+                               g->bulk_threshold, g->sort_bits, (uint64_t)(g->pull_fraction * 1e9)};
+    const Buf *bufs[] = {&g->keys, &g->values, &g->seps, &g->row_offset, &g->ctr, &g->step_n, &g->in_k, &g->in_v,
+                         &g->sk0, &g->sv0, &g->sk1, &g->win, &g->pk0, &g->pv0, &g->pw0, &g->pd0, &g->pk1, &g->pv1,
+                         &g->pw1, &g->pd1, &g->ccstat, &g->msd_kp, &g->msd_rk, &g->msd_mat, &g->msd_bstart,
+                         &g->msd_status, &g->ms_seen, &g->ms_fr0, &g->ms_fr1, &g->ms_nx, &g->ms_lv16, &g->ms_q0,
+                         &g->ms_q1, &g->ms_qb0, &g->ms_qb1, &g->ms_touched, &g->ms_cnt, &g->ms_cntb, &g->ms_tcnt,
+                         &g->ms_partial, &g->ms_starts, &g->ms_reach};
+    for (const Buf *b : bufs) {
+        v.push_back((uint64_t)(uintptr_t)b->p);
+        v.push_back(b->bytes);
+    }
+    return v;
+}
+
+// Capture one step for batches of up to 2^cls records: counters and look-back
+// status cleared, the update's device part with `ulevels` levels, the gate,
+// and (nsrc > 0) the multi-source BFS with `blevels` levels into out.
+int capture_step(gpma_handle *g, uint32_t cls, const MsdPlan &pl, uint32_t ulevels, uint32_t nsrc, uint32_t *out,
+                 uint32_t blevels, hipGraphExec_t *exec, uint32_t *spec_out) {
+    hipStream_t st = g->st;
+    uint32_t *ctr = g->ctr.as<uint32_t>();
+    // the graph's look-back tags must stay clear of a wrap-around while it lives
+    if (g->cc_epoch > 0xFFFF0000u) {
+        fill_u64(g->ccstat.as<uint64_t>(), 0, g->ccstat.bytes / 8, st);
+        if (g->msd_status.p) fill_u64(g->msd_status.as<uint64_t>(), 0, g->msd_status.bytes / 8, st);
+        g->cc_epoch = 0;
+        drop_graphs(g);
+    }
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    g->capturing = true;
+    fill_u32(ctr, 0, C_COUNT, st);
+    // each replay reuses the same tags, so the status words start clear
+    fill_u64(g->ccstat.as<uint64_t>(), 0, g->ccstat.bytes / 8, st);
+    fill_u64(g->msd_status.as<uint64_t>(), 0, g->msd_status.bytes / 8, st);
+    g->cc_ticket = 0;
+    const uint32_t spec = enqueue_update(g, g->in_k.as<gkey>(), g->in_v.as<gval>(), 1ull << cls,
+                                         g->step_n.as<uint32_t>(), false, &pl, ulevels);
+    *spec_out = spec;
+    launch_step_gate(ctr + C_LVL, ctr + C_LVL + spec + 1, ctr + C_SORTERR, g->bulk_threshold, ctr + C_GATE, st);
+    if (nsrc) {
+        const MsState m = ms_state(g);
+        const unsigned long long thr = (unsigned long long)(g->pull_fraction * (double)g->C);
+        ms_init(m, g->N, g->ms_starts.as<uint32_t>(), nsrc, g->row_offset.as<uint32_t>(), st, ctr + C_GATE);
+        ms_clear(m, 2, blevels + 2, st);
+        for (uint32_t L = 1; L <= blevels; L++)
+            ms_level(g->keys.as<gkey>(), g->values.as<gval>(), g->C, g->N, g->row_offset.as<uint32_t>(), m, L, thr,
+                     st);
+        ms_output(m, g->N, nsrc, out, g->ms_reach.as<uint32_t>(), st);
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(st, &graph);
+    g->capturing = false;
+    const hipError_t el = hipGetLastError();
+    if (ec != hipSuccess || el != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return fail(GPMA_ERR_HIP, std::string("step capture: ") + hipGetErrorString(ec != hipSuccess ? ec : el));
+    }
+    const hipError_t ei = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) return fail(GPMA_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    g->sst.captures++;
+    return GPMA_OK;
+}
+
+int step_host_path(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint64_t n, int on_device,
+                   const uint32_t *starts, uint32_t nsrc, uint32_t *levels, int levels_on_device,
+                   uint32_t *reachable);
+
+// The graph for size class cls (batches up to 2^cls records), nsrc sources
+// and levels destination dlev (device; nullptr: the handle's own buffer),
+// captured now if the cache has none for the current layout. *out: where
+// its BFS writes levels; *blevels: BFS levels it holds.
+int step_graph(gpma_handle *g, uint32_t cls, uint32_t nsrc, uint32_t *dlev, gpma_handle::StepGraph **res,
+               uint32_t **out, uint32_t *blevels) {
+    const uint64_t cap = 1ull << cls;
+    TRY(ensure_update_buffers(g, cap));
+    const MsdPlan pl = msd_plan(cap, g->N, g->sort_bits);
+    TRY(ensure_msd(g, cap, pl));
+    TRY(g->step_n.ensure(256, &g->dev_bytes));
+    if (!g->step_h)
+        HIP_TRY(hipHostMalloc((void **)&g->step_h, (2 * STEP_MAX_BLEVELS + 4 + 64) * 8, hipHostMallocDefault));
+    *out = nullptr;
+    if (nsrc) {
+        TRY(ensure_msbfs(g));
+        if (dlev) {
+            *out = dlev;
+        } else {
+            TRY(g->ms_out.ensure((uint64_t)nsrc * g->N * 4, &g->dev_bytes));
+            *out = g->ms_out.as<uint32_t>();
+        }
+    }
+    uint32_t ulevels = 0;
+    while (ulevels + 1 <= g->h && ((uint64_t)g->S << (ulevels + 1)) <= WIN_MAX && ulevels + 2 < C_LVL_SLOTS)
+        ulevels++;
+    *blevels = nsrc ? std::min<uint32_t>(g->step_blevels, g->N) : 0;
+    const std::vector<uint64_t> sig = step_signature(g, cls, nsrc, *out, ulevels, *blevels);
+    for (auto &x : g->graphs)
+        if (x.sig == sig) {
+            *res = &x;
+            return GPMA_OK;
+        }
+    if (g->graphs.size() >= STEP_MAX_GRAPHS) {
+        auto lru = std::min_element(g->graphs.begin(), g->graphs.end(),
+                                    [](const auto &a, const auto &b) { return a.used < b.used; });
+        (void)hipGraphExecDestroy(lru->exec);
+        g->graphs.erase(lru);
+    }
+    hipGraphExec_t exec = nullptr;
+    uint32_t spec = 0;
+    TRY(capture_step(g, cls, pl, ulevels, nsrc, *out, *blevels, &exec, &spec));
+    g->graphs.push_back({sig, exec, ++g->step_clock, spec});
+    *res = &g->graphs.back();
+    return GPMA_OK;
+}
+
+int run_step(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint64_t n, int on_device,
+             const uint32_t *starts, uint32_t nsrc, uint32_t *levels, int levels_on_device, uint32_t *reachable) {
+    g->sst.steps++;
+    if (n == 0 || !msd_usable(g, n) || g->prof)
+        return step_host_path(g, keys, values, n, on_device, starts, nsrc, levels, levels_on_device, reachable);
+    hipStream_t st = g->st;
+    uint32_t cls = STEP_MIN_CLASS;
+    while ((1ull << cls) < n) cls++;
+    uint32_t *out = nullptr;
+    uint32_t blevels = 0;
+    gpma_handle::StepGraph *sg = nullptr;
+    TRY(step_graph(g, cls, nsrc, levels && levels_on_device ? levels : nullptr, &sg, &out, &blevels));
+    sg->used = ++g->step_clock;
+    // stage the step's inputs where the graph reads them
+    const hipMemcpyKind kin = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    HIP_TRY(hipMemcpyAsync(g->in_k.p, keys, n * 8, kin, st));
+    HIP_TRY(hipMemcpyAsync(g->in_v.p, values, n, kin, st));
+    fill_u32(g->step_n.as<uint32_t>(), (uint32_t)n, 1, st);
+    if (nsrc) HIP_TRY(hipMemcpyAsync(g->ms_starts.p, starts, nsrc * 4, hipMemcpyHostToDevice, st));
+    const uint32_t spec = sg->spec;
+    HIP_TRY(hipGraphLaunch(sg->exec, st));
+    g->sst.replays++;
+    // one read-back: counters, the frontier counts after the last captured
+    // level, reachable counts
+    unsigned long long *hb = g->step_h;
+    HIP_TRY(hipMemcpyAsync(g->hctr, g->ctr.p, C_COUNT * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    if (nsrc) {
+        HIP_TRY(hipMemcpyAsync(hb, g->ms_cnt.as<unsigned long long>() + blevels + 1, 8, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(hb + 1, g->ms_cntb.as<unsigned long long>() + blevels + 1, 8, hipMemcpyDeviceToHost,
+                               st));
+        HIP_TRY(hipMemcpyAsync(hb + 2, g->ms_reach.p, nsrc * 4, hipMemcpyDeviceToHost, st));
+    }
+    TRY(stream_sync(g));
+    HIP_TRY(hipGetLastError());
+    const bool gated = g->hctr[C_GATE] != 0;
+    const int urc = finish_update(g, n, spec, true);
+    if (urc != GPMA_OK && urc != GPMA_ERR_KEY_RANGE) return urc;
+    if (gated) g->sst.update_tails++;
+    if (!nsrc) return urc;
+    bool reach_ready = true;
+    if (gated) {
+        // the captured BFS stood down: traverse the finished graph now
+        uint32_t depth = 0;
+        TRY(run_msbfs(g, g->ms_starts.as<uint32_t>(), nsrc, out, g->ms_reach.as<uint32_t>(), &depth));
+        if (depth + 2 > g->step_blevels) g->step_blevels = std::min(depth + 2, STEP_MAX_BLEVELS);
+        reach_ready = false;
+    } else if ((uint32_t)hb[0] != 0 || (uint32_t)hb[1] != 0) {
+        // deeper than the captured levels: continue from the next one
+        g->sst.bfs_tails++;
+        const MsState m = ms_state(g);
+        uint32_t depth = 0;
+        TRY(msbfs_levels(g, m, blevels + 1, &depth));
+        ms_output(m, g->N, nsrc, out, g->ms_reach.as<uint32_t>(), st);
+        if (depth + 2 > g->step_blevels) g->step_blevels = std::min(depth + 2, STEP_MAX_BLEVELS);
+        reach_ready = false;
+    }
+    if (reachable) {
+        if (reach_ready) {
+            std::memcpy(reachable, hb + 2, nsrc * 4);
+        } else {
+            HIP_TRY(hipMemcpyAsync(reachable, g->ms_reach.p, nsrc * 4, hipMemcpyDeviceToHost, st));
+        }
+    }
+    if (levels && !levels_on_device)
+        HIP_TRY(hipMemcpyAsync(levels, out, (uint64_t)nsrc * g->N * 4, hipMemcpyDeviceToHost, st));
+    TRY(stream_sync(g));
+    HIP_TRY(hipGetLastError());
+    return urc;
+}
+
+// gpma_bfs_multi after argument checks.
+int bfs_multi_impl(gpma_handle *g, const uint32_t *starts, uint32_t nsrc, uint32_t *levels, int on_device,
+                   uint32_t *reachable) {
+    HIP_TRY(hipSetDevice(g->device));
+    TRY(ensure_msbfs(g));
+    HIP_TRY(hipMemcpyAsync(g->ms_starts.p, starts, nsrc * 4, hipMemcpyHostToDevice, g->st));
+    uint32_t *out = levels;
+    if (!on_device || !levels) {
+        TRY(g->ms_out.ensure((uint64_t)nsrc * g->N * 4, &g->dev_bytes));
+        out = g->ms_out.as<uint32_t>();
+    }
+    TRY(run_msbfs(g, g->ms_starts.as<uint32_t>(), nsrc, out, g->ms_reach.as<uint32_t>()));
+    if (levels && !on_device)
+        HIP_TRY(hipMemcpyAsync(levels, out, (uint64_t)nsrc * g->N * 4, hipMemcpyDeviceToHost, g->st));
+    if (reachable) HIP_TRY(hipMemcpyAsync(reachable, g->ms_reach.p, nsrc * 4, hipMemcpyDeviceToHost, g->st));
+    HIP_TRY(hipStreamSynchronize(g->st));
+    return GPMA_OK;
+}
+
+int step_host_path(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint64_t n, int on_device,
+                   const uint32_t *starts, uint32_t nsrc, uint32_t *levels, int levels_on_device,
+                   uint32_t *reachable) {
+    g->sst.host_steps++;
+    const int urc = do_update(g, keys, values, n, on_device);
+    if (urc != GPMA_OK && urc != GPMA_ERR_KEY_RANGE) return urc;
+    if (nsrc) TRY(bfs_multi_impl(g, starts, nsrc, levels, levels_on_device, reachable));
+    return urc;
 }
 
 }  // namespace
@@ -898,8 +1201,10 @@ int gpma_destroy(gpma_t g) {
                   &g->chunkoff, &g->lev, &g->fr0, &g->fr1, &g->fb0, &g->fb1, &g->vis, &g->bcnt, &g->bcntb,
                   &g->ms_seen, &g->ms_fr0, &g->ms_fr1, &g->ms_nx, &g->ms_lv16, &g->ms_q0, &g->ms_q1, &g->ms_qb0,
                   &g->ms_qb1, &g->ms_touched, &g->ms_cnt, &g->ms_cntb, &g->ms_tcnt, &g->ms_partial, &g->ms_starts,
-                  &g->ms_out, &g->ms_reach, &g->ctr};
+                  &g->ms_out, &g->ms_reach, &g->ctr, &g->step_n};
+    drop_graphs(g);
     for (Buf *b : all) b->release(&g->dev_bytes);
+    if (g->step_h) (void)hipHostFree(g->step_h);
     if (g->hctr) (void)hipHostFree(g->hctr);
     if (g->hcnt) (void)hipHostFree(g->hcnt);
     if (g->ms_hcnt) (void)hipHostFree(g->ms_hcnt);
@@ -950,19 +1255,60 @@ int gpma_bfs_multi(gpma_t g, const uint32_t *starts, uint32_t nsrc, uint32_t *le
     if (nsrc == 0 || nsrc > 64) return fail(GPMA_ERR_ARG, "gpma_bfs_multi: nsrc must be in [1, 64]");
     for (uint32_t i = 0; i < nsrc; i++)
         if (starts[i] >= g->N) return fail(GPMA_ERR_ARG, "gpma_bfs_multi: start node >= row_num");
+    return bfs_multi_impl(g, starts, nsrc, levels, on_device, reachable);
+}
+
+int gpma_step(gpma_t g, const uint64_t *keys, const uint8_t *values, const uint64_t *offsets, uint32_t nbatches,
+              int on_device, const uint32_t *starts, uint32_t nsrc, uint32_t *levels, int levels_on_device,
+              uint32_t *reachable) {
+    if (!g || !offsets) return fail(GPMA_ERR_ARG, "gpma_step: null argument");
+    if (offsets[0] != 0) return fail(GPMA_ERR_ARG, "gpma_step: offsets[0] must be 0");
+    for (uint32_t b = 0; b < nbatches; b++)
+        if (offsets[b + 1] < offsets[b]) return fail(GPMA_ERR_ARG, "gpma_step: offsets must not decrease");
+    const uint64_t n = nbatches ? offsets[nbatches] : 0;
+    if (n && (!keys || !values)) return fail(GPMA_ERR_ARG, "gpma_step: null keys or values");
+    if (n >= (1ull << 30)) return fail(GPMA_ERR_CAPACITY, "gpma_step: step larger than 2^30 records");
+    if (nsrc > 64) return fail(GPMA_ERR_ARG, "gpma_step: nsrc must be in [0, 64]");
+    if (nsrc && !starts) return fail(GPMA_ERR_ARG, "gpma_step: null starts");
+    for (uint32_t i = 0; i < nsrc; i++)
+        if (starts[i] >= g->N) return fail(GPMA_ERR_ARG, "gpma_step: start node >= row_num");
     HIP_TRY(hipSetDevice(g->device));
-    TRY(ensure_msbfs(g));
-    HIP_TRY(hipMemcpyAsync(g->ms_starts.p, starts, nsrc * 4, hipMemcpyHostToDevice, g->st));
-    uint32_t *out = levels;
-    if (!on_device || !levels) {
-        TRY(g->ms_out.ensure((uint64_t)nsrc * g->N * 4, &g->dev_bytes));
-        out = g->ms_out.as<uint32_t>();
+    return run_step(g, keys, values, n, on_device, starts, nsrc, levels, levels_on_device, reachable);
+}
+
+int gpma_step_prepare(gpma_t g, uint64_t max_records, uint32_t nsrc, uint32_t *levels, uint32_t bfs_levels) {
+    if (!g) return fail(GPMA_ERR_ARG, "gpma_step_prepare: null handle");
+    if (nsrc > 64) return fail(GPMA_ERR_ARG, "gpma_step_prepare: nsrc must be in [0, 64]");
+    if (bfs_levels > STEP_MAX_BLEVELS) return fail(GPMA_ERR_ARG, "gpma_step_prepare: bfs_levels above 64");
+    HIP_TRY(hipSetDevice(g->device));
+    if (bfs_levels) g->step_blevels = bfs_levels;
+    if (!msd_sort_fits(g->N)) return GPMA_OK;  // such steps take the host path
+    const uint64_t top = std::min<uint64_t>(std::max<uint64_t>(max_records, 1), MSD_MAX_N);
+    // size every buffer for the largest class first: growing one later would
+    // move it and leave the graphs captured before unusable
+    uint32_t top_cls = STEP_MIN_CLASS;
+    while ((1ull << top_cls) < top) top_cls++;
+    TRY(ensure_update_buffers(g, 1ull << top_cls));
+    TRY(ensure_msd(g, 1ull << top_cls, msd_plan(1ull << top_cls, g->N, g->sort_bits)));
+    if (nsrc) {
+        TRY(ensure_msbfs(g));
+        if (!levels) TRY(g->ms_out.ensure((uint64_t)nsrc * g->N * 4, &g->dev_bytes));
     }
-    TRY(run_msbfs(g, g->ms_starts.as<uint32_t>(), nsrc, out, g->ms_reach.as<uint32_t>()));
-    if (levels && !on_device)
-        HIP_TRY(hipMemcpyAsync(levels, out, (uint64_t)nsrc * g->N * 4, hipMemcpyDeviceToHost, g->st));
-    if (reachable) HIP_TRY(hipMemcpyAsync(reachable, g->ms_reach.p, nsrc * 4, hipMemcpyDeviceToHost, g->st));
-    HIP_TRY(hipStreamSynchronize(g->st));
+    for (uint32_t cls = STEP_MIN_CLASS;; cls++) {
+        uint32_t *out = nullptr;
+        uint32_t bl = 0;
+        gpma_handle::StepGraph *sg = nullptr;
+        TRY(step_graph(g, cls, nsrc, levels, &sg, &out, &bl));
+        if ((1ull << cls) >= top) break;
+    }
+    return GPMA_OK;
+}
+
+int gpma_get_step_stats(gpma_t g, gpma_step_stats *out) {
+    if (!g || !out) return fail(GPMA_ERR_ARG, "gpma_get_step_stats: null argument");
+    *out = g->sst;
+    out->graphs = (uint32_t)g->graphs.size();
+    out->bfs_levels = g->step_blevels;
     return GPMA_OK;
 }
 

@@ -52,11 +52,12 @@ struct MsdWork {
 };
 // Sorted (ok, ov) from (ik, iv); same contract as stage_and_sort. epoch:
 // non-zero tag distinct from earlier calls on the same status words;
-// *ticket zero before the call; *maxb raised to the largest bucket above 512
-// items.
+// *ticket zero before the call; *maxb raised to the largest bucket above 1024
+// items. dn (optional): the batch size is read from *dn on the device, and
+// n and the plan are an upper bound of it (launch shapes fixed in advance).
 void msd_sort(const gkey *ik, const gval *iv, gkey *ok, gval *ov, const MsdPlan &pl, const MsdWork &w, uint64_t n,
-              uint32_t N, bool guards_ok, uint32_t *invalid, uint32_t epoch, uint32_t *ticket, uint32_t *err,
-              uint32_t *maxb, hipStream_t st);
+              const uint32_t *dn, uint32_t N, bool guards_ok, uint32_t *invalid, uint32_t epoch, uint32_t *ticket,
+              uint32_t *err, uint32_t *maxb, hipStream_t st);
 
 // ---- update.hip
 constexpr uint32_t CMP_BLOCK = 256;  // items per compaction block
@@ -65,9 +66,15 @@ constexpr uint32_t CHUNK = 4096;     // slots per workgroup in device-wide windo
 
 // Locate + in-place delete/modify. win[i] = leaf base or SIZE_NONE. With
 // row_hint the search starts from row_offset (valid guards in the array).
-void launch_locate(const gkey *ks, const gval *vs, uint64_t n, gkey *keys, gval *values, const gkey *seps,
-                   uint64_t nseg, uint32_t S, const uint32_t *row_offset, bool row_hint, uint32_t *win,
-                   hipStream_t st);
+// dn (optional): the record count is *dn (<= n) on the device.
+void launch_locate(const gkey *ks, const gval *vs, uint64_t n, const uint32_t *dn, gkey *keys, gval *values,
+                   const gkey *seps, uint64_t nseg, uint32_t S, const uint32_t *row_offset, bool row_hint,
+                   uint32_t *win, hipStream_t st);
+// *gate = 1 when the update before it left work for the host: a bulk batch
+// (*np0 >= bulk_threshold), inserts pending above the device-driven levels
+// (*np_top != 0) or a look-back error (*err != 0); else 0.
+void launch_step_gate(const uint32_t *np0, const uint32_t *np_top, const uint32_t *err, uint32_t bulk_threshold,
+                      uint32_t *gate, hipStream_t st);
 // Exclusive scan of per-chunk counts (one workgroup); *total gets the sum.
 // The chunk count is nb, or ceil(*d_items / CMP_BLOCK) when d_items is given.
 void launch_scan_blocks(const uint32_t *cnt, uint64_t nb, uint32_t *off, uint32_t *total, hipStream_t st,
@@ -151,7 +158,10 @@ struct MsState {
     uint32_t *tcnt;            // per level touched count
     uint32_t *partial;         // [ms_output_blocks(N)][64] reachable partials
 };
-void ms_init(const MsState &m, uint32_t N, const uint32_t *starts, uint32_t nsrc, const uint32_t *ro, hipStream_t st);
+// gate (optional): when *gate != 0 no source is queued, so every level finds
+// an empty frontier.
+void ms_init(const MsState &m, uint32_t N, const uint32_t *starts, uint32_t nsrc, const uint32_t *ro, hipStream_t st,
+             const uint32_t *gate = nullptr);
 void ms_clear(const MsState &m, uint32_t a, uint32_t b, hipStream_t st);
 void ms_level(const gkey *keys, const gval *values, uint64_t C, uint32_t N, const uint32_t *ro, const MsState &m,
               uint32_t level, unsigned long long pull_threshold, hipStream_t st);

@@ -62,8 +62,9 @@ __global__ void k_ms_sources(const uint32_t *__restrict__ starts, uint32_t nsrc,
                              uint64_t *__restrict__ fr, uint16_t *__restrict__ lv16, uint32_t *__restrict__ q,
                              uint32_t *__restrict__ qbig, unsigned long long *__restrict__ cnt,
                              unsigned long long *__restrict__ cnt_big, uint32_t *__restrict__ tcnt,
-                             const uint32_t *__restrict__ ro) {
+                             const uint32_t *__restrict__ ro, const uint32_t *__restrict__ gate) {
     __shared__ uint32_t ss[WAVE];
+    if (gate && *gate) nsrc = 0;
     const uint32_t lane = threadIdx.x;  // one wave
     const bool valid = lane < nsrc;
     const uint32_t s = valid ? starts[lane] : 0;
@@ -452,11 +453,11 @@ __global__ void k_ms_clear(unsigned long long *__restrict__ cnt, unsigned long l
 
 }  // namespace
 
-void ms_init(const MsState &m, uint32_t N, const uint32_t *starts, uint32_t nsrc, const uint32_t *ro,
-             hipStream_t st) {
+void ms_init(const MsState &m, uint32_t N, const uint32_t *starts, uint32_t nsrc, const uint32_t *ro, hipStream_t st,
+             const uint32_t *gate) {
     hipLaunchKernelGGL(k_ms_init, dim3(2048), dim3(256), 0, st, m.seen, m.fr[0], m.fr[1], m.nx, N);
     hipLaunchKernelGGL(k_ms_sources, dim3(1), dim3(64), 0, st, starts, nsrc, m.seen, m.fr[1], m.lv16, m.q[1],
-                       m.qbig[1], m.cnt, m.cnt_big, m.tcnt, ro);
+                       m.qbig[1], m.cnt, m.cnt_big, m.tcnt, ro, gate);
 }
 
 void ms_clear(const MsState &m, uint32_t a, uint32_t b, hipStream_t st) {

@@ -17,7 +17,7 @@
 //             order, chained across blocks of 64 buckets;
 //   scatter   (k_msd_scatter) every item to its bucket's range, stable;
 //   local     (k_msd_local) one workgroup per bucket sorts the bucket by the
-//             remaining bits: comparison ranking in LDS up to 512 items,
+//             remaining bits: runs sorted in registers and merged in LDS up to 1024 items,
 //             else LSD passes with 8-bit digits over the bucket's range, and
 //             writes the unpacked keys and values.
 //
@@ -37,7 +37,9 @@ constexpr uint32_t MWI = WAVE * MIPT;        // items per wave
 constexpr uint32_t MTILE = MT * MIPT;        // items per tile
 constexpr uint32_t MAX_D = 12;               // bucket bits
 constexpr uint32_t MAX_NB = 1u << MAX_D;
-constexpr uint32_t SMALL = 512;              // buckets ranked by comparison in LDS
+constexpr uint32_t TINY = 64;                // buckets ranked by plain comparison in LDS
+constexpr uint32_t SMALL = 1024;             // buckets sorted in LDS (runs of 64, then a merge rank)
+constexpr uint32_t RUN = WAVE;               // items per register-sorted run
 constexpr uint32_t SCAN_B = WAVE;            // buckets per scan block
 
 struct PackInfo {
@@ -62,10 +64,12 @@ __device__ __forceinline__ void unpack_key(uint64_t p, const PackInfo &pi, gkey 
 }
 
 __global__ __launch_bounds__(MT) void k_msd_rank(const gkey *__restrict__ ik, const gval *__restrict__ iv, uint64_t n,
-                                                 PackInfo pi, uint32_t bshift, uint32_t D,
-                                                 uint64_t *__restrict__ kp, uint16_t *__restrict__ rk,
+                                                 const uint32_t *__restrict__ dn, PackInfo pi, uint32_t bshift,
+                                                 uint32_t D, uint64_t *__restrict__ kp, uint16_t *__restrict__ rk,
                                                  uint32_t *__restrict__ counts, uint32_t *__restrict__ invalid) {
     __shared__ uint16_t cnt[MW][MAX_NB];
+    if (dn) n = *dn;
+    if ((uint64_t)blockIdx.x * MTILE >= n) return;  // tiles past the batch (grid sized for a capacity)
     const uint32_t NB = 1u << D;
     for (uint32_t i = threadIdx.x; i < MW * MAX_NB / 2; i += MT) reinterpret_cast<uint32_t *>(&cnt[0][0])[i] = 0;
     const uint32_t wid = threadIdx.x / WAVE, lane = lane_id();
@@ -122,15 +126,20 @@ __global__ __launch_bounds__(MT) void k_msd_rank(const gkey *__restrict__ ik, co
 
 // offs[t][b] = items in buckets < b + items of bucket b in tiles < t;
 // bstart[b] = offs[0][b], bstart[NB] = n. A block covers SCAN_B buckets with
-// four tile groups; blocks chain their bucket totals by look-back.
+// four tile groups; blocks chain their bucket totals by look-back. With dn,
+// n = *dn and only its ceil(n / MTILE) tiles take part.
 __global__ __launch_bounds__(MT) void k_msd_scan(const uint32_t *__restrict__ counts, uint32_t ntiles, uint32_t NB,
-                                                 uint64_t n, uint32_t *__restrict__ offs,
-                                                 uint32_t *__restrict__ bstart, unsigned long long *status,
-                                                 uint32_t epoch, uint32_t *__restrict__ ticket,
-                                                 uint32_t *__restrict__ err) {
+                                                 uint64_t n, const uint32_t *__restrict__ dn,
+                                                 uint32_t *__restrict__ offs, uint32_t *__restrict__ bstart,
+                                                 unsigned long long *status, uint32_t epoch,
+                                                 uint32_t *__restrict__ ticket, uint32_t *__restrict__ err) {
     __shared__ uint32_t gs[MW][SCAN_B];
     __shared__ uint32_t base_s[SCAN_B];
     __shared__ uint32_t s_blk;
+    if (dn) {
+        n = *dn;
+        ntiles = min(ntiles, (uint32_t)((n + MTILE - 1) / MTILE));
+    }
     if (threadIdx.x == 0) s_blk = atomicAdd(ticket, 1u);
     __syncthreads();
     const uint32_t blk = s_blk;
@@ -178,9 +187,12 @@ __global__ __launch_bounds__(MT) void k_msd_scan(const uint32_t *__restrict__ co
 }
 
 __global__ __launch_bounds__(MT) void k_msd_scatter(const uint64_t *__restrict__ kp, const uint16_t *__restrict__ rk,
-                                                    uint64_t n, uint32_t bshift, uint32_t NB,
-                                                    const uint32_t *__restrict__ offs, uint64_t *__restrict__ kb) {
+                                                    uint64_t n, const uint32_t *__restrict__ dn, uint32_t bshift,
+                                                    uint32_t NB, const uint32_t *__restrict__ offs,
+                                                    uint64_t *__restrict__ kb) {
+    if (dn) n = *dn;
     const uint64_t base = (uint64_t)blockIdx.x * MTILE;
+    if (base >= n) return;
     const uint32_t *o = offs + (uint64_t)blockIdx.x * NB;
     uint64_t p[MIPT];
     uint32_t r[MIPT];
@@ -197,15 +209,50 @@ __global__ __launch_bounds__(MT) void k_msd_scatter(const uint64_t *__restrict__
     }
 }
 
+// (key, position) order of two bucket items: the packed key without its
+// value byte, then the position in the bucket (which makes the sort stable).
+__device__ __forceinline__ bool item_less(uint64_t pa, uint32_t ia, uint64_t pb, uint32_t ib) {
+    const uint64_t a = pa >> 8, b = pb >> 8;
+    return a < b || (a == b && ia < ib);
+}
+
+// One wave sorts its 64 (item, position) pairs across lanes, ascending:
+// bitonic network over lane exchanges.
+__device__ __forceinline__ void wave_bitonic(uint64_t &p, uint32_t &id) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (uint32_t k2 = 2; k2 <= RUN; k2 <<= 1) {
+#pragma unroll
+        for (uint32_t j = k2 >> 1; j > 0; j >>= 1) {
+            const uint64_t q = __shfl_xor(p, (int)j, WAVE);
+            const uint32_t qi = __shfl_xor(id, (int)j, WAVE);
+            const bool want_min = ((lane & k2) == 0) == ((lane & j) == 0);
+            const bool q_less = item_less(q, qi, p, id);
+            if (want_min ? q_less : item_less(p, id, q, qi)) {
+                p = q;
+                id = qi;
+            }
+        }
+    }
+}
+
 // One workgroup per bucket: stable sort of kb[bstart[b], bstart[b+1]) by the
 // packed key's bits [8, 8 + R) (the bucket bits above them are equal), then
 // unpack into (ok, ov) at the same positions. tmp is scratch of the same
 // range. Buckets above SMALL items raise *maxb to their size.
+//
+// Up to TINY items: each item's rank is the count of smaller (key, position)
+// pairs, by comparison with all of them. Up to SMALL: each wave sorts runs
+// of 64 in registers (bitonic), and an item's rank is its place in its run
+// plus, for every other run, the count of smaller pairs there (binary search
+// in LDS). Larger buckets: LSD passes with 8-bit digits over the bucket's
+// range in global memory.
 __global__ __launch_bounds__(MT) void k_msd_local(uint64_t *__restrict__ kb, uint64_t *__restrict__ tmp,
                                                   const uint32_t *__restrict__ bstart, uint32_t R, PackInfo pi,
                                                   gkey *__restrict__ ok, gval *__restrict__ ov,
                                                   uint32_t *__restrict__ maxb) {
     __shared__ uint64_t lk[SMALL];
+    __shared__ uint16_t lid[SMALL];
     __shared__ uint32_t cnt[MW][256];
     __shared__ uint32_t run[256];
     __shared__ uint32_t tot[256];
@@ -214,7 +261,42 @@ __global__ __launch_bounds__(MT) void k_msd_local(uint64_t *__restrict__ kb, uin
     const uint32_t start = bstart[blockIdx.x], end = bstart[blockIdx.x + 1];
     const uint32_t c = end - start;
     if (c == 0) return;
-    if (c <= SMALL) {
+    if (c > TINY && c <= SMALL) {
+        const uint32_t nrun = (c + RUN - 1) / RUN;
+        const uint32_t lane = lane_id();
+        for (uint32_t r = tid / WAVE; r < nrun; r += MW) {
+            const uint32_t i = r * RUN + lane;
+            uint64_t p = i < c ? kb[(uint64_t)start + i] : ~0ull;  // padding sorts last
+            uint32_t id = i < c ? i : 0xFFFFu;
+            wave_bitonic(p, id);
+            lk[i] = p;
+            lid[i] = (uint16_t)id;
+        }
+        __syncthreads();
+        for (uint32_t x = tid; x < nrun * RUN; x += MT) {
+            const uint32_t id = lid[x];
+            if (id == 0xFFFFu) continue;  // padding
+            const uint64_t p = lk[x];
+            const uint32_t r = x / RUN;
+            uint32_t rank = x % RUN;
+            for (uint32_t q = 0; q < nrun; q++) {
+                if (q == r) continue;
+                const uint32_t b = q * RUN;
+                uint32_t lo = 0;
+#pragma unroll
+                for (uint32_t step = RUN; step > 0; step >>= 1)
+                    if (lo + step <= RUN && item_less(lk[b + lo + step - 1], lid[b + lo + step - 1], p, id)) lo += step;
+                rank += lo;
+            }
+            gkey k;
+            gval v;
+            unpack_key(p, pi, k, v);
+            ok[(uint64_t)start + rank] = k;
+            ov[(uint64_t)start + rank] = v;
+        }
+        return;
+    }
+    if (c <= TINY) {
         for (uint32_t i = tid; i < c; i += MT) lk[i] = kb[(uint64_t)start + i];
         __syncthreads();
         for (uint32_t i = tid; i < c; i += MT) {
@@ -334,17 +416,18 @@ MsdPlan msd_plan(uint64_t n, uint32_t N, uint32_t bucket_bits) {
 }
 
 void msd_sort(const gkey *ik, const gval *iv, gkey *ok, gval *ov, const MsdPlan &pl, const MsdWork &w, uint64_t n,
-              uint32_t N, bool guards_ok, uint32_t *invalid, uint32_t epoch, uint32_t *ticket, uint32_t *err,
-              uint32_t *maxb, hipStream_t st) {
+              const uint32_t *dn, uint32_t N, bool guards_ok, uint32_t *invalid, uint32_t epoch, uint32_t *ticket,
+              uint32_t *err, uint32_t *maxb, hipStream_t st) {
     if (n == 0) return;
     const PackInfo pi{N, pl.bs, guards_ok ? 1u : 0u};
     uint32_t *counts = w.mat;
     uint32_t *offs = w.mat + (uint64_t)pl.ntiles * pl.NB;
-    hipLaunchKernelGGL(k_msd_rank, dim3(pl.ntiles), dim3(MT), 0, st, ik, iv, n, pi, pl.bshift, pl.D, w.kp, w.rk, counts,
-                       invalid);
-    hipLaunchKernelGGL(k_msd_scan, dim3(pl.nscan), dim3(MT), 0, st, counts, pl.ntiles, pl.NB, n, offs, w.bstart,
+    hipLaunchKernelGGL(k_msd_rank, dim3(pl.ntiles), dim3(MT), 0, st, ik, iv, n, dn, pi, pl.bshift, pl.D, w.kp, w.rk,
+                       counts, invalid);
+    hipLaunchKernelGGL(k_msd_scan, dim3(pl.nscan), dim3(MT), 0, st, counts, pl.ntiles, pl.NB, n, dn, offs, w.bstart,
                        w.status, epoch, ticket, err);
-    hipLaunchKernelGGL(k_msd_scatter, dim3(pl.ntiles), dim3(MT), 0, st, w.kp, w.rk, n, pl.bshift, pl.NB, offs, w.kb);
+    hipLaunchKernelGGL(k_msd_scatter, dim3(pl.ntiles), dim3(MT), 0, st, w.kp, w.rk, n, dn, pl.bshift, pl.NB, offs,
+                       w.kb);
     hipLaunchKernelGGL(k_msd_local, dim3(pl.NB), dim3(MT), 0, st, w.kb, w.kp, w.bstart, pl.R, pi, ok, ov, maxb);
 }
 

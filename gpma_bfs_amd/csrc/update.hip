@@ -19,6 +19,12 @@ namespace gpma {
 
 namespace {
 
+// Host-work flag of a captured step (see launch_step_gate).
+__global__ void k_step_gate(const uint32_t *np0, const uint32_t *np_top, const uint32_t *err, uint32_t bulk_threshold,
+                            uint32_t *gate) {
+    if (threadIdx.x == 0) *gate = (*np0 >= bulk_threshold || *np_top != 0 || *err != 0) ? 1u : 0u;
+}
+
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = v;
@@ -117,10 +123,12 @@ __device__ __forceinline__ bool search_path_matches(const gkey *__restrict__ sep
 // result is always the reference search's.
 template <uint32_t S>
 __global__ __launch_bounds__(CMP_BLOCK) void k_locate(const gkey *__restrict__ ks, const gval *__restrict__ vs,
-                                                      uint64_t n, gkey *__restrict__ keys, gval *__restrict__ values,
+                                                      uint64_t n, const uint32_t *__restrict__ dn,
+                                                      gkey *__restrict__ keys, gval *__restrict__ values,
                                                       const gkey *__restrict__ seps, uint64_t nseg,
                                                       const uint32_t *__restrict__ row_offset, uint32_t row_hint,
                                                       uint32_t *__restrict__ win) {
+    if (dn) n = *dn;
     const uint64_t i = (uint64_t)blockIdx.x * CMP_BLOCK + threadIdx.x;
     if (i >= n) return;
     const gkey k = ks[i];
@@ -663,6 +671,10 @@ void fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st) {
     if (!n) return;
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 2048)), dim3(256), 0, st, p, v, n);
 }
+void launch_step_gate(const uint32_t *np0, const uint32_t *np_top, const uint32_t *err, uint32_t bulk_threshold,
+                      uint32_t *gate, hipStream_t st) {
+    hipLaunchKernelGGL(k_step_gate, dim3(1), dim3(64), 0, st, np0, np_top, err, bulk_threshold, gate);
+}
 void fill_u8(uint8_t *p, uint8_t v, uint64_t n, hipStream_t st) {
     if (!n) return;
     hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(n, 256, 2048)), dim3(256), 0, st, p, v, n);
@@ -672,14 +684,14 @@ void fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256, 2048)), dim3(256), 0, st, p, v, n);
 }
 
-void launch_locate(const gkey *ks, const gval *vs, uint64_t n, gkey *keys, gval *values, const gkey *seps,
-                   uint64_t nseg, uint32_t S, const uint32_t *row_offset, bool row_hint, uint32_t *win,
-                   hipStream_t st) {
+void launch_locate(const gkey *ks, const gval *vs, uint64_t n, const uint32_t *dn, gkey *keys, gval *values,
+                   const gkey *seps, uint64_t nseg, uint32_t S, const uint32_t *row_offset, bool row_hint,
+                   uint32_t *win, hipStream_t st) {
     if (!n) return;
     dim3 g((uint32_t)((n + CMP_BLOCK - 1) / CMP_BLOCK)), b(CMP_BLOCK);
     const uint32_t rh = row_hint ? 1u : 0u;
 #define GPMA_LOCATE(SS) \
-    hipLaunchKernelGGL(k_locate<SS>, g, b, 0, st, ks, vs, n, keys, values, seps, nseg, row_offset, rh, win)
+    hipLaunchKernelGGL(k_locate<SS>, g, b, 0, st, ks, vs, n, dn, keys, values, seps, nseg, row_offset, rh, win)
     switch (S) {
     case 2: GPMA_LOCATE(2); break;
     case 4: GPMA_LOCATE(4); break;

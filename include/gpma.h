@@ -125,6 +125,50 @@ int gpma_bfs(gpma_t g, uint32_t start, uint32_t *levels, int on_device, uint32_t
 int gpma_bfs_multi(gpma_t g, const uint32_t *starts, uint32_t nsrc, uint32_t *levels, int on_device,
                    uint32_t *reachable);
 
+/* One batched step through a captured HIP graph (no reference counterpart;
+ * the batched form of the update -> rebalance -> BFS step that the
+ * reference's driver runs as update_gpma then gpma_bfs,
+ * gpma_bfs_demo.cu:93-104): gpma_update_batches(keys, values, offsets,
+ * nbatches, on_device) followed by gpma_bfs_multi(starts, nsrc, levels,
+ * levels_on_device, reachable), with the same results and error codes.
+ * nsrc = 0 runs the update alone.
+ *
+ * The device part is one graph launch and one host synchronisation. Graphs
+ * are captured on first use and kept per size class (the step's record count
+ * rounded up to a power of two, at least 2^16), source count and device
+ * levels pointer, for the current array layout (a resize captures anew).
+ * The kernels read the step's record count on the device and launches past
+ * it exit at once. A graph holds the rebalance levels of every window up to
+ * 4096 slots and a fixed number of BFS levels (grown to the deepest BFS seen);
+ * what lies beyond them (a bulk load, wider windows, a root resize, a deeper
+ * BFS) continues on the host path after the replay. Steps above 4M records,
+ * steps while a skewed batch keeps the bucket sort off, and steps while phase
+ * profiling is on run the host path only. */
+int gpma_step(gpma_t g, const uint64_t *keys, const uint8_t *values, const uint64_t *offsets, uint32_t nbatches,
+              int on_device, const uint32_t *starts, uint32_t nsrc, uint32_t *levels, int levels_on_device,
+              uint32_t *reachable);
+
+typedef struct gpma_step_stats {
+    uint64_t steps;         /* gpma_step calls */
+    uint64_t replays;       /* steps run as a graph replay */
+    uint64_t captures;      /* graphs captured */
+    uint64_t host_steps;    /* steps run on the host path only */
+    uint64_t update_tails;  /* replays whose update continued on the host path */
+    uint64_t bfs_tails;     /* replays whose BFS continued past the captured levels */
+    uint32_t graphs;        /* graphs held now */
+    uint32_t bfs_levels;    /* BFS levels the next capture holds */
+} gpma_step_stats;
+
+int gpma_get_step_stats(gpma_t g, gpma_step_stats *out);
+
+/* Capture ahead of time the graphs gpma_step would capture on first use, for
+ * every size class up to max_records (<= 4M), nsrc sources and the given
+ * levels destination (device pointer, or NULL for levels copied to the host
+ * or not kept), so no step pays a capture later. bfs_levels > 0 sets the BFS
+ * levels a graph holds (at most 64; a BFS deeper than that continues on the
+ * host path) for this and later captures. */
+int gpma_step_prepare(gpma_t g, uint64_t max_records, uint32_t nsrc, uint32_t *levels, uint32_t bfs_levels);
+
 /* gpma_bfs(keys, values, row_offsets, node_size, edge_size, start_node,
  * results) (reference gpma_bfs.cuh:511-515): BFS over a caller-provided
  * device CSR view in the GPMA encoding. edge_size is accepted for signature
