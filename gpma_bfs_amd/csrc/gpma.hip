@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -111,8 +112,9 @@ enum Ctr : uint32_t {
 };
 constexpr uint32_t N_TICKETS = C_LVL - C_TICKET;
 constexpr uint32_t C_LVL_SLOTS = C_COUNT - C_LVL;
-// Bucket sort for batches up to this size (its count matrix is 8 bytes per
-// item); larger batches and row counts >= 2^28 use the LSD passes.
+// Bucket sort for batches up to this size (its count and offset matrices
+// take 8 to 32 bytes per record); larger batches and row counts >= 2^28 use
+// the LSD passes.
 constexpr uint64_t MSD_MAX_N = 4ull << 20;
 constexpr uint32_t MSD_BIG_BUCKET = 16384;
 constexpr uint32_t MSD_COOL_BATCHES = 64;
@@ -159,6 +161,9 @@ struct gpma_handle {
     double pull_fraction = 0.25;  // pull when the frontier's slots exceed this share of C
     Buf ctr;
     uint32_t *hctr = nullptr;  // pinned mirror of ctr
+    // completion signal (host-coherent pinned): [0] sequence, [1..] payload
+    uint32_t *sig = nullptr;
+    uint32_t sig_seq = 0;
     // stats
     uint64_t applied = 0, invalid = 0, dropped = 0, resizes = 0, bulk = 0;
     uint32_t last_levels = 0;
@@ -272,6 +277,49 @@ int sync_read_ctr(gpma_handle *g) {
     TRY(stream_sync(g));
     HIP_TRY(hipGetLastError());
     return GPMA_OK;
+}
+
+constexpr uint32_t SIG_WORDS = 1024;
+
+// Spin until the signal holds seq. The stream is polled now and then, so an
+// error, or a drained stream whose signal never arrived, ends the wait.
+int wait_signal(gpma_handle *g, uint32_t seq) {
+    const volatile uint32_t *f = g->sig;
+    uint32_t drained = 0;  // spins since the stream was seen idle
+    for (uint32_t spin = 1; f[0] != seq; spin++) {
+        if (drained) {
+            if (++drained > (1u << 24))
+                return fail(GPMA_ERR_HIP, "completion signal missing after the stream drained");
+        } else if ((spin & 1023) == 0) {
+            const hipError_t e = hipStreamQuery(g->st);
+            if (e == hipSuccess) drained = 1;
+            else if (e != hipErrorNotReady)
+                return fail(GPMA_ERR_HIP, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    return GPMA_OK;
+}
+
+// out[0 .. na + nb) = a[0 .. na) ++ b[0 .. nb) (device u32 words, na + nb <
+// SIG_WORDS), read once everything queued before on the stream is done: a
+// one-wave kernel writes them to host-coherent memory and raises a sequence
+// number the host spins on, which takes less time than a stream
+// synchronisation after a device-to-host copy.
+int publish_and_wait(gpma_handle *g, const uint32_t *a, uint32_t na, const uint32_t *b, uint32_t nb, uint32_t *out) {
+    if (++g->sig_seq == 0) g->sig_seq = 1;
+    launch_publish(a, na, b, nb, g->sig, g->sig_seq, g->st);
+    HIP_TRY(hipGetLastError());
+    TRY(wait_signal(g, g->sig_seq));
+    std::memcpy(out, g->sig + 1, (size_t)(na + nb) * 4);
+    return GPMA_OK;
+}
+
+// Counters into hctr once the queued work is done (phase timing needs the
+// stream synchronisation, so it keeps the copy path).
+int read_ctr(gpma_handle *g) {
+    if (g->prof) return sync_read_ctr(g);
+    return publish_and_wait(g, g->ctr.as<uint32_t>(), C_COUNT, nullptr, 0, g->hctr);
 }
 
 int ensure_tmp(gpma_handle *g, uint64_t n) {
@@ -611,7 +659,7 @@ int finish_update(gpma_handle *g, uint64_t n, uint32_t spec, bool used_msd) {
         compact_step(g, pk, pv, pw, pd, np, d_np, PK[pc ^ 1]->as<gkey>(), PV[pc ^ 1]->as<gval>(),
                      PW[pc ^ 1]->as<uint32_t>(), PD[pc ^ 1]->as<uint8_t>(), mask, ctr + npslot[pc ^ 1]);
         phase_end(g, P_COMPACT, 1);
-        TRY(sync_read_ctr(g));
+        TRY(read_ctr(g));
         if (g->hctr[C_SORTERR]) return fail(GPMA_ERR_HIP, "compaction look-back exceeded its spin limit");
         pc ^= 1;
         np = g->hctr[npslot[pc]];
@@ -620,8 +668,10 @@ int finish_update(gpma_handle *g, uint64_t n, uint32_t spec, bool used_msd) {
     if (np > 0) {
         // gpma.cuh:1034-1044: grow (or shrink) and rebalance the root
         TRY(rebuild(g, PK[pc]->as<gkey>(), PV[pc]->as<gval>(), np, true));
+        TRY(stream_sync(g));
+    } else if (g->prof) {
+        TRY(stream_sync(g));  // fold the phase events
     }
-    TRY(stream_sync(g));
     HIP_TRY(hipGetLastError());
     if (inval) return fail(GPMA_ERR_KEY_RANGE, std::to_string(inval) + " keys with src/dst >= row_num skipped");
     return GPMA_OK;
@@ -658,7 +708,7 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
     }
     const uint32_t spec = enqueue_update(g, src_k, src_v, n, nullptr, guards_ok, use_msd ? &pl : nullptr,
                                          g->spec_levels);
-    TRY(sync_read_ctr(g));
+    TRY(read_ctr(g));
     return finish_update(g, n, spec, use_msd);
 }
 
@@ -709,10 +759,19 @@ int run_bfs(const gkey *keys, const gval *values, uint64_t C, const uint32_t *ro
         if (prof) phase_begin(prof, P_BFS);
         for (uint32_t j = 0; j < chunk; j++) launch_bfs_level(keys, values, C, N, row_offset, b, level + j, thr, st);
         if (prof) phase_end(prof, P_BFS, chunk);
-        HIP_TRY(hipMemcpyAsync(hcnt, b.cnt + level + 1, chunk * 8, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipMemcpyAsync(hcnt + BFS_MAX_CHUNK + 1, b.cnt_big + level + 1, chunk * 8, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        if (prof) phase_collect(prof);
+        if (prof && !prof->prof) {
+            uint32_t w[4 * BFS_MAX_CHUNK];
+            TRY(publish_and_wait(prof, (const uint32_t *)(b.cnt + level + 1), 2 * chunk,
+                                 (const uint32_t *)(b.cnt_big + level + 1), 2 * chunk, w));
+            std::memcpy(hcnt, w, chunk * 8);
+            std::memcpy(hcnt + BFS_MAX_CHUNK + 1, w + 2 * chunk, chunk * 8);
+        } else {
+            HIP_TRY(hipMemcpyAsync(hcnt, b.cnt + level + 1, chunk * 8, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipMemcpyAsync(hcnt + BFS_MAX_CHUNK + 1, b.cnt_big + level + 1, chunk * 8, hipMemcpyDeviceToHost,
+                                   st));
+            HIP_TRY(hipStreamSynchronize(st));
+            if (prof) phase_collect(prof);
+        }
         for (uint32_t j = 0; j < chunk && !done; j++) {
             const uint64_t c = (uint64_t)(uint32_t)hcnt[j] + (uint32_t)hcnt[BFS_MAX_CHUNK + 1 + j];
             if (c == 0) done = true;
@@ -792,10 +851,18 @@ int msbfs_levels(gpma_handle *g, const MsState &m, uint32_t level, uint32_t *dep
             ms_level(g->keys.as<gkey>(), g->values.as<gval>(), g->C, N, g->row_offset.as<uint32_t>(), m, level + j,
                      thr, st);
         phase_end(g, P_BFS, 2 * chunk);
-        HIP_TRY(hipMemcpyAsync(g->ms_hcnt, m.cnt + level + 1, chunk * 8, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipMemcpyAsync(g->ms_hcnt + BFS_MAX_CHUNK + 1, m.cnt_big + level + 1, chunk * 8, hipMemcpyDeviceToHost,
-                               st));
-        TRY(stream_sync(g));
+        if (!g->prof) {
+            uint32_t w[4 * BFS_MAX_CHUNK];
+            TRY(publish_and_wait(g, (const uint32_t *)(m.cnt + level + 1), 2 * chunk,
+                                 (const uint32_t *)(m.cnt_big + level + 1), 2 * chunk, w));
+            std::memcpy(g->ms_hcnt, w, chunk * 8);
+            std::memcpy(g->ms_hcnt + BFS_MAX_CHUNK + 1, w + 2 * chunk, chunk * 8);
+        } else {
+            HIP_TRY(hipMemcpyAsync(g->ms_hcnt, m.cnt + level + 1, chunk * 8, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipMemcpyAsync(g->ms_hcnt + BFS_MAX_CHUNK + 1, m.cnt_big + level + 1, chunk * 8,
+                                   hipMemcpyDeviceToHost, st));
+            TRY(stream_sync(g));
+        }
         for (uint32_t j = 0; j < chunk && !done; j++) {
             *depth = level + j;
             if ((uint32_t)g->ms_hcnt[j] == 0 && (uint32_t)g->ms_hcnt[BFS_MAX_CHUNK + 1 + j] == 0) done = true;
@@ -819,7 +886,7 @@ int run_msbfs(gpma_handle *g, const uint32_t *d_starts, uint32_t nsrc, uint32_t 
     phase_begin(g, P_BFS);
     ms_output(m, g->N, nsrc, out, reach, g->st);
     phase_end(g, P_BFS, 2);
-    TRY(stream_sync(g));
+    if (g->prof) TRY(stream_sync(g));  // callers synchronise before reading results
     HIP_TRY(hipGetLastError());
     return GPMA_OK;
 }
@@ -1167,10 +1234,12 @@ int gpma_create(uint32_t row_num, const gpma_config *cfg, gpma_t *out) {
             break;
         }
         if ((rc = g->ctr.ensure(C_COUNT * 4, &g->dev_bytes)) != GPMA_OK) break;
-        if (hipHostMalloc((void **)&g->hctr, C_COUNT * 4, hipHostMallocDefault) != hipSuccess) {
+        if (hipHostMalloc((void **)&g->hctr, C_COUNT * 4, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void **)&g->sig, SIG_WORDS * 4, hipHostMallocCoherent) != hipSuccess) {
             rc = fail(GPMA_ERR_OOM, "hipHostMalloc failed");
             break;
         }
+        std::memset(g->sig, 0, SIG_WORDS * 4);
         if ((rc = init_structure(g)) != GPMA_OK) break;
         // gpma.cuh:70-78: insert every row guard (v << 32 | 0xFFFFFFFF) with value 1
         if (row_num) {
@@ -1206,6 +1275,7 @@ int gpma_destroy(gpma_t g) {
     for (Buf *b : all) b->release(&g->dev_bytes);
     if (g->step_h) (void)hipHostFree(g->step_h);
     if (g->hctr) (void)hipHostFree(g->hctr);
+    if (g->sig) (void)hipHostFree(g->sig);
     if (g->hcnt) (void)hipHostFree(g->hcnt);
     if (g->ms_hcnt) (void)hipHostFree(g->ms_hcnt);
     if (g->st) (void)hipStreamSynchronize(g->st);

@@ -37,7 +37,8 @@ struct MsdPlan {
     uint32_t D;       // bucket bits
     uint32_t R;       // bits sorted inside a bucket
     uint32_t bshift;  // shift of the bucket bits in a packed key
-    uint32_t ntiles;  // 4096-item tiles of the batch
+    uint32_t ipt;     // items per lane of the rank and scatter tiles (4, 8 or 16)
+    uint32_t ntiles;  // tiles of 256 * ipt items in the batch
     uint32_t NB;      // buckets
     uint32_t nscan;   // scan blocks (status words used)
 };
@@ -70,6 +71,11 @@ constexpr uint32_t CHUNK = 4096;     // slots per workgroup in device-wide windo
 void launch_locate(const gkey *ks, const gval *vs, uint64_t n, const uint32_t *dn, gkey *keys, gval *values,
                    const gkey *seps, uint64_t nseg, uint32_t S, const uint32_t *row_offset, bool row_hint,
                    uint32_t *win, hipStream_t st);
+// Completion signal: copies a[0..na) then b[0..nb) (u32 words) to dst[1..]
+// (host-coherent pinned memory) and then stores seq to dst[0] with
+// system-scope release, after everything queued before it on the stream.
+void launch_publish(const uint32_t *a, uint32_t na, const uint32_t *b, uint32_t nb, uint32_t *dst, uint32_t seq,
+                    hipStream_t st);
 // *gate = 1 when the update before it left work for the host: a bulk batch
 // (*np0 >= bulk_threshold), inserts pending above the device-driven levels
 // (*np_top != 0) or a look-back error (*err != 0); else 0.

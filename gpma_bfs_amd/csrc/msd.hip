@@ -9,7 +9,7 @@
 //             one u64 carries key and value; a guard's dst (0xFFFFFFFF)
 //             packs as row_num and an out-of-range key as all ones, which
 //             keeps the order of the unpacked keys.
-//   rank      (k_msd_rank) each 4096-item tile ranks its items by bucket =
+//   rank      (k_msd_rank) each tile (1024..4096 items) ranks its items by bucket =
 //             the top D bits of the packed key (wave ballots), writes the
 //             packed keys, their rank inside (tile, bucket) and the tile's
 //             bucket counts;
@@ -37,7 +37,7 @@ constexpr uint32_t MWI = WAVE * MIPT;        // items per wave
 constexpr uint32_t MTILE = MT * MIPT;        // items per tile
 constexpr uint32_t MAX_D = 12;               // bucket bits
 constexpr uint32_t MAX_NB = 1u << MAX_D;
-constexpr uint32_t TINY = 64;                // buckets ranked by plain comparison in LDS
+constexpr uint32_t TINY = 128;               // buckets ranked by plain comparison in LDS
 constexpr uint32_t SMALL = 1024;             // buckets sorted in LDS (runs of 64, then a merge rank)
 constexpr uint32_t RUN = WAVE;               // items per register-sorted run
 constexpr uint32_t SCAN_B = WAVE;            // buckets per scan block
@@ -63,30 +63,34 @@ __device__ __forceinline__ void unpack_key(uint64_t p, const PackInfo &pi, gkey 
     k = src >= pi.N ? KEY_NONE : (((uint64_t)src << 32) | (dst == pi.N ? COL_NONE : dst));
 }
 
+// Tiles of MT * IPT items (IPT items per lane): the plan picks IPT from the
+// batch size so that small batches still spread over every CU.
+template <uint32_t IPT>
 __global__ __launch_bounds__(MT) void k_msd_rank(const gkey *__restrict__ ik, const gval *__restrict__ iv, uint64_t n,
                                                  const uint32_t *__restrict__ dn, PackInfo pi, uint32_t bshift,
                                                  uint32_t D, uint64_t *__restrict__ kp, uint16_t *__restrict__ rk,
                                                  uint32_t *__restrict__ counts, uint32_t *__restrict__ invalid) {
+    constexpr uint32_t TILE = MT * IPT, WI = WAVE * IPT;
     __shared__ uint16_t cnt[MW][MAX_NB];
     if (dn) n = *dn;
-    if ((uint64_t)blockIdx.x * MTILE >= n) return;  // tiles past the batch (grid sized for a capacity)
+    if ((uint64_t)blockIdx.x * TILE >= n) return;  // tiles past the batch (grid sized for a capacity)
     const uint32_t NB = 1u << D;
     for (uint32_t i = threadIdx.x; i < MW * MAX_NB / 2; i += MT) reinterpret_cast<uint32_t *>(&cnt[0][0])[i] = 0;
     const uint32_t wid = threadIdx.x / WAVE, lane = lane_id();
-    const uint64_t wbase = (uint64_t)blockIdx.x * MTILE + (uint64_t)wid * MWI;
-    gkey k[MIPT];
-    gval v[MIPT];
+    const uint64_t wbase = (uint64_t)blockIdx.x * TILE + (uint64_t)wid * WI;
+    gkey k[IPT];
+    gval v[IPT];
 #pragma unroll
-    for (uint32_t j = 0; j < MIPT; j++) {
+    for (uint32_t j = 0; j < IPT; j++) {
         const uint64_t idx = wbase + (uint64_t)j * WAVE + lane;
         k[j] = idx < n ? ik[idx] : 0;
         v[j] = idx < n ? iv[idx] : 0;
     }
-    uint64_t p[MIPT];
-    uint32_t b[MIPT];
+    uint64_t p[IPT];
+    uint32_t b[IPT];
     uint32_t nbad = 0;
 #pragma unroll
-    for (uint32_t j = 0; j < MIPT; j++) {
+    for (uint32_t j = 0; j < IPT; j++) {
         const uint64_t idx = wbase + (uint64_t)j * WAVE + lane;
         bool bad = false;
         p[j] = 0;
@@ -99,9 +103,9 @@ __global__ __launch_bounds__(MT) void k_msd_rank(const gkey *__restrict__ ik, co
     }
     if (lane == 0 && nbad) atomicAdd(invalid, nbad);
     __syncthreads();  // counters zeroed
-    uint32_t r[MIPT];
+    uint32_t r[IPT];
 #pragma unroll
-    for (uint32_t j = 0; j < MIPT; j++) r[j] = wave_rank<MAX_D>(b[j], b[j] < NB, cnt[wid]);
+    for (uint32_t j = 0; j < IPT; j++) r[j] = wave_rank<MAX_D>(b[j], b[j] < NB, cnt[wid]);
     __syncthreads();
     for (uint32_t d = threadIdx.x; d < NB; d += MT) {
         uint32_t run = 0;
@@ -115,7 +119,7 @@ __global__ __launch_bounds__(MT) void k_msd_rank(const gkey *__restrict__ ik, co
     }
     __syncthreads();
 #pragma unroll
-    for (uint32_t j = 0; j < MIPT; j++) {
+    for (uint32_t j = 0; j < IPT; j++) {
         const uint64_t idx = wbase + (uint64_t)j * WAVE + lane;
         if (idx < n) {
             kp[idx] = p[j];
@@ -127,9 +131,9 @@ __global__ __launch_bounds__(MT) void k_msd_rank(const gkey *__restrict__ ik, co
 // offs[t][b] = items in buckets < b + items of bucket b in tiles < t;
 // bstart[b] = offs[0][b], bstart[NB] = n. A block covers SCAN_B buckets with
 // four tile groups; blocks chain their bucket totals by look-back. With dn,
-// n = *dn and only its ceil(n / MTILE) tiles take part.
-__global__ __launch_bounds__(MT) void k_msd_scan(const uint32_t *__restrict__ counts, uint32_t ntiles, uint32_t NB,
-                                                 uint64_t n, const uint32_t *__restrict__ dn,
+// n = *dn and only its ceil(n / tile) tiles take part.
+__global__ __launch_bounds__(MT) void k_msd_scan(const uint32_t *__restrict__ counts, uint32_t ntiles, uint32_t tile,
+                                                 uint32_t NB, uint64_t n, const uint32_t *__restrict__ dn,
                                                  uint32_t *__restrict__ offs, uint32_t *__restrict__ bstart,
                                                  unsigned long long *status, uint32_t epoch,
                                                  uint32_t *__restrict__ ticket, uint32_t *__restrict__ err) {
@@ -138,7 +142,7 @@ __global__ __launch_bounds__(MT) void k_msd_scan(const uint32_t *__restrict__ co
     __shared__ uint32_t s_blk;
     if (dn) {
         n = *dn;
-        ntiles = min(ntiles, (uint32_t)((n + MTILE - 1) / MTILE));
+        ntiles = min(ntiles, (uint32_t)((n + tile - 1) / tile));
     }
     if (threadIdx.x == 0) s_blk = atomicAdd(ticket, 1u);
     __syncthreads();
@@ -186,24 +190,25 @@ __global__ __launch_bounds__(MT) void k_msd_scan(const uint32_t *__restrict__ co
     if (blk == 0 && threadIdx.x == 0) bstart[NB] = (uint32_t)n;
 }
 
+template <uint32_t IPT>
 __global__ __launch_bounds__(MT) void k_msd_scatter(const uint64_t *__restrict__ kp, const uint16_t *__restrict__ rk,
                                                     uint64_t n, const uint32_t *__restrict__ dn, uint32_t bshift,
                                                     uint32_t NB, const uint32_t *__restrict__ offs,
                                                     uint64_t *__restrict__ kb) {
     if (dn) n = *dn;
-    const uint64_t base = (uint64_t)blockIdx.x * MTILE;
+    const uint64_t base = (uint64_t)blockIdx.x * (MT * IPT);
     if (base >= n) return;
     const uint32_t *o = offs + (uint64_t)blockIdx.x * NB;
-    uint64_t p[MIPT];
-    uint32_t r[MIPT];
+    uint64_t p[IPT];
+    uint32_t r[IPT];
 #pragma unroll
-    for (uint32_t j = 0; j < MIPT; j++) {
+    for (uint32_t j = 0; j < IPT; j++) {
         const uint64_t idx = base + (uint64_t)j * MT + threadIdx.x;
         p[j] = idx < n ? kp[idx] : 0;
         r[j] = idx < n ? rk[idx] : 0;
     }
 #pragma unroll
-    for (uint32_t j = 0; j < MIPT; j++) {
+    for (uint32_t j = 0; j < IPT; j++) {
         const uint64_t idx = base + (uint64_t)j * MT + threadIdx.x;
         if (idx < n) kb[(uint64_t)o[(uint32_t)(p[j] >> bshift) & (NB - 1)] + r[j]] = p[j];
     }
@@ -409,7 +414,9 @@ MsdPlan msd_plan(uint64_t n, uint32_t N, uint32_t bucket_bits) {
     pl.D = D;
     pl.R = B - D;
     pl.bshift = 8 + B - D;
-    pl.ntiles = (uint32_t)((n + MTILE - 1) / MTILE);
+    // items per lane: enough tiles (about 400) to occupy every CU
+    pl.ipt = n >= 400ull * MT * 16 ? 16 : n >= 400ull * MT * 8 ? 8 : 4;
+    pl.ntiles = (uint32_t)((n + MT * pl.ipt - 1) / (MT * pl.ipt));
     pl.NB = 1u << D;
     pl.nscan = (pl.NB + SCAN_B - 1) / SCAN_B;
     return pl;
@@ -422,12 +429,19 @@ void msd_sort(const gkey *ik, const gval *iv, gkey *ok, gval *ov, const MsdPlan 
     const PackInfo pi{N, pl.bs, guards_ok ? 1u : 0u};
     uint32_t *counts = w.mat;
     uint32_t *offs = w.mat + (uint64_t)pl.ntiles * pl.NB;
-    hipLaunchKernelGGL(k_msd_rank, dim3(pl.ntiles), dim3(MT), 0, st, ik, iv, n, dn, pi, pl.bshift, pl.D, w.kp, w.rk,
-                       counts, invalid);
-    hipLaunchKernelGGL(k_msd_scan, dim3(pl.nscan), dim3(MT), 0, st, counts, pl.ntiles, pl.NB, n, dn, offs, w.bstart,
-                       w.status, epoch, ticket, err);
-    hipLaunchKernelGGL(k_msd_scatter, dim3(pl.ntiles), dim3(MT), 0, st, w.kp, w.rk, n, dn, pl.bshift, pl.NB, offs,
-                       w.kb);
+#define GPMA_MSD_TILE(IPT)                                                                                    \
+    hipLaunchKernelGGL(k_msd_rank<IPT>, dim3(pl.ntiles), dim3(MT), 0, st, ik, iv, n, dn, pi, pl.bshift, pl.D, w.kp, \
+                       w.rk, counts, invalid);                                                                    \
+    hipLaunchKernelGGL(k_msd_scan, dim3(pl.nscan), dim3(MT), 0, st, counts, pl.ntiles, MT * IPT, pl.NB, n, dn,     \
+                       offs, w.bstart, w.status, epoch, ticket, err);                                              \
+    hipLaunchKernelGGL(k_msd_scatter<IPT>, dim3(pl.ntiles), dim3(MT), 0, st, w.kp, w.rk, n, dn, pl.bshift, pl.NB,   \
+                       offs, w.kb)
+    switch (pl.ipt) {
+    case 4: GPMA_MSD_TILE(4); break;
+    case 8: GPMA_MSD_TILE(8); break;
+    default: GPMA_MSD_TILE(16); break;
+    }
+#undef GPMA_MSD_TILE
     hipLaunchKernelGGL(k_msd_local, dim3(pl.NB), dim3(MT), 0, st, w.kb, w.kp, w.bstart, pl.R, pi, ok, ov, maxb);
 }
 

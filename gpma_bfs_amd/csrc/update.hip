@@ -19,6 +19,17 @@ namespace gpma {
 
 namespace {
 
+// Copies a[0..na) and b[0..nb) to dst[1..] in host-coherent memory, then
+// raises dst[0] to seq (system-scope release) for a host spinning on it.
+__global__ void k_publish(const uint32_t *a, uint32_t na, const uint32_t *b, uint32_t nb, uint32_t *dst,
+                          uint32_t seq) {
+    const uint32_t lane = threadIdx.x;  // one wave
+    for (uint32_t i = lane; i < na; i += WAVE) dst[1 + i] = a[i];
+    for (uint32_t i = lane; i < nb; i += WAVE) dst[1 + na + i] = b[i];
+    __threadfence_system();
+    if (lane == 0) __hip_atomic_store(dst, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // Host-work flag of a captured step (see launch_step_gate).
 __global__ void k_step_gate(const uint32_t *np0, const uint32_t *np_top, const uint32_t *err, uint32_t bulk_threshold,
                             uint32_t *gate) {
@@ -670,6 +681,10 @@ __global__ __launch_bounds__(256) void k_write_compacted(const gkey *__restrict_
 void fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st) {
     if (!n) return;
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 2048)), dim3(256), 0, st, p, v, n);
+}
+void launch_publish(const uint32_t *a, uint32_t na, const uint32_t *b, uint32_t nb, uint32_t *dst, uint32_t seq,
+                    hipStream_t st) {
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(WAVE), 0, st, a, na, b, nb, dst, seq);
 }
 void launch_step_gate(const uint32_t *np0, const uint32_t *np_top, const uint32_t *err, uint32_t bulk_threshold,
                       uint32_t *gate, hipStream_t st) {
