@@ -829,6 +829,15 @@ MsState ms_state(gpma_handle *g) {
     return m;
 }
 
+// Pull threshold of the multi-source traversal, in frontier slots: twice the
+// single-source share of C. Its streaming step ORs a 64-bit mask per slot,
+// so pushing stays cheaper for longer: on the power-law 1.63M-node graph
+// with 64 random sources, 2.22 ms with the single-source 25%, 1.90 ms at 50%
+// or above, 3.45 ms when every level pulls.
+unsigned long long ms_pull_threshold(const gpma_handle *g) {
+    return (unsigned long long)(std::min(1.0, 2.0 * g->pull_fraction) * (double)g->C);
+}
+
 // Multi-source levels from `level` on, in chunks without a host round trip
 // per level (like run_bfs), until a level adds no vertex. cnt[level] must
 // hold the level's frontier counts and the touched count of `level` must be
@@ -836,7 +845,7 @@ MsState ms_state(gpma_handle *g) {
 int msbfs_levels(gpma_handle *g, const MsState &m, uint32_t level, uint32_t *depth) {
     hipStream_t st = g->st;
     const uint32_t N = g->N;
-    const unsigned long long thr = (unsigned long long)(g->pull_fraction * (double)g->C);
+    const unsigned long long thr = ms_pull_threshold(g);
     uint32_t chunk = 4;
     bool done = false;
     *depth = level;
@@ -948,7 +957,7 @@ int capture_step(gpma_handle *g, uint32_t cls, const MsdPlan &pl, uint32_t uleve
     launch_step_gate(ctr + C_LVL, ctr + C_LVL + spec + 1, ctr + C_SORTERR, g->bulk_threshold, ctr + C_GATE, st);
     if (nsrc) {
         const MsState m = ms_state(g);
-        const unsigned long long thr = (unsigned long long)(g->pull_fraction * (double)g->C);
+        const unsigned long long thr = ms_pull_threshold(g);
         ms_init(m, g->N, g->ms_starts.as<uint32_t>(), nsrc, g->row_offset.as<uint32_t>(), st, ctr + C_GATE);
         ms_clear(m, 2, blevels + 2, st);
         for (uint32_t L = 1; L <= blevels; L++)

@@ -385,52 +385,45 @@ __global__ __launch_bounds__(T) void k_ms_finalize(const uint32_t *__restrict__ 
 
 // Transpose the vertex-major u16 levels into source-major u32 arrays (an
 // entry counts only where the source's bit is set in seen[v]) and count
-// reachable vertices per source: wave w writes the rows of sources w, w+4,
-// ... as coalesced 64-vertex runs and counts them with a ballot; partials per
-// workgroup.
+// reachable vertices per source. One wave per 64-vertex group, no LDS and no
+// barrier: each lane loads its vertex's 128-byte level row (only if some
+// source reached it), then for every source the wave writes the group's 64
+// levels as one coalesced store, and lane s counts source s with a ballot.
+// partial[wave][s]: per-wave counts.
 __global__ __launch_bounds__(T) void k_ms_output(const uint16_t *__restrict__ lv16, const uint64_t *__restrict__ seen,
                                                  uint32_t N, uint32_t nsrc, uint32_t *__restrict__ out,
                                                  uint32_t *__restrict__ partial) {
-    constexpr uint32_t NW = T / WAVE;
-    __shared__ uint16_t tile[64][64 + 2];
-    __shared__ uint64_t sm[64];
-    const uint32_t wid = threadIdx.x / WAVE, lane = lane_id();
-    uint32_t acc[64 / NW];
+    const uint32_t lane = lane_id();
+    const uint32_t gw = (blockIdx.x * T + threadIdx.x) / WAVE;
+    const uint32_t nw = gridDim.x * (T / WAVE);
+    uint32_t acc = 0;
+    for (uint64_t v0 = (uint64_t)gw * WAVE; v0 < N; v0 += (uint64_t)nw * WAVE) {
+        const uint64_t v = v0 + lane;
+        const bool in = v < N;
+        const uint64_t mask = in ? seen[v] : 0;
+        uint4 row[8];
+        if (mask) {
+            const uint4 *src = reinterpret_cast<const uint4 *>(lv16 + v * 64);
 #pragma unroll
-    for (uint32_t k = 0; k < 64 / NW; k++) acc[k] = 0;
-    for (uint64_t v0 = (uint64_t)blockIdx.x * 64; v0 < N; v0 += (uint64_t)gridDim.x * 64) {
-        for (uint32_t i = threadIdx.x; i < 64 * 32; i += T) {
-            const uint32_t vv = i / 32, pair = i % 32;
-            uint32_t x = 0;
-            if (v0 + vv < N) x = ((const uint32_t *)(lv16 + (v0 + vv) * 64))[pair];
-            tile[vv][2 * pair] = (uint16_t)x;
-            tile[vv][2 * pair + 1] = (uint16_t)(x >> 16);
+            for (uint32_t q = 0; q < 8; q++) row[q] = src[q];
+        } else {
+#pragma unroll
+            for (uint32_t q = 0; q < 8; q++) row[q] = make_uint4(0, 0, 0, 0);
         }
-        if (threadIdx.x < 64) sm[threadIdx.x] = v0 + threadIdx.x < N ? seen[v0 + threadIdx.x] : 0;
-        __syncthreads();
-        const bool in = v0 + lane < N;
-        const uint64_t mask = sm[lane];
 #pragma unroll
-        for (uint32_t k = 0; k < 64 / NW; k++) {
-            const uint32_t s = wid + NW * k;
-            if (s < nsrc) {
-                const uint32_t l = ((mask >> s) & 1ull) ? tile[lane][s] : 0u;
-                if (in) out[(uint64_t)s * N + v0 + lane] = l;
-                acc[k] += (uint32_t)__popcll(ballot(l != 0));
-            }
-        }
-        __syncthreads();
-    }
-    if (lane == 0) {
-#pragma unroll
-        for (uint32_t k = 0; k < 64 / NW; k++) {
-            const uint32_t s = wid + NW * k;
-            if (s < nsrc) partial[(uint64_t)blockIdx.x * 64 + s] = acc[k];
+        for (uint32_t s = 0; s < 64; s++) {
+            const uint4 &r4 = row[s / 8];
+            const uint32_t word = (s % 8) < 2 ? r4.x : (s % 8) < 4 ? r4.y : (s % 8) < 6 ? r4.z : r4.w;
+            const bool hit = (mask >> s) & 1ull;
+            if (s < nsrc && in) out[(uint64_t)s * N + v] = hit ? (word >> (16 * (s & 1))) & 0xFFFFu : 0u;
+            const uint32_t c = (uint32_t)__popcll(ballot(hit));
+            if (lane == s) acc += c;
         }
     }
+    partial[(uint64_t)gw * WAVE + lane] = acc;
 }
 
-// reach[s] = sum of the workgroups' partials (one workgroup per source).
+// reach[s] = sum of the partials' column s (one workgroup per source).
 __global__ __launch_bounds__(T) void k_ms_reach(const uint32_t *__restrict__ partial, uint32_t nblocks,
                                                 uint32_t *__restrict__ reach) {
     __shared__ uint32_t scratch[T / WAVE + 1];
@@ -477,12 +470,17 @@ void ms_level(const gkey *keys, const gval *values, uint64_t C, uint32_t N, cons
                        m.cnt + level + 1, m.cnt_big + level + 1, ro, level);
 }
 
-void ms_output(const MsState &m, uint32_t N, uint32_t nsrc, uint32_t *out, uint32_t *reach, hipStream_t st) {
-    const uint32_t nb = (uint32_t)std::min<uint64_t>(2048, ((uint64_t)N + 63) / 64);
-    hipLaunchKernelGGL(k_ms_output, dim3(std::max(nb, 1u)), dim3(T), 0, st, m.lv16, m.seen, N, nsrc, out, m.partial);
-    hipLaunchKernelGGL(k_ms_reach, dim3(nsrc), dim3(T), 0, st, m.partial, std::max(nb, 1u), reach);
+// Workgroups of the output pass: one wave per 64-vertex group, at most 2048.
+static uint32_t ms_output_grid(uint32_t N) {
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(2048, ((uint64_t)N + T - 1) / T));
 }
 
-uint32_t ms_output_blocks(uint32_t N) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(2048, ((uint64_t)N + 63) / 64)); }
+void ms_output(const MsState &m, uint32_t N, uint32_t nsrc, uint32_t *out, uint32_t *reach, hipStream_t st) {
+    const uint32_t nb = ms_output_grid(N);
+    hipLaunchKernelGGL(k_ms_output, dim3(nb), dim3(T), 0, st, m.lv16, m.seen, N, nsrc, out, m.partial);
+    hipLaunchKernelGGL(k_ms_reach, dim3(nsrc), dim3(T), 0, st, m.partial, nb * (T / WAVE), reach);
+}
+
+uint32_t ms_output_blocks(uint32_t N) { return ms_output_grid(N) * (T / WAVE); }
 
 }  // namespace gpma

@@ -46,7 +46,8 @@ typedef struct gpma_config {
     int32_t device;            /* HIP device ordinal, -1 = current device */
     uint32_t bulk_threshold;   /* pending inserts that take the bulk path (reference: 5,000,000) */
     uint32_t pull_permille;    /* BFS streams the whole array when the frontier's slots exceed
-                                  this many thousandths of the capacity (0 = 250) */
+                                  this many thousandths of the capacity (0 = 250); the
+                                  multi-source BFS uses twice this share (at most all of it) */
     /* Tuning knobs (0 = default). None changes the live-key set or BFS results;
      * segment_length changes the physical layout (the reference's formula is
      * S = 2^(fls(fls C) - 1)). */
