@@ -37,6 +37,7 @@ S_REF = 16  # reference segment length at every capacity in [2^15, 2^30]
 
 TRAFFIC_FILE = "profiles/c2_update_hbm_traffic.md"
 TUNED_FILE = "gpma_bfs_amd/tuned/mi355x_powerlaw_1p6m.json"
+TUNED_BATCHED_FILE = "gpma_bfs_amd/tuned/mi355x_powerlaw_1p6m_batched.json"
 
 
 def measured_traffic(phase):
@@ -122,11 +123,13 @@ def run_batched(G, N, keys, half, steps, warmup, nsrc=64, fused=8, check=True):
     need = int(sizes.sum() // 2)
     if half + need > len(keys):
         raise RuntimeError("not enough edges for the requested batched steps")
-    knobs = {}
-    tuned = os.path.join(os.path.dirname(os.path.abspath(__file__)), TUNED_FILE)
-    if os.path.exists(tuned):
-        from gpma_bfs_amd import tune
-        knobs = tune.load(tuned)
+    knobs, tuned_file = {}, None
+    for f in (TUNED_BATCHED_FILE, TUNED_FILE):  # the batched tuning first
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), f)
+        if os.path.exists(path):
+            from gpma_bfs_amd import tune
+            knobs, tuned_file = tune.load(path), f
+            break
     g = G.GPMA(N, tuning=knobs)
     G.update_gpma(g, keys[:half], np.ones(half, np.uint8))
     # stage every step's fused batches in HBM
@@ -186,7 +189,7 @@ def run_batched(G, N, keys, half, steps, warmup, nsrc=64, fused=8, check=True):
            "step_p50_ms": float(np.percentile(np.array(per) * 1e3, 50)),
            "step_p99_ms": float(np.percentile(np.array(per) * 1e3, 99)),
            "phase_ms_per_step": brk, "phase_breakdown_steps": nprof,
-           "tuning": {"file": TUNED_FILE if knobs else None, "knobs": knobs}}
+           "tuning": {"file": tuned_file, "knobs": knobs}}
     # algorithmic bytes and TEPS of the last step's BFS, and parity spot checks
     win = keys[dele: ins]
     outdeg = np.bincount((win >> np.uint64(32)).astype(np.int64), minlength=N)

@@ -10,6 +10,12 @@ JSON file that GPMA(..., tuning=load(path)) takes.
 
     python -m gpma_bfs_amd.tune --out tuned.json            # power-law 1.63M-node graph
     python -m gpma_bfs_amd.tune --rmat-scale 20 --out t.json
+    python -m gpma_bfs_amd.tune --batched --out tb.json     # captured batched steps
+
+--batched tunes the batched form instead: steps of 8 fused log-uniform
+1e3..1e6-record batches plus a 64-source BFS, each one gpma_step (a captured
+HIP graph), checked against the closed-form live set and the levels of the
+default configuration.
 """
 import argparse
 import json
@@ -23,6 +29,11 @@ UPDATE_SPACE = {
     "sort_bucket_bits": [0, 9, 10, 11, 12],
 }
 BFS_SPACE = {"pull_permille": [0, 50, 100, 500]}
+BATCHED_SPACE = {
+    "segment_length": [0, 8, 32],
+    "sort_bucket_bits": [0, 10, 11, 12],
+    "pull_permille": [0, 500],
+}
 
 
 def coordinate_descent(measure, space, start=None, max_sweeps=3):
@@ -140,6 +151,79 @@ def tune_bfs(G, N, keys, knobs, space=BFS_SPACE, reps=3, log=print):
     return coordinate_descent(measure, space)
 
 
+def batched_steps(N, keys, steps, fused=8, nsrc=64, seed=2):
+    """Step inputs of the batched form: per step, `fused` batches of
+    log-uniform 1e3..1e6 records (half inserts of the next window edges, half
+    deletes of the oldest) and `nsrc` uniform sources. Returns (list of
+    (keys, values, offsets, starts), inserted, deleted)."""
+    half = len(keys) // 2
+    rs = np.random.default_rng(seed)
+    sizes = np.exp(rs.uniform(np.log(1e3), np.log(1e6), steps * fused)).astype(np.int64) // 2 * 2
+    starts = rs.integers(0, N, (steps, nsrc)).astype(np.uint32)
+    ins, dele, out = half, 0, []
+    for i in range(steps):
+        ks, vs, offs = [], [], [0]
+        for b in range(fused):
+            h = int(sizes[i * fused + b] // 2)
+            ks += [keys[ins: ins + h], keys[dele: dele + h]]
+            vs += [np.ones(h, np.uint8), np.zeros(h, np.uint8)]
+            ins += h
+            dele += h
+            offs.append(offs[-1] + 2 * h)
+        out.append((np.concatenate(ks), np.concatenate(vs), np.array(offs, np.uint64), starts[i]))
+    if ins > len(keys):
+        raise RuntimeError("not enough edges for the batched steps")
+    return out, ins, dele
+
+
+def tune_batched(G, N, keys, steps=12, warmup=2, space=BATCHED_SPACE, log=print):
+    """Time `steps` captured batched steps per candidate (graphs captured
+    before the timed steps); a candidate counts only if its live set equals
+    the closed form and its last step's levels equal the default
+    configuration's. Returns (best knobs, seconds per step, evaluations)."""
+    half = len(keys) // 2
+    plan, ins, dele = batched_steps(N, keys, warmup + steps)
+    dk = [G.DeviceBuffer.from_numpy(p[0]) for p in plan]
+    dv = [G.DeviceBuffer.from_numpy(p[1]) for p in plan]
+    dlev = G.DeviceBuffer(64 * N * 4)
+    expect = _closed_form(G, keys[dele: ins], N)
+    max_records = max(int(p[2][-1]) for p in plan)
+    ref = {}
+
+    def measure(cfg):
+        try:
+            g = G.GPMA(N, tuning=cfg)
+        except G.GpmaError:
+            return None
+        try:
+            G.update_gpma(g, keys[:half], np.ones(half, np.uint8))
+            G.gpma_step_prepare(g, max_records, 64, dlev.ptr)
+            for i in range(warmup):
+                G.gpma_step(g, dk[i].ptr, dv[i].ptr, plan[i][2], plan[i][3], levels_device_ptr=dlev.ptr)
+            G.gpma_step_prepare(g, max_records, 64, dlev.ptr)  # at the BFS depth the warm-up found
+            g.sync()
+            a = time.perf_counter()
+            for i in range(warmup, warmup + steps):
+                G.gpma_step(g, dk[i].ptr, dv[i].ptr, plan[i][2], plan[i][3], levels_device_ptr=dlev.ptr)
+            g.sync()
+            t = (time.perf_counter() - a) / steps
+            ok = np.array_equal(g.export_live(), expect)
+            lv = dlev.to_numpy(np.uint32, 64 * N)
+            if "levels" not in ref:
+                ref["levels"] = lv
+            ok = ok and np.array_equal(lv, ref["levels"])
+        finally:
+            g.close()
+        log(f"tune: {cfg} -> {t * 1e3:.3f} ms/step{'' if ok else ' (WRONG result: rejected)'}")
+        return t if ok else None
+
+    try:
+        return coordinate_descent(measure, space)
+    finally:
+        for b in dk + dv + [dlev]:
+            b.free()
+
+
 def load(path):
     """Knobs from a tuning file (for GPMA(..., tuning=...))."""
     with open(path) as f:
@@ -152,6 +236,7 @@ def main():
     ap.add_argument("--out", required=True)
     ap.add_argument("--rmat-scale", type=int, default=0, help="R-MAT graph (2^scale nodes, 16x draws) instead")
     ap.add_argument("--slides", type=int, default=95)
+    ap.add_argument("--batched", action="store_true", help="tune the captured batched step instead")
     args = ap.parse_args()
     import gpma_bfs_amd as G
 
@@ -165,6 +250,15 @@ def main():
         workload = "power-law N=1,632,803 E=30,622,564 (alpha 2.1, soft cap 2800, p1 0.24, seed 0)"
     keys = G.make_keys(src, dst)
     t0 = time.time()
+    if args.batched:
+        knobs, t_step, n_eval = tune_batched(G, N, keys)
+        out = {"workload": workload + "; batched steps: 8 fused log-uniform 1e3..1e6-record batches + "
+                                      "64-source BFS (seed 2), one captured graph per step",
+               "knobs": knobs, "step_ms": t_step * 1e3, "evaluations": n_eval, "seconds": time.time() - t0}
+        with open(args.out, "w") as f:
+            json.dump(out, f, indent=1)
+        print(json.dumps(out))
+        return
     knobs, t_upd, n_upd = tune_updates(G, N, keys, slides=args.slides)
     bfs_knobs, t_bfs, n_bfs = tune_bfs(G, N, keys, knobs)
     knobs.update(bfs_knobs)
