@@ -161,6 +161,7 @@ struct gpma_handle {
     double pull_fraction = 0.25;  // pull when the frontier's slots exceed this share of C
     Buf ctr;
     uint32_t *hctr = nullptr;  // pinned mirror of ctr
+    bool ctr_clean = false;    // ctr is all zero on the device (the last read cleared it)
     // completion signal (host-coherent pinned): [0] sequence, [1..] payload
     uint32_t *sig = nullptr;
     uint32_t sig_seq = 0;
@@ -306,20 +307,27 @@ int wait_signal(gpma_handle *g, uint32_t seq) {
 // one-wave kernel writes them to host-coherent memory and raises a sequence
 // number the host spins on, which takes less time than a stream
 // synchronisation after a device-to-host copy.
-int publish_and_wait(gpma_handle *g, const uint32_t *a, uint32_t na, const uint32_t *b, uint32_t nb, uint32_t *out) {
+int publish_and_wait(gpma_handle *g, const uint32_t *a, uint32_t na, const uint32_t *b, uint32_t nb, uint32_t *out,
+                     bool clear_a = false) {
     if (++g->sig_seq == 0) g->sig_seq = 1;
-    launch_publish(a, na, b, nb, g->sig, g->sig_seq, g->st);
+    launch_publish(a, na, b, nb, g->sig, g->sig_seq, clear_a, g->st);
     HIP_TRY(hipGetLastError());
     TRY(wait_signal(g, g->sig_seq));
     std::memcpy(out, g->sig + 1, (size_t)(na + nb) * 4);
     return GPMA_OK;
 }
 
-// Counters into hctr once the queued work is done (phase timing needs the
-// stream synchronisation, so it keeps the copy path).
+// Counters into hctr once the queued work is done, leaving them zero on the
+// device (the next update then needs no clearing launch). Phase timing needs
+// the stream synchronisation, so it keeps the copy path.
 int read_ctr(gpma_handle *g) {
-    if (g->prof) return sync_read_ctr(g);
-    return publish_and_wait(g, g->ctr.as<uint32_t>(), C_COUNT, nullptr, 0, g->hctr);
+    if (g->prof) {
+        g->ctr_clean = false;
+        return sync_read_ctr(g);
+    }
+    TRY(publish_and_wait(g, g->ctr.as<uint32_t>(), C_COUNT, nullptr, 0, g->hctr, true));
+    g->ctr_clean = true;
+    return GPMA_OK;
 }
 
 int ensure_tmp(gpma_handle *g, uint64_t n) {
@@ -349,7 +357,9 @@ int count_live(gpma_handle *g, uint64_t w, uint64_t W, uint64_t *c) {
     launch_count_live(g->keys.as<gkey>(), g->values.as<gval>(), w, W, g->chunkcnt.as<uint32_t>(), g->st);
     launch_scan_blocks(g->chunkcnt.as<uint32_t>(), nch, g->chunkoff.as<uint32_t>(), g->ctr.as<uint32_t>() + C_LIVE,
                        g->st);
-    TRY(sync_read_ctr(g));
+    g->ctr_clean = false;
+    TRY(read_ctr(g));
+    if (g->hctr[C_SORTERR]) return fail(GPMA_ERR_HIP, "compaction look-back exceeded its spin limit");
     *c = g->hctr[C_LIVE];
     return GPMA_OK;
 }
@@ -608,12 +618,11 @@ int finish_update(gpma_handle *g, uint64_t n, uint32_t spec, bool used_msd) {
     int pc = level & 1;
     uint64_t np = g->hctr[C_LVL + level];
     const uint32_t npslot[2] = {C_NP0, C_NP1};
-    if (np) {
-        // move the count into the host loop's slot for this list
-        fill_u32(ctr + npslot[pc], (uint32_t)np, 1, st);
-    }
     while (np > 0 && level <= g->h) {
         const uint64_t W = (uint64_t)g->S << level;
+        // the list's count into its device slot (reading the counters cleared them)
+        fill_u32(ctr + npslot[pc], (uint32_t)np, 1, st);
+        g->ctr_clean = false;
         gkey *pk = PK[pc]->as<gkey>();
         gval *pv = PV[pc]->as<gval>();
         uint32_t *pw = PW[pc]->as<uint32_t>();
@@ -655,6 +664,8 @@ int finish_update(gpma_handle *g, uint64_t n, uint32_t spec, bool used_msd) {
             }
         }
         const uint32_t mask = ~(uint32_t)(((uint64_t)g->S << (level + 1)) - 1);
+        if (W > WIN_MAX) fill_u32(d_np, (uint32_t)np, 1, st);  // count_live's reads cleared it
+        g->ctr_clean = false;
         phase_begin(g, P_COMPACT);
         compact_step(g, pk, pv, pw, pd, np, d_np, PK[pc ^ 1]->as<gkey>(), PV[pc ^ 1]->as<gval>(),
                      PW[pc ^ 1]->as<uint32_t>(), PD[pc ^ 1]->as<uint8_t>(), mask, ctr + npslot[pc ^ 1]);
@@ -689,7 +700,17 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
     HIP_TRY(hipSetDevice(g->device));
     TRY(ensure_update_buffers(g, n));
     hipStream_t st = g->st;
-    fill_u32(g->ctr.as<uint32_t>(), 0, C_COUNT, st);
+    // host-side preparation before the first launch, so the launches that
+    // follow reach the GPU back to back
+    const bool use_msd = msd_usable(g, n);
+    if (g->msd_cool) g->msd_cool--;
+    MsdPlan pl{};
+    if (use_msd) {
+        pl = msd_plan(n, g->N, g->sort_bits);
+        TRY(ensure_msd(g, n, pl));
+    }
+    if (!g->ctr_clean) fill_u32(g->ctr.as<uint32_t>(), 0, C_COUNT, st);
+    g->ctr_clean = false;
     g->cc_ticket = 0;
     const gkey *src_k = keys;
     const gval *src_v = values;
@@ -698,13 +719,6 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
         HIP_TRY(hipMemcpyAsync(g->in_v.p, values, n, hipMemcpyHostToDevice, st));
         src_k = g->in_k.as<gkey>();
         src_v = g->in_v.as<gval>();
-    }
-    const bool use_msd = msd_usable(g, n);
-    if (g->msd_cool) g->msd_cool--;
-    MsdPlan pl{};
-    if (use_msd) {
-        pl = msd_plan(n, g->N, g->sort_bits);
-        TRY(ensure_msd(g, n, pl));
     }
     const uint32_t spec = enqueue_update(g, src_k, src_v, n, nullptr, guards_ok, use_msd ? &pl : nullptr,
                                          g->spec_levels);
@@ -1051,6 +1065,7 @@ int run_step(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint64
     fill_u32(g->step_n.as<uint32_t>(), (uint32_t)n, 1, st);
     if (nsrc) HIP_TRY(hipMemcpyAsync(g->ms_starts.p, starts, nsrc * 4, hipMemcpyHostToDevice, st));
     const uint32_t spec = sg->spec;
+    g->ctr_clean = false;
     HIP_TRY(hipGraphLaunch(sg->exec, st));
     g->sst.replays++;
     // one read-back: counters, the frontier counts after the last captured

@@ -72,10 +72,11 @@ void launch_locate(const gkey *ks, const gval *vs, uint64_t n, const uint32_t *d
                    const gkey *seps, uint64_t nseg, uint32_t S, const uint32_t *row_offset, bool row_hint,
                    uint32_t *win, hipStream_t st);
 // Completion signal: copies a[0..na) then b[0..nb) (u32 words) to dst[1..]
-// (host-coherent pinned memory) and then stores seq to dst[0] with
+// (host-coherent pinned memory), zeroes a[0..na) when clear_a (a must then
+// be writable device memory), and then stores seq to dst[0] with
 // system-scope release, after everything queued before it on the stream.
 void launch_publish(const uint32_t *a, uint32_t na, const uint32_t *b, uint32_t nb, uint32_t *dst, uint32_t seq,
-                    hipStream_t st);
+                    bool clear_a, hipStream_t st);
 // *gate = 1 when the update before it left work for the host: a bulk batch
 // (*np0 >= bulk_threshold), inserts pending above the device-driven levels
 // (*np_top != 0) or a look-back error (*err != 0); else 0.

@@ -19,12 +19,16 @@ namespace gpma {
 
 namespace {
 
-// Copies a[0..na) and b[0..nb) to dst[1..] in host-coherent memory, then
-// raises dst[0] to seq (system-scope release) for a host spinning on it.
-__global__ void k_publish(const uint32_t *a, uint32_t na, const uint32_t *b, uint32_t nb, uint32_t *dst,
-                          uint32_t seq) {
+// Copies a[0..na) and b[0..nb) to dst[1..] in host-coherent memory (and
+// zeroes a[0..na) if clear_a), then raises dst[0] to seq (system-scope
+// release) for a host spinning on it.
+__global__ void k_publish(uint32_t *a, uint32_t na, const uint32_t *b, uint32_t nb, uint32_t *dst, uint32_t seq,
+                          uint32_t clear_a) {
     const uint32_t lane = threadIdx.x;  // one wave
-    for (uint32_t i = lane; i < na; i += WAVE) dst[1 + i] = a[i];
+    for (uint32_t i = lane; i < na; i += WAVE) {
+        dst[1 + i] = a[i];
+        if (clear_a) a[i] = 0;
+    }
     for (uint32_t i = lane; i < nb; i += WAVE) dst[1 + na + i] = b[i];
     __threadfence_system();
     if (lane == 0) __hip_atomic_store(dst, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -683,8 +687,9 @@ void fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 2048)), dim3(256), 0, st, p, v, n);
 }
 void launch_publish(const uint32_t *a, uint32_t na, const uint32_t *b, uint32_t nb, uint32_t *dst, uint32_t seq,
-                    hipStream_t st) {
-    hipLaunchKernelGGL(k_publish, dim3(1), dim3(WAVE), 0, st, a, na, b, nb, dst, seq);
+                    bool clear_a, hipStream_t st) {
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(WAVE), 0, st, const_cast<uint32_t *>(a), na, b, nb, dst, seq,
+                       clear_a ? 1u : 0u);
 }
 void launch_step_gate(const uint32_t *np0, const uint32_t *np_top, const uint32_t *err, uint32_t bulk_threshold,
                       uint32_t *gate, hipStream_t st) {
