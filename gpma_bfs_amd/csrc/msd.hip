@@ -41,6 +41,8 @@ constexpr uint32_t TINY = 128;               // buckets ranked by plain comparis
 constexpr uint32_t SMALL = 1024;             // buckets sorted in LDS (runs of 64, then a merge rank)
 constexpr uint32_t RUN = WAVE;               // items per register-sorted run
 constexpr uint32_t SCAN_B = WAVE;            // buckets per scan block
+constexpr uint32_t SCAN_T = 1024;            // threads per scan block
+constexpr uint32_t SCAN_G = SCAN_T / SCAN_B; // tile groups per scan block
 
 struct PackInfo {
     uint32_t N;          // row_num
@@ -130,14 +132,15 @@ __global__ __launch_bounds__(MT) void k_msd_rank(const gkey *__restrict__ ik, co
 
 // offs[t][b] = items in buckets < b + items of bucket b in tiles < t;
 // bstart[b] = offs[0][b], bstart[NB] = n. A block covers SCAN_B buckets with
-// four tile groups; blocks chain their bucket totals by look-back. With dn,
+// SCAN_G tile groups (each thread walks ntiles / SCAN_G tiles of one
+// bucket's column); blocks chain their bucket totals by look-back. With dn,
 // n = *dn and only its ceil(n / tile) tiles take part.
-__global__ __launch_bounds__(MT) void k_msd_scan(const uint32_t *__restrict__ counts, uint32_t ntiles, uint32_t tile,
+__global__ __launch_bounds__(SCAN_T) void k_msd_scan(const uint32_t *__restrict__ counts, uint32_t ntiles, uint32_t tile,
                                                  uint32_t NB, uint64_t n, const uint32_t *__restrict__ dn,
                                                  uint32_t *__restrict__ offs, uint32_t *__restrict__ bstart,
                                                  unsigned long long *status, uint32_t epoch,
                                                  uint32_t *__restrict__ ticket, uint32_t *__restrict__ err) {
-    __shared__ uint32_t gs[MW][SCAN_B];
+    __shared__ uint32_t gs[SCAN_G][SCAN_B];
     __shared__ uint32_t base_s[SCAN_B];
     __shared__ uint32_t s_blk;
     if (dn) {
@@ -149,7 +152,7 @@ __global__ __launch_bounds__(MT) void k_msd_scan(const uint32_t *__restrict__ co
     const uint32_t blk = s_blk;
     const uint32_t g = threadIdx.x / SCAN_B, bl = threadIdx.x % SCAN_B;
     const uint32_t bkt = blk * SCAN_B + bl;
-    const uint32_t tg = (ntiles + MW - 1) / MW;
+    const uint32_t tg = (ntiles + SCAN_G - 1) / SCAN_G;
     const uint32_t t0 = min(ntiles, g * tg), t1 = min(ntiles, t0 + tg);
     uint32_t sum = 0;
     if (bkt < NB) {
@@ -160,7 +163,7 @@ __global__ __launch_bounds__(MT) void k_msd_scan(const uint32_t *__restrict__ co
     __syncthreads();
     uint32_t gpre = 0, col = 0;
 #pragma unroll
-    for (uint32_t q = 0; q < MW; q++) {
+    for (uint32_t q = 0; q < SCAN_G; q++) {
         if (q < g) gpre += gs[q][bl];
         col += gs[q][bl];
     }
@@ -180,6 +183,7 @@ __global__ __launch_bounds__(MT) void k_msd_scan(const uint32_t *__restrict__ co
     if (bkt < NB) {
         uint32_t run = base_s[bl] + gpre;
         if (g == 0) bstart[bkt] = base_s[bl];
+#pragma unroll 8
         for (uint32_t t = t0; t < t1; t++) {
             const uint64_t at = (uint64_t)t * NB + bkt;
             const uint32_t c = counts[at];
@@ -432,7 +436,7 @@ void msd_sort(const gkey *ik, const gval *iv, gkey *ok, gval *ov, const MsdPlan 
 #define GPMA_MSD_TILE(IPT)                                                                                    \
     hipLaunchKernelGGL(k_msd_rank<IPT>, dim3(pl.ntiles), dim3(MT), 0, st, ik, iv, n, dn, pi, pl.bshift, pl.D, w.kp, \
                        w.rk, counts, invalid);                                                                    \
-    hipLaunchKernelGGL(k_msd_scan, dim3(pl.nscan), dim3(MT), 0, st, counts, pl.ntiles, MT * IPT, pl.NB, n, dn,     \
+    hipLaunchKernelGGL(k_msd_scan, dim3(pl.nscan), dim3(SCAN_T), 0, st, counts, pl.ntiles, MT * IPT, pl.NB, n, dn, \
                        offs, w.bstart, w.status, epoch, ticket, err);                                              \
     hipLaunchKernelGGL(k_msd_scatter<IPT>, dim3(pl.ntiles), dim3(MT), 0, st, w.kp, w.rk, n, dn, pl.bshift, pl.NB,   \
                        offs, w.kb)
