@@ -21,6 +21,13 @@ __device__ __forceinline__ bool is_live(gkey k, gval v) { return k != KEY_NONE &
 __device__ __forceinline__ bool is_guard(gkey k) { return (uint32_t)k == COL_NONE; }
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+// Orders one wave's LDS accesses before and after this point (the lanes of a
+// wave then see each other's LDS writes).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint64_t lanemask_lt() {
     uint32_t l = __lane_id();

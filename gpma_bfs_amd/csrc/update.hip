@@ -519,9 +519,113 @@ __device__ void window_one(uint32_t S, uint32_t W, const gkey *__restrict__ pk, 
     }
 }
 
+// window_one for W <= WAVE_WIN_MAX by one wave, without workgroup barriers:
+// the run length and the live-item compaction by ballots, lk/lv the wave's
+// own LDS rows. Same results as window_one.
+constexpr uint32_t WAVE_WIN_MAX = 256;
+__device__ void window_wave(uint32_t S, uint32_t W, const gkey *__restrict__ pk, const gval *__restrict__ pv,
+                            const uint32_t *__restrict__ pw, uint8_t *__restrict__ pd, uint64_t np, uint64_t i,
+                            uint32_t w, gkey *__restrict__ keys, gval *__restrict__ values, gkey *__restrict__ seps,
+                            uint32_t *__restrict__ row_offset, uint32_t lower, uint32_t upper, gkey *lk, gval *lv) {
+    const uint32_t lane = lane_id();
+    // run length: the first t <= upper whose entry i + t is not in window w
+    uint32_t r = upper + 1;
+    for (uint32_t t0 = 0; t0 <= upper; t0 += WAVE) {
+        const uint32_t t = t0 + lane;
+        const bool out = t <= upper && !((i + t < np) && pw[i + t] == w);
+        const uint64_t b = ballot(out);
+        if (b) {
+            r = t0 + (uint32_t)__ffsll((unsigned long long)b) - 1;
+            break;
+        }
+    }
+    if (r > upper) return;
+    uint32_t c = 0;
+    for (uint32_t b0 = 0; b0 < W; b0 += WAVE) {
+        const uint32_t p = b0 + lane;
+        gkey k = KEY_NONE;
+        gval v = 0;
+        bool live = false;
+        if (p < W) {
+            k = keys[(uint64_t)w + p];
+            v = values[(uint64_t)w + p];
+            live = is_live(k, v);
+        }
+        const uint64_t m = ballot(live);
+        if (live) {
+            const uint32_t pos = c + (uint32_t)__popcll(m & lanemask_lt());
+            lk[pos] = k;
+            lv[pos] = v;
+        }
+        c += (uint32_t)__popcll(m);
+    }
+    wave_sync();
+    const uint32_t m = c + r;
+    if (m >= lower && m <= upper) {
+        const uint64_t f = W / S;
+        for (uint32_t t = lane; t < c; t += WAVE) {
+            const gkey k = lk[t];
+            const uint64_t mi = t + lower_bound_g(pk + i, r, k);
+            const uint32_t slot = redistribute_slot(f, S, m, mi);
+            keys[(uint64_t)w + slot] = k;
+            values[(uint64_t)w + slot] = lv[t];
+            if (is_guard(k)) row_offset[(k >> 32) + 1] = w + slot;
+            if (slot % S == 0) seps[(w + slot) / S] = k;
+        }
+        for (uint32_t u = lane; u < r; u += WAVE) {
+            const gkey k = pk[i + u];
+            uint32_t lo = 0, hi = c;
+            while (lo < hi) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (lk[mid] < k) lo = mid + 1;
+                else hi = mid;
+            }
+            const uint64_t mi = u + lo;
+            const uint32_t slot = redistribute_slot(f, S, m, mi);
+            keys[(uint64_t)w + slot] = k;
+            values[(uint64_t)w + slot] = pv[i + u];
+            if (is_guard(k)) row_offset[(k >> 32) + 1] = w + slot;
+            if (slot % S == 0) seps[(w + slot) / S] = k;
+            pd[i + u] = 1;
+        }
+        for (uint32_t p = lane; p < W; p += WAVE) {
+            const uint64_t s = p / S, l = p % S;
+            const uint64_t a = seg_first_item(f, m, s);
+            const uint64_t cnt = seg_first_item(f, m, s + 1) - a;
+            if (l >= cnt) {
+                keys[(uint64_t)w + p] = KEY_NONE;
+                values[(uint64_t)w + p] = 0;
+                if (l == 0) seps[(w + p) / S] = KEY_NONE;
+            }
+        }
+    }
+    wave_sync();  // the wave's next window reuses lk / lv
+}
 
 // Workgroups stride over the pending list; each run head is one window.
 constexpr uint64_t WINDOW_GRID = 1024;
+
+// The same for W <= WAVE_WIN_MAX with one wave per window.
+__global__ __launch_bounds__(256) void k_window_wave(uint32_t S, uint32_t W, const gkey *__restrict__ pk,
+                                                     const gval *__restrict__ pv, const uint32_t *__restrict__ pw,
+                                                     uint8_t *__restrict__ pd, const uint32_t *__restrict__ d_np,
+                                                     gkey *__restrict__ keys, gval *__restrict__ values,
+                                                     gkey *__restrict__ seps, uint32_t *__restrict__ row_offset,
+                                                     uint32_t lower, uint32_t upper,
+                                                     const uint32_t *__restrict__ d_gate, uint32_t gate) {
+    constexpr uint32_t NW = 256 / WAVE;
+    __shared__ gkey lk[NW][WAVE_WIN_MAX];
+    __shared__ gval lv[NW][WAVE_WIN_MAX];
+    if (d_gate && *d_gate >= gate) return;
+    const uint64_t np = *d_np;
+    const uint32_t wid = threadIdx.x / WAVE;
+    const uint64_t nw = (uint64_t)gridDim.x * NW;
+    for (uint64_t i = (uint64_t)blockIdx.x * NW + wid; i < np; i += nw) {
+        const uint32_t w = pw[i];
+        if (i > 0 && pw[i - 1] == w) continue;
+        window_wave(S, W, pk, pv, pw, pd, np, i, w, keys, values, seps, row_offset, lower, upper, lk[wid], lv[wid]);
+    }
+}
 __global__ __launch_bounds__(256) void k_window(uint32_t S, uint32_t W, const gkey *__restrict__ pk,
                                                 const gval *__restrict__ pv, const uint32_t *__restrict__ pw,
                                                 uint8_t *__restrict__ pd, const uint32_t *__restrict__ d_np,
@@ -778,8 +882,13 @@ void launch_window(uint32_t S, uint32_t W, const gkey *pk, const gval *pv, const
                    uint32_t *row_offset, uint32_t lower, uint32_t upper, const uint32_t *d_gate, uint32_t gate,
                    hipStream_t st) {
     if (!n_ub) return;
-    hipLaunchKernelGGL(k_window, dim3((uint32_t)std::min<uint64_t>(n_ub, WINDOW_GRID)), dim3(256), 0, st, S, W, pk, pv, pw,
-                       pd, d_np, keys, values, seps, row_offset, lower, upper, d_gate, gate);
+    const dim3 grid((uint32_t)std::min<uint64_t>(n_ub, WINDOW_GRID));
+    if (W <= WAVE_WIN_MAX)
+        hipLaunchKernelGGL(k_window_wave, grid, dim3(256), 0, st, S, W, pk, pv, pw, pd, d_np, keys, values, seps,
+                           row_offset, lower, upper, d_gate, gate);
+    else
+        hipLaunchKernelGGL(k_window, grid, dim3(256), 0, st, S, W, pk, pv, pw, pd, d_np, keys, values, seps,
+                           row_offset, lower, upper, d_gate, gate);
 }
 
 void launch_run_heads(const uint32_t *pw, uint64_t n_ub, const uint32_t *d_np, uint32_t *blkcnt, uint32_t *off,
