@@ -113,7 +113,7 @@ enum Ctr : uint32_t {
 constexpr uint32_t N_TICKETS = C_LVL - C_TICKET;
 constexpr uint32_t C_LVL_SLOTS = C_COUNT - C_LVL;
 // Bucket sort for batches up to this size (its count and offset matrices
-// take 8 to 32 bytes per record); larger batches and row counts >= 2^28 use
+// take 6 to 24 bytes per record); larger batches and row counts >= 2^28 use
 // the LSD passes.
 constexpr uint64_t MSD_MAX_N = 4ull << 20;
 constexpr uint32_t MSD_BIG_BUCKET = 16384;

@@ -47,7 +47,7 @@ struct MsdWork {
     uint64_t *kp;               // n packed keys
     uint16_t *rk;               // n ranks
     uint64_t *kb;               // n bucketed packed keys
-    uint32_t *mat;              // 2 * ntiles * NB counts and offsets
+    uint32_t *mat;              // ntiles * NB u16 counts, then ntiles * NB u32 offsets (2 * ntiles * NB words)
     uint32_t *bstart;           // NB + 1 bucket starts
     unsigned long long *status; // nscan look-back words (zeroed once)
 };

@@ -71,7 +71,7 @@ template <uint32_t IPT>
 __global__ __launch_bounds__(MT) void k_msd_rank(const gkey *__restrict__ ik, const gval *__restrict__ iv, uint64_t n,
                                                  const uint32_t *__restrict__ dn, PackInfo pi, uint32_t bshift,
                                                  uint32_t D, uint64_t *__restrict__ kp, uint16_t *__restrict__ rk,
-                                                 uint32_t *__restrict__ counts, uint32_t *__restrict__ invalid) {
+                                                 uint16_t *__restrict__ counts, uint32_t *__restrict__ invalid) {
     constexpr uint32_t TILE = MT * IPT, WI = WAVE * IPT;
     __shared__ uint16_t cnt[MW][MAX_NB];
     if (dn) n = *dn;
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(MT) void k_msd_rank(const gkey *__restrict__ ik, co
             cnt[w][d] = (uint16_t)run;
             run += c;
         }
-        counts[(uint64_t)blockIdx.x * NB + d] = run;
+        counts[(uint64_t)blockIdx.x * NB + d] = (uint16_t)run;  // a tile holds <= 4096 items
     }
     __syncthreads();
 #pragma unroll
@@ -135,7 +135,7 @@ __global__ __launch_bounds__(MT) void k_msd_rank(const gkey *__restrict__ ik, co
 // SCAN_G tile groups (each thread walks ntiles / SCAN_G tiles of one
 // bucket's column); blocks chain their bucket totals by look-back. With dn,
 // n = *dn and only its ceil(n / tile) tiles take part.
-__global__ __launch_bounds__(SCAN_T) void k_msd_scan(const uint32_t *__restrict__ counts, uint32_t ntiles, uint32_t tile,
+__global__ __launch_bounds__(SCAN_T) void k_msd_scan(const uint16_t *__restrict__ counts, uint32_t ntiles, uint32_t tile,
                                                  uint32_t NB, uint64_t n, const uint32_t *__restrict__ dn,
                                                  uint32_t *__restrict__ offs, uint32_t *__restrict__ bstart,
                                                  unsigned long long *status, uint32_t epoch,
@@ -412,7 +412,7 @@ MsdPlan msd_plan(uint64_t n, uint32_t N, uint32_t bucket_bits) {
     pl.bs = bs;
     const uint32_t B = 2 * bs;
     uint32_t D = 1;
-    while (D < MAX_D && (n >> (D + 6)) != 0) D++;  // about 64 items per bucket
+    while (D < MAX_D && (n >> (D + 7)) != 0) D++;  // about 128 items per bucket
     if (bucket_bits) D = std::min(bucket_bits, MAX_D);
     if (D > B) D = B;
     pl.D = D;
@@ -431,8 +431,8 @@ void msd_sort(const gkey *ik, const gval *iv, gkey *ok, gval *ov, const MsdPlan 
               uint32_t *err, uint32_t *maxb, hipStream_t st) {
     if (n == 0) return;
     const PackInfo pi{N, pl.bs, guards_ok ? 1u : 0u};
-    uint32_t *counts = w.mat;
-    uint32_t *offs = w.mat + (uint64_t)pl.ntiles * pl.NB;
+    uint16_t *counts = reinterpret_cast<uint16_t *>(w.mat);                        // [ntiles][NB] u16
+    uint32_t *offs = w.mat + ((uint64_t)pl.ntiles * pl.NB + 1) / 2;             // [ntiles][NB] u32
 #define GPMA_MSD_TILE(IPT)                                                                                    \
     hipLaunchKernelGGL(k_msd_rank<IPT>, dim3(pl.ntiles), dim3(MT), 0, st, ik, iv, n, dn, pi, pl.bshift, pl.D, w.kp, \
                        w.rk, counts, invalid);                                                                    \
