@@ -257,10 +257,14 @@ void phase_end(gpma_handle *g, int p, uint64_t launches) {
     g->ph_calls[p] += launches;
 }
 // After a stream synchronisation: fold finished event pairs into the totals.
+// Also called right after a completion signal, when the events have run on
+// the GPU but the runtime may not have seen them yet: then wait for them.
 void phase_collect(gpma_handle *g) {
     for (const auto &r : g->ev_done) {
         float ms = 0;
-        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) g->ph_ms[r.phase] += ms;
+        hipError_t e = hipEventElapsedTime(&ms, r.a, r.b);
+        if (e == hipErrorNotReady && hipEventSynchronize(r.b) == hipSuccess) e = hipEventElapsedTime(&ms, r.a, r.b);
+        if (e == hipSuccess) g->ph_ms[r.phase] += ms;
         g->ev_pool.push_back(r.a);
         g->ev_pool.push_back(r.b);
     }
@@ -270,13 +274,6 @@ void phase_collect(gpma_handle *g) {
 int stream_sync(gpma_handle *g) {
     HIP_TRY(hipStreamSynchronize(g->st));
     phase_collect(g);
-    return GPMA_OK;
-}
-
-int sync_read_ctr(gpma_handle *g) {
-    HIP_TRY(hipMemcpyAsync(g->hctr, g->ctr.p, C_COUNT * sizeof(uint32_t), hipMemcpyDeviceToHost, g->st));
-    TRY(stream_sync(g));
-    HIP_TRY(hipGetLastError());
     return GPMA_OK;
 }
 
@@ -318,15 +315,11 @@ int publish_and_wait(gpma_handle *g, const uint32_t *a, uint32_t na, const uint3
 }
 
 // Counters into hctr once the queued work is done, leaving them zero on the
-// device (the next update then needs no clearing launch). Phase timing needs
-// the stream synchronisation, so it keeps the copy path.
+// device (the next update then needs no clearing launch).
 int read_ctr(gpma_handle *g) {
-    if (g->prof) {
-        g->ctr_clean = false;
-        return sync_read_ctr(g);
-    }
     TRY(publish_and_wait(g, g->ctr.as<uint32_t>(), C_COUNT, nullptr, 0, g->hctr, true));
     g->ctr_clean = true;
+    if (g->prof) phase_collect(g);
     return GPMA_OK;
 }
 
@@ -680,8 +673,6 @@ int finish_update(gpma_handle *g, uint64_t n, uint32_t spec, bool used_msd) {
         // gpma.cuh:1034-1044: grow (or shrink) and rebalance the root
         TRY(rebuild(g, PK[pc]->as<gkey>(), PV[pc]->as<gval>(), np, true));
         TRY(stream_sync(g));
-    } else if (g->prof) {
-        TRY(stream_sync(g));  // fold the phase events
     }
     HIP_TRY(hipGetLastError());
     if (inval) return fail(GPMA_ERR_KEY_RANGE, std::to_string(inval) + " keys with src/dst >= row_num skipped");
@@ -773,18 +764,18 @@ int run_bfs(const gkey *keys, const gval *values, uint64_t C, const uint32_t *ro
         if (prof) phase_begin(prof, P_BFS);
         for (uint32_t j = 0; j < chunk; j++) launch_bfs_level(keys, values, C, N, row_offset, b, level + j, thr, st);
         if (prof) phase_end(prof, P_BFS, chunk);
-        if (prof && !prof->prof) {
+        if (prof) {
             uint32_t w[4 * BFS_MAX_CHUNK];
             TRY(publish_and_wait(prof, (const uint32_t *)(b.cnt + level + 1), 2 * chunk,
                                  (const uint32_t *)(b.cnt_big + level + 1), 2 * chunk, w));
             std::memcpy(hcnt, w, chunk * 8);
             std::memcpy(hcnt + BFS_MAX_CHUNK + 1, w + 2 * chunk, chunk * 8);
+            if (prof->prof) phase_collect(prof);
         } else {
             HIP_TRY(hipMemcpyAsync(hcnt, b.cnt + level + 1, chunk * 8, hipMemcpyDeviceToHost, st));
             HIP_TRY(hipMemcpyAsync(hcnt + BFS_MAX_CHUNK + 1, b.cnt_big + level + 1, chunk * 8, hipMemcpyDeviceToHost,
                                    st));
             HIP_TRY(hipStreamSynchronize(st));
-            if (prof) phase_collect(prof);
         }
         for (uint32_t j = 0; j < chunk && !done; j++) {
             const uint64_t c = (uint64_t)(uint32_t)hcnt[j] + (uint32_t)hcnt[BFS_MAX_CHUNK + 1 + j];
@@ -874,17 +865,13 @@ int msbfs_levels(gpma_handle *g, const MsState &m, uint32_t level, uint32_t *dep
             ms_level(g->keys.as<gkey>(), g->values.as<gval>(), g->C, N, g->row_offset.as<uint32_t>(), m, level + j,
                      thr, st);
         phase_end(g, P_BFS, 2 * chunk);
-        if (!g->prof) {
+        {
             uint32_t w[4 * BFS_MAX_CHUNK];
             TRY(publish_and_wait(g, (const uint32_t *)(m.cnt + level + 1), 2 * chunk,
                                  (const uint32_t *)(m.cnt_big + level + 1), 2 * chunk, w));
             std::memcpy(g->ms_hcnt, w, chunk * 8);
             std::memcpy(g->ms_hcnt + BFS_MAX_CHUNK + 1, w + 2 * chunk, chunk * 8);
-        } else {
-            HIP_TRY(hipMemcpyAsync(g->ms_hcnt, m.cnt + level + 1, chunk * 8, hipMemcpyDeviceToHost, st));
-            HIP_TRY(hipMemcpyAsync(g->ms_hcnt + BFS_MAX_CHUNK + 1, m.cnt_big + level + 1, chunk * 8,
-                                   hipMemcpyDeviceToHost, st));
-            TRY(stream_sync(g));
+            if (g->prof) phase_collect(g);
         }
         for (uint32_t j = 0; j < chunk && !done; j++) {
             *depth = level + j;
