@@ -152,7 +152,7 @@ struct gpma_handle {
     uint32_t cc_epoch = 0;   // tag of the latest compaction call (never 0)
     uint32_t cc_ticket = 0;  // next ticket slot of this update
     // bfs scratch
-    Buf lev, fr0, fr1, fb0, fb1, vis, bcnt, bcntb;
+    Buf lev, fr0, fr1, fb0, fb1, bcnt, bcntb;
     // multi-source bfs scratch
     Buf ms_seen, ms_fr0, ms_fr1, ms_nx, ms_lv16, ms_q0, ms_q1, ms_qb0, ms_qb1, ms_touched, ms_cnt, ms_cntb, ms_tcnt,
         ms_partial, ms_starts, ms_out, ms_reach;
@@ -725,7 +725,6 @@ int ensure_bfs(gpma_handle *g) {
     TRY(g->fr1.ensure(n * 4, t));
     TRY(g->fb0.ensure(n * 4, t));
     TRY(g->fb1.ensure(n * 4, t));
-    TRY(g->vis.ensure((n / 32 + 1) * 4, t));
     TRY(g->bcnt.ensure((n + BFS_MAX_CHUNK + 2) * 8, t));
     TRY(g->bcntb.ensure((n + BFS_MAX_CHUNK + 2) * 8, t));
     if (!g->hcnt) HIP_TRY(hipHostMalloc((void **)&g->hcnt, 2 * (BFS_MAX_CHUNK + 1) * 8, hipHostMallocDefault));
@@ -735,7 +734,6 @@ int ensure_bfs(gpma_handle *g) {
 BfsState bfs_state(gpma_handle *g) {
     BfsState b;
     b.levels = g->lev.as<uint32_t>();
-    b.vis = g->vis.as<uint32_t>();
     b.q[0] = g->fr0.as<uint32_t>();
     b.q[1] = g->fr1.as<uint32_t>();
     b.qbig[0] = g->fb0.as<uint32_t>();
@@ -1278,7 +1276,7 @@ int gpma_destroy(gpma_t g) {
                   &g->sk1, &g->sv1, &g->win, &g->blkcnt, &g->blkoff, &g->sortwork, &g->msd_kp, &g->msd_rk,
                   &g->msd_mat, &g->msd_bstart, &g->msd_status, &g->ccstat, &g->pk0, &g->pv0,
                   &g->pw0, &g->pd0, &g->pk1, &g->pv1, &g->pw1, &g->pd1, &g->tmpk, &g->tmpv, &g->chunkcnt,
-                  &g->chunkoff, &g->lev, &g->fr0, &g->fr1, &g->fb0, &g->fb1, &g->vis, &g->bcnt, &g->bcntb,
+                  &g->chunkoff, &g->lev, &g->fr0, &g->fr1, &g->fb0, &g->fb1, &g->bcnt, &g->bcntb,
                   &g->ms_seen, &g->ms_fr0, &g->ms_fr1, &g->ms_nx, &g->ms_lv16, &g->ms_q0, &g->ms_q1, &g->ms_qb0,
                   &g->ms_qb1, &g->ms_touched, &g->ms_cnt, &g->ms_cntb, &g->ms_tcnt, &g->ms_partial, &g->ms_starts,
                   &g->ms_out, &g->ms_reach, &g->ctr, &g->step_n};
@@ -1403,7 +1401,7 @@ int gpma_csr_bfs(const uint64_t *keys, const uint8_t *values, const uint32_t *ro
     HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     const uint64_t n = (uint64_t)node_size + 1;
     const uint64_t cw = n + BFS_MAX_CHUNK + 2;
-    const uint64_t words = 5 * n + (n / 32 + 1) + 4 * cw + 2;
+    const uint64_t words = 5 * n + 4 * cw + 2;
     uint32_t *buf = nullptr;
     unsigned long long *hc = nullptr;
     int rc = GPMA_OK;
@@ -1423,8 +1421,6 @@ int gpma_csr_bfs(const uint64_t *keys, const uint8_t *values, const uint32_t *ro
         p += n;
         b.qbig[1] = p;
         p += n;
-        b.vis = p;
-        p += n / 32 + 1;
         if ((uintptr_t)p & 7) p++;
         b.cnt = (unsigned long long *)p;
         p += 2 * cw;

@@ -138,15 +138,16 @@ void launch_write_compacted(const gkey *tk, const gval *tv, uint64_t c, gkey *ke
                             gkey *seps, uint32_t *row_offset, uint32_t S, hipStream_t st);
 
 // ---- bfs.hip
-// Device state of one BFS: levels[N], visited bitmap, two frontier queues per
-// parity (short ranges / long ranges) and per-level counts and slot work.
+// Device state of one BFS: levels[N] (0 = not reached; also the claim
+// words), two frontier queues per parity (short ranges / long ranges) and
+// per-level counts and slot work.
 struct BfsState {
-    uint32_t *levels, *vis;
+    uint32_t *levels;
     uint32_t *q[2], *qbig[2];
     // per level: low 32 bits = queued vertices, high 32 bits = their slot count
     unsigned long long *cnt, *cnt_big;
 };
-// levels = 0 except start (1); bitmap cleared except start; start queued for level 1.
+// levels = 0 except start (1); start queued for level 1.
 void launch_bfs_init(const BfsState &b, uint32_t N, uint32_t start, const uint32_t *row_offset, hipStream_t st);
 void launch_bfs_clear(const BfsState &b, uint32_t a, uint32_t e, hipStream_t st);
 // Expand level `level` into level+1: push when the level's slot count is at
