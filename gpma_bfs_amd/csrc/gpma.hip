@@ -490,7 +490,7 @@ void compact_step(gpma_handle *g, const gkey *ik, const gval *iv, const uint32_t
 int ensure_msd(gpma_handle *g, uint64_t n, const MsdPlan &pl) {
     uint64_t *t = &g->dev_bytes;
     TRY(g->msd_kp.ensure(n * 8, t));
-    TRY(g->msd_rk.ensure(n * 2, t));
+    TRY(g->msd_rk.ensure(n * 4, t));
     TRY(g->msd_mat.ensure(2ull * pl.ntiles * pl.NB * 4, t));
     TRY(g->msd_bstart.ensure(((uint64_t)pl.NB + 1) * 4, t));
     void *before = g->msd_status.p;
@@ -513,10 +513,11 @@ uint32_t enqueue_update(gpma_handle *g, const gkey *src_k, const gval *src_v, ui
     phase_begin(g, P_SORT);
     int cur = 0;
     if (pl) {
-        MsdWork w{g->msd_kp.as<uint64_t>(), g->msd_rk.as<uint16_t>(), g->sk1.as<uint64_t>(), g->msd_mat.as<uint32_t>(),
+        MsdWork w{g->msd_kp.as<uint64_t>(), g->msd_rk.as<uint32_t>(), g->sk1.as<uint64_t>(), g->msd_mat.as<uint32_t>(),
                   g->msd_bstart.as<uint32_t>(), g->msd_status.as<unsigned long long>()};
         msd_sort(src_k, src_v, g->sk0.as<gkey>(), g->sv0.as<gval>(), *pl, w, n_ub, d_n, g->N, guards_ok,
-                 ctr + C_INVALID, next_epoch(g), ctr + C_MSD_TICKET, ctr + C_SORTERR, ctr + C_MSD_MAXB, st);
+                 g->row_offset.as<uint32_t>(), g->C, ctr + C_INVALID, next_epoch(g), ctr + C_MSD_TICKET,
+                 ctr + C_SORTERR, ctr + C_MSD_MAXB, st);
         phase_end(g, P_SORT, 4);
     } else {
         // digits: bits of N-1 plus one (so KEY_NONE sorts after every valid key)
@@ -697,7 +698,7 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
     if (g->msd_cool) g->msd_cool--;
     MsdPlan pl{};
     if (use_msd) {
-        pl = msd_plan(n, g->N, g->sort_bits);
+        pl = msd_plan(n, g->N, g->sort_bits, !guards_ok);
         TRY(ensure_msd(g, n, pl));
     }
     if (!g->ctr_clean) fill_u32(g->ctr.as<uint32_t>(), 0, C_COUNT, st);
@@ -991,7 +992,7 @@ int step_graph(gpma_handle *g, uint32_t cls, uint32_t nsrc, uint32_t *dlev, gpma
                uint32_t **out, uint32_t *blevels) {
     const uint64_t cap = 1ull << cls;
     TRY(ensure_update_buffers(g, cap));
-    const MsdPlan pl = msd_plan(cap, g->N, g->sort_bits);
+    const MsdPlan pl = msd_plan(cap, g->N, g->sort_bits, true);
     TRY(ensure_msd(g, cap, pl));
     TRY(g->step_n.ensure(256, &g->dev_bytes));
     if (!g->step_h)
@@ -1368,7 +1369,7 @@ int gpma_step_prepare(gpma_t g, uint64_t max_records, uint32_t nsrc, uint32_t *l
     uint32_t top_cls = STEP_MIN_CLASS;
     while ((1ull << top_cls) < top) top_cls++;
     TRY(ensure_update_buffers(g, 1ull << top_cls));
-    TRY(ensure_msd(g, 1ull << top_cls, msd_plan(1ull << top_cls, g->N, g->sort_bits)));
+    TRY(ensure_msd(g, 1ull << top_cls, msd_plan(1ull << top_cls, g->N, g->sort_bits, true)));
     if (nsrc) {
         TRY(ensure_msbfs(g));
         if (!levels) TRY(g->ms_out.ensure((uint64_t)nsrc * g->N * 4, &g->dev_bytes));

@@ -35,6 +35,7 @@ bool msd_sort_fits(uint32_t N);
 struct MsdPlan {
     uint32_t bs;      // bits per packed src / dst field
     uint32_t D;       // bucket bits
+    bool by_row;      // buckets from row_offset (the row's slot position), not the key's top bits
     uint32_t R;       // bits sorted inside a bucket
     uint32_t bshift;  // shift of the bucket bits in a packed key
     uint32_t ipt;     // items per lane of the rank and scatter tiles (4, 8 or 16)
@@ -42,10 +43,13 @@ struct MsdPlan {
     uint32_t NB;      // buckets
     uint32_t nscan;   // scan blocks (status words used)
 };
-MsdPlan msd_plan(uint64_t n, uint32_t N, uint32_t bucket_bits);  // bucket_bits 0: from n
+// bucket_bits 0: from n. by_row: buckets by the row's slot position
+// (row_offset valid, capacity C), balanced for skewed sources; else by the
+// key's top bits (the guard insertion, before row_offset exists).
+MsdPlan msd_plan(uint64_t n, uint32_t N, uint32_t bucket_bits, bool by_row);
 struct MsdWork {
     uint64_t *kp;               // n packed keys
-    uint16_t *rk;               // n ranks
+    uint32_t *rk;               // n (bucket << 16 | rank in (tile, bucket))
     uint64_t *kb;               // n bucketed packed keys
     uint32_t *mat;              // ntiles * NB u16 counts, then ntiles * NB u32 offsets (2 * ntiles * NB words)
     uint32_t *bstart;           // NB + 1 bucket starts
@@ -57,8 +61,8 @@ struct MsdWork {
 // items. dn (optional): the batch size is read from *dn on the device, and
 // n and the plan are an upper bound of it (launch shapes fixed in advance).
 void msd_sort(const gkey *ik, const gval *iv, gkey *ok, gval *ov, const MsdPlan &pl, const MsdWork &w, uint64_t n,
-              const uint32_t *dn, uint32_t N, bool guards_ok, uint32_t *invalid, uint32_t epoch, uint32_t *ticket,
-              uint32_t *err, uint32_t *maxb, hipStream_t st);
+              const uint32_t *dn, uint32_t N, bool guards_ok, const uint32_t *row_offset, uint64_t C,
+              uint32_t *invalid, uint32_t epoch, uint32_t *ticket, uint32_t *err, uint32_t *maxb, hipStream_t st);
 
 // ---- update.hip
 constexpr uint32_t CMP_BLOCK = 256;  // items per compaction block

@@ -66,11 +66,16 @@ __device__ __forceinline__ void unpack_key(uint64_t p, const PackInfo &pi, gkey 
 }
 
 // Tiles of MT * IPT items (IPT items per lane): the plan picks IPT from the
-// batch size so that small batches still spread over every CU.
+// batch size so that small batches still spread over every CU. The bucket of
+// an item is the top D bits of its packed key, or, with ro (row buckets), its
+// row's slot position in the array scaled to D bits (ro[src] >> rs, or << -rs),
+// which follows the key order as well and stays balanced when the sources are
+// skewed (R-MAT). rk[i] = bucket << 16 | rank inside (tile, bucket).
 template <uint32_t IPT>
 __global__ __launch_bounds__(MT) void k_msd_rank(const gkey *__restrict__ ik, const gval *__restrict__ iv, uint64_t n,
                                                  const uint32_t *__restrict__ dn, PackInfo pi, uint32_t bshift,
-                                                 uint32_t D, uint64_t *__restrict__ kp, uint16_t *__restrict__ rk,
+                                                 uint32_t D, const uint32_t *__restrict__ ro, int32_t rs,
+                                                 uint64_t *__restrict__ kp, uint32_t *__restrict__ rk,
                                                  uint16_t *__restrict__ counts, uint32_t *__restrict__ invalid) {
     constexpr uint32_t TILE = MT * IPT, WI = WAVE * IPT;
     __shared__ uint16_t cnt[MW][MAX_NB];
@@ -99,7 +104,14 @@ __global__ __launch_bounds__(MT) void k_msd_rank(const gkey *__restrict__ ik, co
         b[j] = NB;  // no item
         if (idx < n) {
             p[j] = pack_key(k[j], v[j], pi, bad);
-            b[j] = (uint32_t)(p[j] >> bshift) & (NB - 1);
+            if (!ro) {
+                b[j] = (uint32_t)(p[j] >> bshift) & (NB - 1);
+            } else if (bad) {
+                b[j] = NB - 1;  // packs as all ones: after every valid key
+            } else {
+                const uint32_t pos = ro[(uint32_t)(k[j] >> 32)];
+                b[j] = min(NB - 1, rs >= 0 ? pos >> rs : pos << -rs);
+            }
         }
         nbad += (uint32_t)__popcll(ballot(bad));
     }
@@ -125,7 +137,7 @@ __global__ __launch_bounds__(MT) void k_msd_rank(const gkey *__restrict__ ik, co
         const uint64_t idx = wbase + (uint64_t)j * WAVE + lane;
         if (idx < n) {
             kp[idx] = p[j];
-            rk[idx] = (uint16_t)(cnt[wid][b[j]] + r[j]);
+            rk[idx] = (b[j] << 16) | (cnt[wid][b[j]] + r[j]);
         }
     }
 }
@@ -195,10 +207,9 @@ __global__ __launch_bounds__(SCAN_T) void k_msd_scan(const uint16_t *__restrict_
 }
 
 template <uint32_t IPT>
-__global__ __launch_bounds__(MT) void k_msd_scatter(const uint64_t *__restrict__ kp, const uint16_t *__restrict__ rk,
-                                                    uint64_t n, const uint32_t *__restrict__ dn, uint32_t bshift,
-                                                    uint32_t NB, const uint32_t *__restrict__ offs,
-                                                    uint64_t *__restrict__ kb) {
+__global__ __launch_bounds__(MT) void k_msd_scatter(const uint64_t *__restrict__ kp, const uint32_t *__restrict__ rk,
+                                                    uint64_t n, const uint32_t *__restrict__ dn, uint32_t NB,
+                                                    const uint32_t *__restrict__ offs, uint64_t *__restrict__ kb) {
     if (dn) n = *dn;
     const uint64_t base = (uint64_t)blockIdx.x * (MT * IPT);
     if (base >= n) return;
@@ -214,7 +225,7 @@ __global__ __launch_bounds__(MT) void k_msd_scatter(const uint64_t *__restrict__
 #pragma unroll
     for (uint32_t j = 0; j < IPT; j++) {
         const uint64_t idx = base + (uint64_t)j * MT + threadIdx.x;
-        if (idx < n) kb[(uint64_t)o[(uint32_t)(p[j] >> bshift) & (NB - 1)] + r[j]] = p[j];
+        if (idx < n) kb[(uint64_t)o[r[j] >> 16] + (r[j] & 0xFFFFu)] = p[j];
     }
 }
 
@@ -405,7 +416,7 @@ __global__ __launch_bounds__(MT) void k_msd_local(uint64_t *__restrict__ kb, uin
 
 bool msd_sort_fits(uint32_t N) { return N > 0 && N < (1u << 28); }
 
-MsdPlan msd_plan(uint64_t n, uint32_t N, uint32_t bucket_bits) {
+MsdPlan msd_plan(uint64_t n, uint32_t N, uint32_t bucket_bits, bool by_row) {
     MsdPlan pl{};
     uint32_t bs = 0;
     while (bs < 32 && (N >> bs) != 0) bs++;  // bit width of N itself
@@ -416,7 +427,8 @@ MsdPlan msd_plan(uint64_t n, uint32_t N, uint32_t bucket_bits) {
     if (bucket_bits) D = std::min(bucket_bits, MAX_D);
     if (D > B) D = B;
     pl.D = D;
-    pl.R = B - D;
+    pl.by_row = by_row;
+    pl.R = by_row ? B : B - D;  // row buckets do not fix the key's top bits
     pl.bshift = 8 + B - D;
     // items per lane: enough tiles (about 400) to occupy every CU
     pl.ipt = n >= 400ull * MT * 16 ? 16 : n >= 400ull * MT * 8 ? 8 : 4;
@@ -427,19 +439,22 @@ MsdPlan msd_plan(uint64_t n, uint32_t N, uint32_t bucket_bits) {
 }
 
 void msd_sort(const gkey *ik, const gval *iv, gkey *ok, gval *ov, const MsdPlan &pl, const MsdWork &w, uint64_t n,
-              const uint32_t *dn, uint32_t N, bool guards_ok, uint32_t *invalid, uint32_t epoch, uint32_t *ticket,
-              uint32_t *err, uint32_t *maxb, hipStream_t st) {
+              const uint32_t *dn, uint32_t N, bool guards_ok, const uint32_t *row_offset, uint64_t C,
+              uint32_t *invalid, uint32_t epoch, uint32_t *ticket, uint32_t *err, uint32_t *maxb, hipStream_t st) {
     if (n == 0) return;
+    const uint32_t *ro = pl.by_row ? row_offset : nullptr;
+    int32_t cs = 0;
+    while ((1ull << (cs + 1)) <= C) cs++;  // C is a power of two
+    const int32_t rs = cs - (int32_t)pl.D;
     const PackInfo pi{N, pl.bs, guards_ok ? 1u : 0u};
     uint16_t *counts = reinterpret_cast<uint16_t *>(w.mat);                        // [ntiles][NB] u16
     uint32_t *offs = w.mat + ((uint64_t)pl.ntiles * pl.NB + 1) / 2;             // [ntiles][NB] u32
 #define GPMA_MSD_TILE(IPT)                                                                                    \
-    hipLaunchKernelGGL(k_msd_rank<IPT>, dim3(pl.ntiles), dim3(MT), 0, st, ik, iv, n, dn, pi, pl.bshift, pl.D, w.kp, \
-                       w.rk, counts, invalid);                                                                    \
+    hipLaunchKernelGGL(k_msd_rank<IPT>, dim3(pl.ntiles), dim3(MT), 0, st, ik, iv, n, dn, pi, pl.bshift, pl.D, ro, rs, \
+                       w.kp, w.rk, counts, invalid);                                                              \
     hipLaunchKernelGGL(k_msd_scan, dim3(pl.nscan), dim3(SCAN_T), 0, st, counts, pl.ntiles, MT * IPT, pl.NB, n, dn, \
                        offs, w.bstart, w.status, epoch, ticket, err);                                              \
-    hipLaunchKernelGGL(k_msd_scatter<IPT>, dim3(pl.ntiles), dim3(MT), 0, st, w.kp, w.rk, n, dn, pl.bshift, pl.NB,   \
-                       offs, w.kb)
+    hipLaunchKernelGGL(k_msd_scatter<IPT>, dim3(pl.ntiles), dim3(MT), 0, st, w.kp, w.rk, n, dn, pl.NB, offs, w.kb)
     switch (pl.ipt) {
     case 4: GPMA_MSD_TILE(4); break;
     case 8: GPMA_MSD_TILE(8); break;
