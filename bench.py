@@ -284,6 +284,40 @@ def run_batched(G, N, keys, half, steps, warmup, nsrc=64, fused=8, check=True):
     return out
 
 
+def cpu_gpma_baseline(N, keys, half, step, slides):
+    """The reference's GPMA CPU path (its DEV == CPU branch of update_gpma,
+    restated in oracle/, OpenMP over updates, windows and the batch sort) on
+    every host core: guard init, the half window (bulk path), then `slides`
+    protocol slides timed one by one. The live set after them is checked
+    against the closed form."""
+    from oracle import oracle as O
+
+    a = time.perf_counter()
+    o = O.OracleGPMA(N)
+    t_guard = time.perf_counter() - a
+    a = time.perf_counter()
+    o.update(keys[:half], np.ones(half, np.uint8))
+    t_window = time.perf_counter() - a
+    vals = np.concatenate([np.ones(step, np.uint8), np.zeros(step, np.uint8)])
+    ts = []
+    for i in range(slides):
+        b = np.concatenate([keys[half + i * step: half + (i + 1) * step], keys[i * step: (i + 1) * step]])
+        a = time.perf_counter()
+        o.update(b, vals)
+        ts.append(time.perf_counter() - a)
+    guards = (np.arange(N, dtype=np.uint64) << np.uint64(32)) | np.uint64(0xFFFFFFFF)
+    expect = np.sort(np.concatenate([keys[slides * step: half + slides * step], guards,
+                                     np.array([0xFFFFFFFFFFFFFFFE] * 2, np.uint64)]))
+    ok = bool(np.array_equal(o.live(), expect))
+    o.close()
+    return {"value": slides * 2 * step / sum(ts), "unit": "edge-updates/s", "cores": O.threads(), "kind": "port",
+            "sample": f"the reference's GPMA CPU path restated in oracle/ (OpenMP), same edge list: guard init, the "
+                      f"half window, {slides} slides of {step} inserts + {step} deletes; value = slide updates / "
+                      f"slide time",
+            "guard_init_s": t_guard, "window_s": t_window, "slide_ms": [t * 1e3 for t in ts],
+            "live_set_closed_form": ok, "cpu_model": cpu_model()}
+
+
 def live_checksum(keys):
     """(count, wrapping sum, xor) of a u64 key set: a size-independent parity
     check that needs no sort."""
@@ -807,6 +841,14 @@ def main():
             log(f"cpu_baseline: {cpu_ups / 1e6:.3f} M edge-updates/s (1 core), bfs {tm['bfs1_s'] * 1e3:.1f} ms")
         except Exception as e:  # report, do not fail the GPU measurement
             log(f"cpu_baseline skipped: {e}")
+    cpu_gpma = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu_gpma = cpu_gpma_baseline(N, keys, half, step, args.cpu_slides)
+            log(f"cpu_gpma_baseline: {cpu_gpma['value'] / 1e6:.3f} M edge-updates/s ({cpu_gpma['cores']} threads), "
+                f"live set ok {cpu_gpma['live_set_closed_form']}")
+        except Exception as e:
+            log(f"cpu_gpma_baseline skipped: {e}")
 
     out = {
         "metric": METRIC,
@@ -819,6 +861,7 @@ def main():
         "roofline": roofline,
         "step_roofline": step_roofline,
         "cpu_baseline": cpu,
+        "cpu_gpma_baseline": cpu_gpma,
         "bfs": bfs_info,
         "tuned_plain": tuned_plain,
         "batched": batched,

@@ -25,9 +25,14 @@
 //     would fail after a resize grows the array instead (the product's
 //     behaviour); by default the pending inserts are dropped exactly as
 //     gpma.cuh:1034-1044 does.
-//   * duplicate keys inside one batch are located sequentially in sorted
-//     order (the reference runs them in parallel, with a write race on the
-//     value byte). Workloads contain no intra-batch duplicates.
+//   * duplicate keys inside one batch: the locate loop runs in parallel like
+//     the reference's, but only the last record of a key writes its value
+//     byte, the result of a sequential pass (the reference races on that
+//     byte). Workloads contain no intra-batch duplicates.
+//
+// Built with OpenMP (oracle/build.py), the locate loop, the per-window
+// rebalance loop and the batch sort run on every host core, as the
+// reference's CPU path does; results do not depend on the thread count.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -37,8 +42,25 @@
 #include <numeric>
 #include <queue>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#include <parallel/algorithm>
+#endif
 
 namespace {
+
+// Stable sort of an index permutation (the batch sort, thrust::sort_by_key
+// in the reference): the libstdc++ parallel stable sort when built with
+// OpenMP (same result: it is stable), else std::stable_sort.
+template <typename Less>
+void stable_sort_order(std::vector<size_t> &order, Less less) {
+#ifdef _OPENMP
+    __gnu_parallel::stable_sort(order.begin(), order.end(), less);
+#else
+    std::stable_sort(order.begin(), order.end(), less);
+#endif
+}
+
 
 bool g_fix_root = false;
 
@@ -115,9 +137,13 @@ inline void redispatch_one(size_t i, const key_t_ *tk, const val_t *tv, key_t_ *
     }
 }
 
-// gpma.cuh:188-240 (locate_leaf_loop_body + handle_del_mod), sequential.
+// gpma.cuh:188-240 (locate_leaf_loop_body + handle_del_mod). Parallel over
+// the updates like the reference's OpenMP loop (gpma.cuh:234); the input is
+// sorted, and of several records of one key only the last writes the value,
+// which is the write a sequential pass leaves (locate reads no values).
 void locate_leaf_batch(Gpma &g, const key_t_ *uk, const val_t *uv, size_type n, size_type *leaf) {
     const size_type S = g.segment_length;
+#pragma omp parallel for schedule(static)
     for (size_type i = 0; i < n; i++) {
         key_t_ key = uk[i];
         val_t value = uv[i];
@@ -129,9 +155,10 @@ void locate_leaf_batch(Gpma &g, const key_t_ *uk, const val_t *uv, size_type n, 
         }
         size_type out = prefix;
         if (value == VALUE_NONE) out = SIZE_NONE;
+        const bool last_of_key = i + 1 == n || uk[i + 1] != key;
         for (size_type j = 0; j < S; j++) {
             if (g.keys[prefix + j] == key) {
-                g.values[prefix + j] = value;
+                if (last_of_key) g.values[prefix + j] = value;
                 out = SIZE_NONE;
                 break;
             }
@@ -188,15 +215,21 @@ void rebalance_batch(Gpma &g, size_type level, std::vector<size_type> &nodes, co
                      const std::vector<size_type> &offset, size_type nuniq, size_type lower, size_type upper) {
     const size_type S = g.segment_length;
     const size_type width = S << level;
-    std::vector<key_t_> tk(width);
-    std::vector<val_t> tv(width);
-    std::vector<size_t> order;
     if (g.level_windows.size() <= level) {
         g.level_windows.resize(level + 1, 0);
         g.level_success.resize(level + 1, 0);
         g.level_slots.resize(level + 1, 0);
     }
+    g.level_windows[level] += nuniq;
+    g.level_slots[level] += (uint64_t)nuniq * width;
+    uint64_t success = 0;
+    // windows are disjoint: parallel like the reference's OpenMP loop
+    // (gpma.cuh:669-678)
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : success)
     for (size_type u = 0; u < nuniq; u++) {
+        std::vector<key_t_> tk(width);
+        std::vector<val_t> tv(width);
+        std::vector<size_t> order;
         size_type base = uniq[u];
         key_t_ *key = g.keys.data() + base;
         val_t *value = g.values.data() + base;
@@ -211,10 +244,8 @@ void rebalance_batch(Gpma &g, size_type level, std::vector<size_type> &nodes, co
         size_type a = offset[u], b = offset[u + 1];
         size_type interval = b - a;
         size_type merge = compacted + interval;
-        g.level_windows[level]++;
-        g.level_slots[level] += width;
         if (lower <= merge && merge <= upper) {
-            g.level_success[level]++;
+            success++;
             for (size_type i = 0; i < interval; i++) {
                 tk[compacted + i] = uk[a + i];
                 tv[compacted + i] = uv[a + i];
@@ -235,6 +266,7 @@ void rebalance_batch(Gpma &g, size_type level, std::vector<size_type> &nodes, co
                 redispatch_one(i, sk.data(), sv.data(), key, value, width, S, merge, g.row_offset.data(), base);
         }
     }
+    g.level_success[level] += success;
 }
 
 // gpma.cuh:881-904
@@ -282,7 +314,7 @@ void significant_insert(Gpma &g, const std::vector<key_t_> &uk, const std::vecto
     size_type merge = valid + n;
     std::vector<size_t> order(merge);
     std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return g.keys[x] < g.keys[y]; });
+    stable_sort_order(order, [&](size_t x, size_t y) { return g.keys[x] < g.keys[y]; });
     std::vector<key_t_> sk(merge);
     std::vector<val_t> sv(merge);
     for (size_type i = 0; i < merge; i++) {
@@ -305,7 +337,7 @@ void update_gpma(Gpma &g, std::vector<key_t_> uk, std::vector<val_t> uv) {
     {
         std::vector<size_t> order(n);
         std::iota(order.begin(), order.end(), 0);
-        std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return uk[x] < uk[y]; });
+        stable_sort_order(order, [&](size_t x, size_t y) { return uk[x] < uk[y]; });
         std::vector<key_t_> k2(n);
         std::vector<val_t> v2(n);
         for (size_type i = 0; i < n; i++) {
@@ -390,6 +422,15 @@ Gpma *create(size_type row_num) {
 extern "C" {
 
 void *orc_create(uint32_t row_num) { return create(row_num); }
+
+// Threads the parallel loops use (1 without OpenMP).
+int orc_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 void orc_set_fix_root(int on) { g_fix_root = on != 0; }
 void orc_destroy(void *h) { delete (Gpma *)h; }
 

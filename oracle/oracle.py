@@ -46,8 +46,14 @@ def lib():
         L.orc_fls.argtypes = [ctypes.c_uint32]
         L.orc_fls.restype = ctypes.c_uint32
         L.orc_density_table.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _P, _P]
+        L.orc_threads.restype = ctypes.c_int
         _lib = L
     return _lib
+
+
+def threads():
+    """Threads the restatement's parallel loops use (OMP_NUM_THREADS)."""
+    return int(lib().orc_threads())
 
 
 class OracleGPMA:
