@@ -195,6 +195,7 @@ struct gpma_handle {
     uint32_t step_blevels = 8;  // BFS levels a new capture holds (grows to the depth seen)
     uint64_t step_clock = 0;
     Buf step_n;                    // device u32: records of the step being replayed
+    Buf step_stage;                // device: BFS counts and reachable counts gathered for the read-back
     unsigned long long *step_h = nullptr;  // pinned: BFS counts and reachable counts read back
     gpma_step_stats sst{};
 };
@@ -304,9 +305,13 @@ int wait_signal(gpma_handle *g, uint32_t seq) {
 // one-wave kernel writes them to host-coherent memory and raises a sequence
 // number the host spins on, which takes less time than a stream
 // synchronisation after a device-to-host copy.
+// A captured step's own publish writes this sequence word; the host clears
+// the word before each replay.
+constexpr uint32_t SIG_STEP = 0xFFFFFFFFu;
+
 int publish_and_wait(gpma_handle *g, const uint32_t *a, uint32_t na, const uint32_t *b, uint32_t nb, uint32_t *out,
                      bool clear_a = false) {
-    if (++g->sig_seq == 0) g->sig_seq = 1;
+    if (++g->sig_seq == 0 || g->sig_seq == SIG_STEP) g->sig_seq = 1;
     launch_publish(a, na, b, nb, g->sig, g->sig_seq, clear_a, g->st);
     HIP_TRY(hipGetLastError());
     TRY(wait_signal(g, g->sig_seq));
@@ -917,7 +922,8 @@ std::vector<uint64_t> step_signature(const gpma_handle *g, uint32_t cls, uint32_
                                      uint32_t ulevels, uint32_t blevels) {
     std::vector<uint64_t> v = {cls, nsrc, (uint64_t)(uintptr_t)out, ulevels, blevels, g->N, g->C, g->S, g->h,
                                g->bulk_threshold, g->sort_bits, (uint64_t)(g->pull_fraction * 1e9)};
-    const Buf *bufs[] = {&g->keys, &g->values, &g->seps, &g->row_offset, &g->ctr, &g->step_n, &g->in_k, &g->in_v,
+    const Buf *bufs[] = {&g->keys, &g->values, &g->seps, &g->row_offset, &g->ctr, &g->step_n, &g->step_stage,
+                         &g->in_k, &g->in_v,
                          &g->sk0, &g->sv0, &g->sk1, &g->win, &g->pk0, &g->pv0, &g->pw0, &g->pd0, &g->pk1, &g->pv1,
                          &g->pw1, &g->pd1, &g->ccstat, &g->msd_kp, &g->msd_rk, &g->msd_mat, &g->msd_bstart,
                          &g->msd_status, &g->ms_seen, &g->ms_fr0, &g->ms_fr1, &g->ms_nx, &g->ms_lv16, &g->ms_q0,
@@ -964,7 +970,11 @@ int capture_step(gpma_handle *g, uint32_t cls, const MsdPlan &pl, uint32_t uleve
             ms_level(g->keys.as<gkey>(), g->values.as<gval>(), g->C, g->N, g->row_offset.as<uint32_t>(), m, L, thr,
                      st);
         ms_output(m, g->N, nsrc, out, g->ms_reach.as<uint32_t>(), st);
+        launch_step_gather(m.cnt + blevels + 1, m.cnt_big + blevels + 1, g->ms_reach.as<uint32_t>(), nsrc,
+                           g->step_stage.as<uint32_t>(), st);
     }
+    // the read-back: counters (cleared on the device) and the BFS words, then the signal
+    launch_publish(ctr, C_COUNT, g->step_stage.as<uint32_t>(), nsrc ? 4 + nsrc : 0, g->sig, SIG_STEP, true, st);
     hipGraph_t graph = nullptr;
     const hipError_t ec = hipStreamEndCapture(st, &graph);
     g->capturing = false;
@@ -995,6 +1005,7 @@ int step_graph(gpma_handle *g, uint32_t cls, uint32_t nsrc, uint32_t *dlev, gpma
     const MsdPlan pl = msd_plan(cap, g->N, g->sort_bits, true);
     TRY(ensure_msd(g, cap, pl));
     TRY(g->step_n.ensure(256, &g->dev_bytes));
+    TRY(g->step_stage.ensure((4 + 64) * 4, &g->dev_bytes));
     if (!g->step_h)
         HIP_TRY(hipHostMalloc((void **)&g->step_h, (2 * STEP_MAX_BLEVELS + 4 + 64) * 8, hipHostMallocDefault));
     *out = nullptr;
@@ -1051,21 +1062,19 @@ int run_step(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint64
     fill_u32(g->step_n.as<uint32_t>(), (uint32_t)n, 1, st);
     if (nsrc) HIP_TRY(hipMemcpyAsync(g->ms_starts.p, starts, nsrc * 4, hipMemcpyHostToDevice, st));
     const uint32_t spec = sg->spec;
-    g->ctr_clean = false;
+    // the graph ends with its own publish: counters (left cleared on the
+    // device), the frontier counts after the last captured level and the
+    // reachable counts, then the step's sequence word
+    *(volatile uint32_t *)g->sig = 0;
+    std::atomic_thread_fence(std::memory_order_seq_cst);
     HIP_TRY(hipGraphLaunch(sg->exec, st));
     g->sst.replays++;
-    // one read-back: counters, the frontier counts after the last captured
-    // level, reachable counts
-    unsigned long long *hb = g->step_h;
-    HIP_TRY(hipMemcpyAsync(g->hctr, g->ctr.p, C_COUNT * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    if (nsrc) {
-        HIP_TRY(hipMemcpyAsync(hb, g->ms_cnt.as<unsigned long long>() + blevels + 1, 8, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipMemcpyAsync(hb + 1, g->ms_cntb.as<unsigned long long>() + blevels + 1, 8, hipMemcpyDeviceToHost,
-                               st));
-        HIP_TRY(hipMemcpyAsync(hb + 2, g->ms_reach.p, nsrc * 4, hipMemcpyDeviceToHost, st));
-    }
-    TRY(stream_sync(g));
+    TRY(wait_signal(g, SIG_STEP));
+    g->ctr_clean = true;
     HIP_TRY(hipGetLastError());
+    unsigned long long *hb = g->step_h;  // [0] cnt, [1] cnt_big, then the reachable counts
+    std::memcpy(g->hctr, g->sig + 1, C_COUNT * 4);
+    if (nsrc) std::memcpy(hb, g->sig + 1 + C_COUNT, (4 + nsrc) * 4);
     const bool gated = g->hctr[C_GATE] != 0;
     const int urc = finish_update(g, n, spec, true);
     if (urc != GPMA_OK && urc != GPMA_ERR_KEY_RANGE) return urc;
@@ -1097,7 +1106,7 @@ int run_step(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint64
     }
     if (levels && !levels_on_device)
         HIP_TRY(hipMemcpyAsync(levels, out, (uint64_t)nsrc * g->N * 4, hipMemcpyDeviceToHost, st));
-    TRY(stream_sync(g));
+    if (!reach_ready || (levels && !levels_on_device)) TRY(stream_sync(g));
     HIP_TRY(hipGetLastError());
     return urc;
 }
@@ -1280,7 +1289,7 @@ int gpma_destroy(gpma_t g) {
                   &g->chunkoff, &g->lev, &g->fr0, &g->fr1, &g->fb0, &g->fb1, &g->bcnt, &g->bcntb,
                   &g->ms_seen, &g->ms_fr0, &g->ms_fr1, &g->ms_nx, &g->ms_lv16, &g->ms_q0, &g->ms_q1, &g->ms_qb0,
                   &g->ms_qb1, &g->ms_touched, &g->ms_cnt, &g->ms_cntb, &g->ms_tcnt, &g->ms_partial, &g->ms_starts,
-                  &g->ms_out, &g->ms_reach, &g->ctr, &g->step_n};
+                  &g->ms_out, &g->ms_reach, &g->ctr, &g->step_n, &g->step_stage};
     drop_graphs(g);
     for (Buf *b : all) b->release(&g->dev_bytes);
     if (g->step_h) (void)hipHostFree(g->step_h);

@@ -81,6 +81,9 @@ void launch_locate(const gkey *ks, const gval *vs, uint64_t n, const uint32_t *d
 // system-scope release, after everything queued before it on the stream.
 void launch_publish(const uint32_t *a, uint32_t na, const uint32_t *b, uint32_t nb, uint32_t *dst, uint32_t seq,
                     bool clear_a, hipStream_t st);
+// stage[0..4) = *cnt, *cnt_big as u32 pairs; stage[4 .. 4 + n) = reach[0 .. n).
+void launch_step_gather(const unsigned long long *cnt, const unsigned long long *cnt_big, const uint32_t *reach,
+                        uint32_t n, uint32_t *stage, hipStream_t st);
 // *gate = 1 when the update before it left work for the host: a bulk batch
 // (*np0 >= bulk_threshold), inserts pending above the device-driven levels
 // (*np_top != 0) or a look-back error (*err != 0); else 0.

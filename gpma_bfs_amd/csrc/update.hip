@@ -34,6 +34,21 @@ __global__ void k_publish(uint32_t *a, uint32_t na, const uint32_t *b, uint32_t 
     if (lane == 0) __hip_atomic_store(dst, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+// stage[0..4) = cnt[0], cnt_big[0] (u64 as two words each), stage[4 .. 4 + n)
+// = reach[0 .. n): one captured step's read-back in one place.
+__global__ void k_step_gather(const unsigned long long *cnt, const unsigned long long *cnt_big, const uint32_t *reach,
+                              uint32_t n, uint32_t *stage) {
+    const uint32_t lane = threadIdx.x;  // one wave
+    if (lane == 0) {
+        const unsigned long long a = *cnt, b = *cnt_big;
+        stage[0] = (uint32_t)a;
+        stage[1] = (uint32_t)(a >> 32);
+        stage[2] = (uint32_t)b;
+        stage[3] = (uint32_t)(b >> 32);
+    }
+    for (uint32_t i = lane; i < n; i += WAVE) stage[4 + i] = reach[i];
+}
+
 // Host-work flag of a captured step (see launch_step_gate).
 __global__ void k_step_gate(const uint32_t *np0, const uint32_t *np_top, const uint32_t *err, uint32_t bulk_threshold,
                             uint32_t *gate) {
@@ -794,6 +809,10 @@ void launch_publish(const uint32_t *a, uint32_t na, const uint32_t *b, uint32_t 
                     bool clear_a, hipStream_t st) {
     hipLaunchKernelGGL(k_publish, dim3(1), dim3(WAVE), 0, st, const_cast<uint32_t *>(a), na, b, nb, dst, seq,
                        clear_a ? 1u : 0u);
+}
+void launch_step_gather(const unsigned long long *cnt, const unsigned long long *cnt_big, const uint32_t *reach,
+                        uint32_t n, uint32_t *stage, hipStream_t st) {
+    hipLaunchKernelGGL(k_step_gather, dim3(1), dim3(WAVE), 0, st, cnt, cnt_big, reach, n, stage);
 }
 void launch_step_gate(const uint32_t *np0, const uint32_t *np_top, const uint32_t *err, uint32_t bulk_threshold,
                       uint32_t *gate, hipStream_t st) {
