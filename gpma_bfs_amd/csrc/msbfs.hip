@@ -347,24 +347,29 @@ __global__ __launch_bounds__(T) void k_ms_finalize(const uint32_t *__restrict__ 
         const uint64_t i = i0 + threadIdx.x;
         bool small = false, big = false;
         uint32_t w = 0, len = 0;
+        uint64_t fresh = 0;
         if (i < nt) {
             w = dense ? (uint32_t)i : touched[i];
             const uint64_t acc = nx[w];
             if (acc) nx[w] = 0;
             const uint64_t sn = acc ? seen[w] : ~0ull;
-            const uint64_t fresh = acc & ~sn;
+            fresh = acc & ~sn;
             if (fresh) {
                 seen[w] = sn | fresh;
                 fr_next[w] = fresh;
-                uint64_t bits = fresh;
-                while (bits) {
-                    const uint32_t s = (uint32_t)__ffsll((unsigned long long)bits) - 1;
-                    bits &= bits - 1;
-                    lv16[(uint64_t)w * 64 + s] = newlev;
-                }
                 len = rlen(ro, w);
                 small = len <= MS_BIG_RANGE;
                 big = !small;
+            }
+        }
+        {
+            const uint32_t lane = lane_id();
+            for (uint64_t todo = __ballot(fresh != 0); todo; todo &= todo - 1) {
+                const int j = __ffsll((unsigned long long)todo) - 1;
+                const uint32_t wj = (uint32_t)__builtin_amdgcn_readlane((int)w, j);
+                const uint32_t flo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)fresh, j);
+                const uint32_t fhi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(fresh >> 32), j);
+                if ((((uint64_t)fhi << 32 | flo) >> lane) & 1ull) lv16[(uint64_t)wj * 64 + lane] = newlev;
             }
         }
         // short ranges through LDS, long ranges directly
