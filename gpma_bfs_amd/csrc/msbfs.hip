@@ -343,6 +343,8 @@ __global__ __launch_bounds__(T) void k_ms_finalize(const uint32_t *__restrict__ 
     }
     __syncthreads();
     const uint16_t newlev = (uint16_t)(level + 1);
+    // a workgroup whose whole share fits the LDS queue never flushes early
+    const bool roomy = (((uint64_t)nt + stride - 1) / stride) * T <= LQ;
     for (uint64_t i0 = (uint64_t)blockIdx.x * T; i0 < nt; i0 += stride) {
         const uint64_t i = i0 + threadIdx.x;
         bool small = false, big = false;
@@ -379,10 +381,10 @@ __global__ __launch_bounds__(T) void k_ms_finalize(const uint32_t *__restrict__ 
             uint32_t b = 0;
             if (lane_id() == leader) b = atomicAdd(&lq_n, (uint32_t)__popcll(ms));
             b = __shfl(b, leader, WAVE);
-            if (small) {
-                lq_buf[b + (uint32_t)__popcll(ms & lanemask_lt())] = w;
-                atomicAdd(&lq_work, len);
-            }
+            if (small) lq_buf[b + (uint32_t)__popcll(ms & lanemask_lt())] = w;
+            uint32_t wsum = small ? len : 0;
+            for (int d = WAVE / 2; d > 0; d >>= 1) wsum += __shfl_xor(wsum, d, WAVE);
+            if (lane_id() == leader) atomicAdd(&lq_work, wsum);
         }
         const uint64_t mb = __ballot(big);
         if (mb) {
@@ -395,6 +397,7 @@ __global__ __launch_bounds__(T) void k_ms_finalize(const uint32_t *__restrict__ 
             b = __shfl(b, leader, WAVE);
             if (big) qbig2[b + (uint32_t)__popcll(mb & lanemask_lt())] = w;
         }
+        if (roomy) continue;
         __syncthreads();
         if (lq_n + T > LQ) {
             if (threadIdx.x == 0) lq_base = (uint32_t)atomicAdd(cnt2, ((unsigned long long)lq_work << 32) | lq_n);
