@@ -120,10 +120,11 @@ __device__ __forceinline__ void emit2(bool claimed, uint32_t dst, uint32_t len, 
         uint32_t base = 0;
         if (lane_id() == leader) base = atomicAdd(lq.n, (uint32_t)__popcll(m));
         base = __shfl(base, leader, WAVE);
-        if (small) {
-            lq.buf[base + (uint32_t)__popcll(m & lanemask_lt())] = dst;
-            atomicAdd(lq.work, len);
-        }
+        if (small) lq.buf[base + (uint32_t)__popcll(m & lanemask_lt())] = dst;
+        // one LDS add per wave: same-word LDS atomics from many lanes serialise
+        uint32_t wsum = small ? len : 0;
+        for (int d = WAVE / 2; d > 0; d >>= 1) wsum += __shfl_xor(wsum, d, WAVE);
+        if (lane_id() == leader) atomicAdd(lq.work, wsum);
     }
     emit_big(claimed && len > BIG_RANGE, dst, len, nx.big, nx.big_cnt);
 }
