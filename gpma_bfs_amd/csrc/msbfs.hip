@@ -422,11 +422,13 @@ __global__ __launch_bounds__(T) void k_ms_finalize(const uint32_t *__restrict__ 
 // barrier: each lane loads its vertex's 128-byte level row (only if some
 // source reached it), then for every source the wave writes the group's 64
 // levels as one coalesced store, and lane s counts source s with a ballot.
-// partial[wave][s]: per-wave counts.
+// partial[wave][s]: per-wave counts. Workgroup 0 also zeroes reach[0 .. nsrc)
+// for k_ms_reach.
 __global__ __launch_bounds__(T) void k_ms_output(const uint16_t *__restrict__ lv16, const uint64_t *__restrict__ seen,
                                                  uint32_t N, uint32_t nsrc, uint32_t *__restrict__ out,
-                                                 uint32_t *__restrict__ partial) {
+                                                 uint32_t *__restrict__ partial, uint32_t *__restrict__ reach) {
     const uint32_t lane = lane_id();
+    if (blockIdx.x == 0 && threadIdx.x < nsrc) reach[threadIdx.x] = 0;  // k_ms_reach adds into it
     const uint32_t gw = (blockIdx.x * T + threadIdx.x) / WAVE;
     const uint32_t nw = gridDim.x * (T / WAVE);
     uint32_t acc = 0;
@@ -456,16 +458,25 @@ __global__ __launch_bounds__(T) void k_ms_output(const uint16_t *__restrict__ lv
     partial[(uint64_t)gw * WAVE + lane] = acc;
 }
 
-// reach[s] = sum of the partials' column s (one workgroup per source).
-__global__ __launch_bounds__(T) void k_ms_reach(const uint32_t *__restrict__ partial, uint32_t nblocks,
-                                                uint32_t *__restrict__ reach) {
-    __shared__ uint32_t scratch[T / WAVE + 1];
-    const uint32_t s = blockIdx.x;
-    uint32_t r = 0;
-    for (uint32_t b = threadIdx.x; b < nblocks; b += T) r += partial[(uint64_t)b * 64 + s];
-    uint32_t tot;
-    (void)block_exclusive_scan(r, scratch, &tot);
-    if (threadIdx.x == 0) reach[s] = tot;
+// reach[s] += the sum of column s over a share of the partials' rows: each
+// workgroup reads whole 64-word rows (coalesced), reduces in LDS and adds
+// one word per source (reach was zeroed by the output pass).
+constexpr uint32_t RT = 1024, RB = 16;  // threads and workgroups of the reduction
+__global__ __launch_bounds__(RT) void k_ms_reach(const uint32_t *__restrict__ partial, uint32_t nrows, uint32_t nsrc,
+                                                 uint32_t *__restrict__ reach) {
+    __shared__ uint32_t red[RT / 64][64];
+    const uint32_t s = threadIdx.x & 63, r0 = threadIdx.x >> 6;
+    const uint32_t per = (nrows + gridDim.x - 1) / gridDim.x;
+    const uint32_t a = blockIdx.x * per, b = min(nrows, a + per);
+    uint32_t sum = 0;
+    for (uint32_t r = a + r0; r < b; r += RT / 64) sum += partial[(uint64_t)r * 64 + s];
+    red[r0][s] = sum;
+    __syncthreads();
+    if (threadIdx.x < nsrc) {
+        uint32_t t = 0;
+        for (uint32_t j = 0; j < RT / 64; j++) t += red[j][threadIdx.x];
+        if (t) atomicAdd(reach + threadIdx.x, t);
+    }
 }
 
 __global__ void k_ms_clear(unsigned long long *__restrict__ cnt, unsigned long long *__restrict__ cnt_big,
@@ -510,8 +521,8 @@ static uint32_t ms_output_grid(uint32_t N) {
 
 void ms_output(const MsState &m, uint32_t N, uint32_t nsrc, uint32_t *out, uint32_t *reach, hipStream_t st) {
     const uint32_t nb = ms_output_grid(N);
-    hipLaunchKernelGGL(k_ms_output, dim3(nb), dim3(T), 0, st, m.lv16, m.seen, N, nsrc, out, m.partial);
-    hipLaunchKernelGGL(k_ms_reach, dim3(nsrc), dim3(T), 0, st, m.partial, nb * (T / WAVE), reach);
+    hipLaunchKernelGGL(k_ms_output, dim3(nb), dim3(T), 0, st, m.lv16, m.seen, N, nsrc, out, m.partial, reach);
+    hipLaunchKernelGGL(k_ms_reach, dim3(RB), dim3(RT), 0, st, m.partial, nb * (T / WAVE), nsrc, reach);
 }
 
 uint32_t ms_output_blocks(uint32_t N) { return ms_output_grid(N) * (T / WAVE); }
