@@ -233,6 +233,31 @@ def test_multi_source_bfs_matches_single_source(pull_permille):
     assert np.array_equal(lv2, lv[:7]) and np.array_equal(reach2, reach[:7])
 
 
+def test_multi_source_bfs_large_dense_levels():
+    """4.2M vertices (R-MAT scale 22, 2^24 draws): on the widest levels every
+    vertex's mask is scanned and each workgroup's share of vertices exceeds
+    its LDS queue, so the next frontier is flushed in several pieces; every
+    source still equals the single-source BFS, two equal the oracle."""
+    N = 1 << 22
+    s, d = G.gen_rmat(22, 1 << 24, seed=41)
+    keys = np.unique(G.make_keys(s, d))
+    keys = keys[(keys >> np.uint64(32)) != (keys & np.uint64(0xFFFFFFFF))]
+    g = G.GPMA(N)
+    G.update_gpma(g, keys, np.ones(len(keys), np.uint8))
+    starts = np.random.default_rng(8).integers(0, N, 64).astype(np.uint32)
+    starts[0] = 0  # the R-MAT hub: the widest traversal
+    lv, reach = G.gpma_bfs_multi(g, starts)
+    assert reach[0] > N // 4
+    for i, st in enumerate(starts):
+        one, r1 = G.gpma_bfs(g, int(st))
+        assert np.array_equal(lv[i], one) and reach[i] == r1, f"source {i} ({st})"
+    srcs = (keys >> np.uint64(32)).astype(np.uint32)
+    dsts = (keys & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    for i in (0, 1):
+        olv, oreach = O.bfs_edges(N, srcs, dsts, int(starts[i]))
+        assert np.array_equal(lv[i], olv) and reach[i] == oreach
+
+
 def test_fused_batches_equal_sequential():
     """gpma_update_batches over uneven batches leaves the same live keys (and
     BFS levels) as applying the batches one by one, including a key inserted
