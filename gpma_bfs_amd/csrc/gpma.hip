@@ -196,7 +196,7 @@ struct gpma_handle {
     uint64_t step_clock = 0;
     Buf step_n;                    // device u32: records of the step being replayed
     Buf step_stage;                // device: BFS counts and reachable counts gathered for the read-back
-    unsigned long long *step_h = nullptr;  // pinned: BFS counts and reachable counts read back
+    uint32_t step_rb[4 + 64] = {};  // a replay's read-back: frontier counts (2 x u64), reachable counts
     gpma_step_stats sst{};
 };
 
@@ -1006,8 +1006,6 @@ int step_graph(gpma_handle *g, uint32_t cls, uint32_t nsrc, uint32_t *dlev, gpma
     TRY(ensure_msd(g, cap, pl));
     TRY(g->step_n.ensure(256, &g->dev_bytes));
     TRY(g->step_stage.ensure((4 + 64) * 4, &g->dev_bytes));
-    if (!g->step_h)
-        HIP_TRY(hipHostMalloc((void **)&g->step_h, (2 * STEP_MAX_BLEVELS + 4 + 64) * 8, hipHostMallocDefault));
     *out = nullptr;
     if (nsrc) {
         TRY(ensure_msbfs(g));
@@ -1072,9 +1070,9 @@ int run_step(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint64
     TRY(wait_signal(g, SIG_STEP));
     g->ctr_clean = true;
     HIP_TRY(hipGetLastError());
-    unsigned long long *hb = g->step_h;  // [0] cnt, [1] cnt_big, then the reachable counts
+    uint32_t *rb = g->step_rb;  // [0..2) count words, [2..4) long-range count words, then reachable
     std::memcpy(g->hctr, g->sig + 1, C_COUNT * 4);
-    if (nsrc) std::memcpy(hb, g->sig + 1 + C_COUNT, (4 + nsrc) * 4);
+    if (nsrc) std::memcpy(rb, g->sig + 1 + C_COUNT, (4 + nsrc) * 4);
     const bool gated = g->hctr[C_GATE] != 0;
     const int urc = finish_update(g, n, spec, true);
     if (urc != GPMA_OK && urc != GPMA_ERR_KEY_RANGE) return urc;
@@ -1087,7 +1085,7 @@ int run_step(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint64
         TRY(run_msbfs(g, g->ms_starts.as<uint32_t>(), nsrc, out, g->ms_reach.as<uint32_t>(), &depth));
         if (depth + 2 > g->step_blevels) g->step_blevels = std::min(depth + 2, STEP_MAX_BLEVELS);
         reach_ready = false;
-    } else if ((uint32_t)hb[0] != 0 || (uint32_t)hb[1] != 0) {
+    } else if (rb[0] != 0 || rb[2] != 0) {  // vertices in the frontier after the last captured level
         // deeper than the captured levels: continue from the next one
         g->sst.bfs_tails++;
         const MsState m = ms_state(g);
@@ -1099,7 +1097,7 @@ int run_step(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint64
     }
     if (reachable) {
         if (reach_ready) {
-            std::memcpy(reachable, hb + 2, nsrc * 4);
+            std::memcpy(reachable, rb + 4, nsrc * 4);
         } else {
             HIP_TRY(hipMemcpyAsync(reachable, g->ms_reach.p, nsrc * 4, hipMemcpyDeviceToHost, st));
         }
@@ -1292,7 +1290,6 @@ int gpma_destroy(gpma_t g) {
                   &g->ms_out, &g->ms_reach, &g->ctr, &g->step_n, &g->step_stage};
     drop_graphs(g);
     for (Buf *b : all) b->release(&g->dev_bytes);
-    if (g->step_h) (void)hipHostFree(g->step_h);
     if (g->hctr) (void)hipHostFree(g->hctr);
     if (g->sig) (void)hipHostFree(g->sig);
     if (g->hcnt) (void)hipHostFree(g->hcnt);
