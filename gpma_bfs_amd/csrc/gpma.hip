@@ -193,6 +193,7 @@ struct gpma_handle {
     std::vector<StepGraph> graphs;
     bool capturing = false;
     uint32_t step_blevels = 8;  // BFS levels a new capture holds (grows to the depth seen)
+    uint32_t step_ulevels = 5;  // rebalance levels a new capture holds (grows when a replay needed more)
     uint64_t step_clock = 0;
     Buf step_n;                    // device u32: records of the step being replayed
     Buf step_stage;                // device: BFS counts and reachable counts gathered for the read-back
@@ -1016,8 +1017,11 @@ int step_graph(gpma_handle *g, uint32_t cls, uint32_t nsrc, uint32_t *dlev, gpma
             *out = g->ms_out.as<uint32_t>();
         }
     }
+    // levels whose window fits one workgroup, at most as many as steps have
+    // needed (an empty captured level still costs two launches)
     uint32_t ulevels = 0;
-    while (ulevels + 1 <= g->h && ((uint64_t)g->S << (ulevels + 1)) <= WIN_MAX && ulevels + 2 < C_LVL_SLOTS)
+    while (ulevels + 1 <= g->h && ((uint64_t)g->S << (ulevels + 1)) <= WIN_MAX && ulevels + 2 < C_LVL_SLOTS &&
+           ulevels < g->step_ulevels)
         ulevels++;
     *blevels = nsrc ? std::min<uint32_t>(g->step_blevels, g->N) : 0;
     const std::vector<uint64_t> sig = step_signature(g, cls, nsrc, *out, ulevels, *blevels);
@@ -1077,6 +1081,10 @@ int run_step(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint64
     const int urc = finish_update(g, n, spec, true);
     if (urc != GPMA_OK && urc != GPMA_ERR_KEY_RANGE) return urc;
     if (gated) g->sst.update_tails++;
+    // inserts climbed above the captured levels: later captures hold one more
+    // than the highest level they reached
+    const uint32_t need = g->last_levels ? g->last_levels - 1 : 0;
+    if (need > spec && need >= g->step_ulevels) g->step_ulevels = need + 1;
     if (!nsrc) return urc;
     bool reach_ready = true;
     if (gated) {
