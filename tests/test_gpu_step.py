@@ -176,3 +176,34 @@ def test_step_host_continuations():
     assert G.gpma_step_stats(c)["host_steps"] >= 2
     assert_invariants(a)
     assert_invariants(c)
+
+
+def test_step_rebalance_levels_grow():
+    """A step whose inserts climb above the captured rebalance levels is
+    finished on the host with the same results; later captures hold the
+    higher levels, so a like step then needs no host continuation."""
+    N = 5000
+    src, dst = G.gen_rmat(13, 60000, seed=7)
+    m = (src < N) & (dst < N)
+    keys = np.unique(G.make_keys(src[m], dst[m]))
+    a, b = G.GPMA(N), G.GPMA(N)
+    for g in (a, b):
+        G.update_gpma(g, keys, np.ones(len(keys), np.uint8))
+    rng = np.random.default_rng(13)
+    starts = rng.integers(0, N, 8).astype(np.uint32)
+    tails = []
+    for row in (1001, 4003):  # far apart: each climbs to about level 6 on its own
+        have = keys[(keys >> np.uint64(32)) == np.uint64(row)] & np.uint64(0xFFFFFFFF)
+        new = np.setdiff1d(rng.permutation(N)[:200].astype(np.uint64), have)[:100]
+        k = (np.uint64(row) << np.uint64(32)) | np.sort(new)
+        v = np.ones(len(k), np.uint8)
+        rc, lv, reach = G.gpma_step(a, k, v, np.array([0, len(k)], np.uint64), starts, return_levels=True)
+        assert rc == G.GPMA_OK
+        G.update_gpma(b, k, v)
+        lv_b, reach_b = G.gpma_bfs_multi(b, starts)
+        assert np.array_equal(lv, lv_b) and np.array_equal(reach, reach_b), f"row {row}"
+        _same_state(a, b, f"row {row}")
+        tails.append((G.gpma_step_stats(a)["update_tails"], b.stats()["last_levels"]))
+    assert 5 < tails[0][1] - 1 <= 8, tails  # the first step climbed above five levels
+    assert tails[0][0] == 1 and tails[1][0] == 1 and tails[1][1] <= tails[0][1], tails
+    assert_invariants(a)
