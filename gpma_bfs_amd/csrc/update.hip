@@ -182,13 +182,20 @@ __global__ __launch_bounds__(CMP_BLOCK) void k_locate(const gkey *__restrict__ k
             p = lo;
             for (uint64_t j = lo + 1; j <= hi; j++)
                 if (seps[j] <= k) p = j;
+        }
+        // the guessed leaf is loaded together with the path check; a failed
+        // check (or no hint) searches and loads again
+        gkey lk[S];
+        if (row_hint) {
+            load_leaf_keys<S>(keys, p * S, lk);
             have = search_path_matches(seps, nseg, k, p);
         }
-        if (!have) p = bitwise_search(seps, nseg, k);
+        if (!have) {
+            p = bitwise_search(seps, nseg, k);
+            load_leaf_keys<S>(keys, p * S, lk);
+        }
         const uint64_t base = p * S;
         // handle_del_mod (gpma.cuh:188-199): the first slot holding the key
-        gkey lk[S];
-        load_leaf_keys<S>(keys, base, lk);
         uint32_t hit = S;
 #pragma unroll
         for (uint32_t j = S; j-- > 0;)
