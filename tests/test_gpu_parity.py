@@ -170,6 +170,31 @@ def test_bulk_path_matches_oracle():
     assert reach == oreach and np.array_equal(lv, olv)
 
 
+def test_million_record_batches():
+    """Batches in the millions through the plain path: a 4M-record window
+    (under the bulk threshold, so it takes the ordinary sort, locate and
+    rebalance) and three slides of 1M inserts + 1M deletes on 2^20 vertices;
+    the live set equals the closed form after every batch and the final BFS
+    equals the oracle."""
+    N = 1 << 20
+    src, dst = G.gen_rmat(20, 1 << 23, seed=17)
+    keys = G.make_keys(src, dst)
+    half = len(keys) // 2
+    assert 4_000_000 < half < 5_000_000
+    g = G.GPMA(N)
+    G.update_gpma(g, keys[:half], np.ones(half, np.uint8))
+    assert g.stats()["bulk_loads"] == 0
+    assert np.array_equal(g.export_live(), closed_form_live(keys, N, 0, num_slide=4))
+    for i, b, vals in protocol_batches(keys, 3, num_slide=4):
+        G.update_gpma(g, b, vals)
+        assert np.array_equal(g.export_live(), closed_form_live(keys, N, i + 1, num_slide=4)), f"slide {i}"
+    assert_invariants(g)
+    step = half // 4
+    lv, reach = G.gpma_bfs(g, 0)
+    olv, oreach = O.bfs_edges(N, src[3 * step: half + 3 * step], dst[3 * step: half + 3 * step], 0)
+    assert reach == oreach and np.array_equal(lv, olv)
+
+
 @pytest.mark.parametrize("pull_permille", [0, 1, 1000])
 def test_bfs_hubs_and_direction_modes(pull_permille):
     """Hub vertices (ranges far above the 2048-slot split) and both expansion
