@@ -192,7 +192,8 @@ __device__ __forceinline__ void visit_batch(const gkey (&k)[U], const gval (&v)[
 //   walked PUSH_UNROLL x 256 slots per round (owner by binary search).
 //   Workgroups [small_grid, gridDim): BIG_CHUNK-slot pieces of the big
 //   queue's ranges, so one hub is spread over many workgroups.
-constexpr uint32_t PUSH_TILE = 64;
+constexpr uint32_t PUSH_TILE = 64;      // most frontier vertices per tile
+constexpr uint32_t PUSH_MIN_TILE = 16;  // fewest (short frontiers)
 constexpr uint32_t PUSH_UNROLL = 4;
 
 __device__ __forceinline__ void push_body(const gkey *__restrict__ keys, const gval *__restrict__ values,
@@ -206,17 +207,20 @@ __device__ __forceinline__ void push_body(const gkey *__restrict__ keys, const g
     const uint32_t newlev = level + 1;
     if (blockIdx.x < small_grid) {
         const uint32_t nf = pk_count(cnt[level]);
-        for (uint64_t tile = (uint64_t)blockIdx.x * PUSH_TILE; tile < nf; tile += (uint64_t)small_grid * PUSH_TILE) {
+        // vertices per tile: a short frontier is spread over every workgroup
+        const uint32_t tn =
+            (uint32_t)min<uint64_t>(PUSH_TILE, max<uint64_t>(PUSH_MIN_TILE, (nf + small_grid - 1) / small_grid));
+        for (uint64_t tile = (uint64_t)blockIdx.x * tn; tile < nf; tile += (uint64_t)small_grid * tn) {
             const uint64_t fi = tile + tid;
             uint32_t b = 0, len = 0;
-            if (tid < PUSH_TILE && fi < nf) {
+            if (tid < tn && fi < nf) {
                 const uint32_t v = front[fi];
                 b = row_offset[v];
                 len = range_len(row_offset, v);
             }
             uint32_t total;
             const uint32_t ofs = block_exclusive_scan(len, scratch, &total);
-            if (tid < PUSH_TILE) {
+            if (tid < tn) {
                 s_beg[tid] = b;
                 s_ofs[tid] = ofs;
             }
@@ -232,7 +236,7 @@ __device__ __forceinline__ void push_body(const gkey *__restrict__ keys, const g
                     k[u] = KEY_NONE;
                     v[u] = 0;
                     if (in[u]) {
-                        uint32_t lo = 0, hi = PUSH_TILE;
+                        uint32_t lo = 0, hi = tn;
                         while (hi - lo > 1) {
                             const uint32_t mid = (lo + hi) >> 1;
                             if (s_ofs[mid] <= j) lo = mid;

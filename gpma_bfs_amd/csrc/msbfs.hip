@@ -34,7 +34,8 @@ namespace gpma {
 namespace {
 
 constexpr uint32_t T = 256;
-constexpr uint32_t MS_TILE = 64;
+constexpr uint32_t MS_TILE = 64;      // most frontier vertices per tile
+constexpr uint32_t MS_MIN_TILE = 16;  // fewest (short frontiers)
 constexpr uint32_t MS_UNROLL = 4;
 constexpr uint32_t MS_BIG_RANGE = 2048;
 constexpr uint32_t MS_BIG_CHUNK = 2048;
@@ -211,10 +212,12 @@ __global__ __launch_bounds__(T) void k_ms_expand(const gkey *__restrict__ keys, 
             }
         }
     } else if (blockIdx.x < small_grid) {
-        for (uint64_t tile = (uint64_t)blockIdx.x * MS_TILE; tile < nf; tile += (uint64_t)small_grid * MS_TILE) {
+        // vertices per tile: a short frontier is spread over every workgroup
+        const uint32_t tn = (uint32_t)min<uint64_t>(MS_TILE, max<uint64_t>(MS_MIN_TILE, (nf + small_grid - 1) / small_grid));
+        for (uint64_t tile = (uint64_t)blockIdx.x * tn; tile < nf; tile += (uint64_t)small_grid * tn) {
             const uint64_t fi = tile + tid;
             uint32_t bb = 0, len = 0;
-            if (tid < MS_TILE && fi < nf) {
+            if (tid < tn && fi < nf) {
                 const uint32_t vv = q[fi];
                 bb = ro[vv];
                 len = rlen(ro, vv);
@@ -222,7 +225,7 @@ __global__ __launch_bounds__(T) void k_ms_expand(const gkey *__restrict__ keys, 
             }
             uint32_t total;
             const uint32_t ofs = block_exclusive_scan(len, scratch, &total);
-            if (tid < MS_TILE) {
+            if (tid < tn) {
                 s_beg[tid] = bb;
                 s_ofs[tid] = ofs;
             }
@@ -238,7 +241,7 @@ __global__ __launch_bounds__(T) void k_ms_expand(const gkey *__restrict__ keys, 
                     v[u] = 0;
                     m[u] = 0;
                     if (j < total) {
-                        uint32_t lo = 0, hi = MS_TILE;
+                        uint32_t lo = 0, hi = tn;
                         while (hi - lo > 1) {
                             const uint32_t mid = (lo + hi) >> 1;
                             if (s_ofs[mid] <= j) lo = mid;
