@@ -93,7 +93,7 @@ struct Buf {
     }
 };
 
-constexpr uint32_t BFS_MAX_CHUNK = 16;
+constexpr uint32_t BFS_MAX_CHUNK = 64;
 
 enum Ctr : uint32_t {
     C_INVALID = 0,
@@ -158,6 +158,7 @@ struct gpma_handle {
         ms_partial, ms_starts, ms_out, ms_reach;
     unsigned long long *ms_hcnt = nullptr;
     unsigned long long *hcnt = nullptr;  // pinned mirror of one chunk of bcnt and bcntb
+    uint32_t bfs_depth = 0, ms_depth = 0;  // depth of the last traversal (sizes the next first chunk)
     double pull_fraction = 0.25;  // pull when the frontier's slots exceed this share of C
     Buf ctr;
     uint32_t *hctr = nullptr;  // pinned mirror of ctr
@@ -761,8 +762,11 @@ int run_bfs(const gkey *keys, const gval *values, uint64_t C, const uint32_t *ro
     const unsigned long long thr =
         C ? (unsigned long long)(pull_fraction * (double)C) : ~0ull;  // no pull without the array size
     uint64_t reach = 1;
+    // 4 levels first (a start with a short reach ends there); a traversal
+    // still alive then takes the rest of the last one's depth + 2 at once,
+    // so one like it ends after two round trips; then chunks of 4, doubling
     uint32_t level = 1, chunk = 4;
-    bool done = false;
+    bool done = false, hinted = false;
     while (!done) {
         if ((uint64_t)level + chunk + 1 > (uint64_t)N + BFS_MAX_CHUNK) chunk = 1;
         launch_bfs_clear(b, level + 1, level + chunk + 1, st);
@@ -784,11 +788,19 @@ int run_bfs(const gkey *keys, const gval *values, uint64_t C, const uint32_t *ro
         }
         for (uint32_t j = 0; j < chunk && !done; j++) {
             const uint64_t c = (uint64_t)(uint32_t)hcnt[j] + (uint32_t)hcnt[BFS_MAX_CHUNK + 1 + j];
-            if (c == 0) done = true;
+            if (c == 0) {
+                done = true;
+                if (prof) prof->bfs_depth = level + j;  // the last level with a frontier
+            }
             reach += c;
         }
         level += chunk;
-        chunk = std::min<uint32_t>(chunk * 2, BFS_MAX_CHUNK);
+        chunk = hinted ? 4 : std::min<uint32_t>(chunk * 2, BFS_MAX_CHUNK);
+        hinted = false;
+        if (level == 5 && prof && prof->bfs_depth + 2 > level + chunk) {
+            chunk = std::min(BFS_MAX_CHUNK, prof->bfs_depth + 2 - 4);
+            hinted = true;
+        }
         if (level >= N + 1) done = true;
     }
     HIP_TRY(hipGetLastError());
@@ -856,7 +868,11 @@ int msbfs_levels(gpma_handle *g, const MsState &m, uint32_t level, uint32_t *dep
     hipStream_t st = g->st;
     const uint32_t N = g->N;
     const unsigned long long thr = ms_pull_threshold(g);
-    uint32_t chunk = 4;
+    // a traversal from level 1 first covers the last one's depth + 2, then
+    // takes chunks of 4, doubling
+    const bool hinted = level == 1 && g->ms_depth + 2 > 4;
+    uint32_t chunk = hinted ? std::min(BFS_MAX_CHUNK, g->ms_depth + 2) : 4;
+    bool first = true;
     bool done = false;
     *depth = level;
     while (!done) {
@@ -883,9 +899,11 @@ int msbfs_levels(gpma_handle *g, const MsState &m, uint32_t level, uint32_t *dep
             if ((uint32_t)g->ms_hcnt[j] == 0 && (uint32_t)g->ms_hcnt[BFS_MAX_CHUNK + 1 + j] == 0) done = true;
         }
         level += chunk;
-        chunk = std::min<uint32_t>(chunk * 2, BFS_MAX_CHUNK);
+        chunk = first && hinted ? 4 : std::min<uint32_t>(chunk * 2, BFS_MAX_CHUNK);
+        first = false;
         if (level >= N + 1) done = true;
     }
+    g->ms_depth = *depth;
     return GPMA_OK;
 }
 
