@@ -329,8 +329,10 @@ def live_checksum(keys):
 def run_large(G, args):
     """Large single-GPU configuration: R-MAT scale 25 (2^25 nodes), 2^29 draws
     (a,b,c = .57,.19,.19, seed 3), deduplicated; the first half is loaded in
-    bulk, then slides of 2^20 inserts + 2^20 deletes (one batch each), with a
-    64-source BFS (sources seed 3) after the timed slides."""
+    bulk, then `--large-steps` (200) slides of 2^20 inserts + 2^20 deletes (one
+    batch each) with a 64-source BFS (sources seed 3 + its index) after every
+    10th slide. Slides are staged in HBM ten at a time, outside the timed
+    slides; update throughput counts the slides, the BFS runs are timed apart."""
     scale, draws = 25, 1 << 29
     N = 1 << scale
     t0 = time.time()
@@ -349,33 +351,45 @@ def run_large(G, args):
     st = g.stats()
     log(f"large: bulk window of {half} edges in {t_load:.2f}s -> C={st['capacity']} S={st['segment_length']} "
         f"h={st['tree_height']}")
-    W, K = 2, args.large_steps
+    W, K, EVERY = 2, args.large_steps, 10
     nsl = W + K
     if half + nsl * step > E:
         raise RuntimeError("not enough edges for the requested slides")
     nb = 2 * step
-    batches = np.empty(nsl * nb, np.uint64)
-    for i in range(nsl):
-        batches[i * nb: i * nb + step] = keys[half + i * step: half + (i + 1) * step]
-        batches[i * nb + step: (i + 1) * nb] = keys[i * step: (i + 1) * step]
-    vals = np.tile(np.concatenate([np.ones(step, np.uint8), np.zeros(step, np.uint8)]), nsl)
-    dk = G.DeviceBuffer.from_numpy(batches)
-    dv = G.DeviceBuffer.from_numpy(vals)
-    del batches, vals
-    for i in range(W):
-        G.update_gpma_device(g, dk.ptr + i * nb * 8, dv.ptr + i * nb, nb)
-    g.sync()
-    per = []
-    t_start = time.perf_counter()
-    for i in range(W, nsl):
-        a = time.perf_counter()
-        G.update_gpma_device(g, dk.ptr + i * nb * 8, dv.ptr + i * nb, nb)
-        per.append(time.perf_counter() - a)
-    g.sync()
-    t_total = time.perf_counter() - t_start
-    ups = K * nb / t_total
-    dk.free()
-    dv.free()
+    vals1 = np.concatenate([np.ones(step, np.uint8), np.zeros(step, np.uint8)])
+    dlev = G.DeviceBuffer(64 * N * 4)
+    per, ms_t = [], []
+    t_upd = 0.0
+    i = 0
+    while i < nsl:
+        group = min(EVERY if i >= W else W, nsl - i)
+        batches = np.empty(group * nb, np.uint64)
+        for j in range(group):
+            sl = i + j
+            batches[j * nb: j * nb + step] = keys[half + sl * step: half + (sl + 1) * step]
+            batches[j * nb + step: (j + 1) * nb] = keys[sl * step: (sl + 1) * step]
+        dk = G.DeviceBuffer.from_numpy(batches)
+        dv = G.DeviceBuffer.from_numpy(np.tile(vals1, group))
+        del batches
+        g.sync()
+        a0 = time.perf_counter()
+        for j in range(group):
+            a = time.perf_counter()
+            G.update_gpma_device(g, dk.ptr + j * nb * 8, dv.ptr + j * nb, nb)
+            if i >= W:
+                per.append(time.perf_counter() - a)
+        g.sync()
+        if i >= W:
+            t_upd += time.perf_counter() - a0
+        dk.free()
+        dv.free()
+        i += group
+        if i > W and (i - W) % EVERY == 0:
+            starts = np.random.default_rng(3 + (i - W) // EVERY).integers(0, N, 64).astype(np.uint32)
+            a = time.perf_counter()
+            G.gpma_bfs_multi(g, starts, dlev.ptr)
+            ms_t.append(time.perf_counter() - a)
+    ups = K * nb / t_upd
     win = keys[nsl * step: half + nsl * step]
     # parity: live set == window + guards + 2 KEY_MAX, by count / sum / xor
     guards = (np.arange(N, dtype=np.uint64) << np.uint64(32)) | np.uint64(0xFFFFFFFF)
@@ -386,7 +400,7 @@ def run_large(G, args):
     expect = (sum(c[0] for c in ce), sum(c[1] for c in ce) % (1 << 64), ce[0][2] ^ ce[1][2] ^ ce[2][2])
     ok_live = c_live == expect
     outdeg = np.bincount((win >> np.uint64(32)).astype(np.int64), minlength=N)
-    # single-source BFS from node 0 and a 64-source BFS
+    # single-source BFS from node 0, and the last 64-source BFS's algorithmic bytes and parity
     lv, reach = G.gpma_bfs(g, 0)
     bfs_b, e_trav = bfs_bytes(N, outdeg, lv)
     bt = []
@@ -395,36 +409,35 @@ def run_large(G, args):
         G.gpma_bfs_count(g, 0)
         bt.append(time.perf_counter() - a)
     bfs_s = float(np.median(bt))
-    starts = np.random.default_rng(3).integers(0, N, 64).astype(np.uint32)
+    starts = np.random.default_rng(3 + K // EVERY).integers(0, N, 64).astype(np.uint32)
     lv_all, mreach = G.gpma_bfs_multi(g, starts)
     ok_ms = bool(np.array_equal(lv_all[0], G.gpma_bfs(g, int(starts[0]))[0]))
     mb, me = ms_bfs_bytes(N, outdeg, lv_all, 64)
     del lv_all
-    dlev = G.DeviceBuffer(64 * N * 4)
-    mt = []
-    for _ in range(3):
-        a = time.perf_counter()
-        G.gpma_bfs_multi(g, starts, dlev.ptr)
-        mt.append(time.perf_counter() - a)
     dlev.free()
-    ms_s = float(np.median(mt))
+    ms_s = float(np.median(ms_t)) if ms_t else float("nan")
     per_ms = np.array(per) * 1e3
     st = g.stats()
     g.close()
     ub = update_bytes(step, step)
     return {
         "workload": f"R-MAT scale 25 (N=2^25), 2^29 draws (a,b,c=.57,.19,.19, seed 3), dedup -> {E} edges; "
-                    f"window = first {half}; step = 2^20 inserts + 2^20 deletes",
+                    f"window = first {half}; {K} slides of 2^20 inserts + 2^20 deletes, a 64-source BFS "
+                    f"after every {EVERY}th",
         "nodes": N, "edges": E, "bulk_load_s": t_load, "steps": K, "warmup": W,
-        "edge_updates_per_s": ups, "ms_per_step": t_total / K * 1e3,
+        "edge_updates_per_s": ups, "ms_per_step": t_upd / K * 1e3,
         "latency_ms": {"p50": float(np.percentile(per_ms, 50)), "p99": float(np.percentile(per_ms, 99))},
-        "step_roofline": {"algorithmic_bytes": ub["total"], "achieved": ub["total"] / (t_total / K) / 1e9,
-                          "frac": ub["total"] / (t_total / K) / 1e9 / HBM_PEAK_GBS, "unit": "GB/s"},
+        "step_roofline": {"algorithmic_bytes": ub["total"], "achieved": ub["total"] / (t_upd / K) / 1e9,
+                          "frac": ub["total"] / (t_upd / K) / 1e9 / HBM_PEAK_GBS, "unit": "GB/s"},
+        "protocol_s": t_upd + sum(ms_t),
         "bfs": {"reachable": int(reach), "edges_traversed": e_trav, "ms": bfs_s * 1e3,
                 "gteps": e_trav / bfs_s / 1e9, "achieved_gbs": bfs_b / bfs_s / 1e9,
                 "frac": bfs_b / bfs_s / 1e9 / HBM_PEAK_GBS},
-        "msbfs": {"sources": 64, "ms": ms_s * 1e3, "edges": me, "gteps": me / ms_s / 1e9,
-                  "algorithmic_bytes": mb, "achieved_gbs": mb / ms_s / 1e9, "frac": mb / ms_s / 1e9 / HBM_PEAK_GBS},
+        "msbfs": {"sources": 64, "runs": len(ms_t), "ms_median": ms_s * 1e3,
+                  "ms_max": float(max(ms_t)) * 1e3 if ms_t else None,
+                  "last_run_edges": me, "gteps": me / ms_s / 1e9,
+                  "algorithmic_bytes": mb, "achieved_gbs": mb / ms_s / 1e9, "frac": mb / ms_s / 1e9 / HBM_PEAK_GBS,
+                  "note": "median over the runs after every 10th slide; bytes and edges from the final graph's run"},
         "layout": {"capacity": st["capacity"], "segment_length": st["segment_length"],
                    "tree_height": st["tree_height"], "device_bytes": st.get("device_bytes")},
         "parity": {"live_set_count_sum_xor": ok_live, "msbfs_row0_equals_bfs": ok_ms},
@@ -572,7 +585,7 @@ def main():
                     help="only the batched form (host path and captured graph) on the power-law graph")
     ap.add_argument("--large", action="store_true",
                     help="R-MAT 2^25 nodes / 2^29 draws instead (bulk load, 2^21-update slides, 64-source BFS)")
-    ap.add_argument("--large-steps", type=int, default=10)
+    ap.add_argument("--large-steps", type=int, default=200)
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="end with one device-to-device copy of this many MiB (counter calibration)")
     ap.add_argument("--serve", action="store_true",
@@ -604,8 +617,10 @@ def main():
         return
     if args.large:
         r = run_large(G, args)
-        log(f"large: {r['edge_updates_per_s'] / 1e9:.3f} G edge-updates/s, p99 {r['latency_ms']['p99']:.2f} ms, "
-            f"BFS {r['bfs']['gteps']:.1f} GTEPS, 64-source BFS {r['msbfs']['gteps']:.1f} GTEPS, parity {r['parity']}")
+        log(f"large: {r['steps']} slides, {r['edge_updates_per_s'] / 1e9:.3f} G edge-updates/s, p99 "
+            f"{r['latency_ms']['p99']:.2f} ms; {r['msbfs']['runs']} 64-source BFS runs, median "
+            f"{r['msbfs']['ms_median']:.1f} ms ({r['msbfs']['gteps']:.1f} GTEPS); BFS {r['bfs']['gteps']:.1f} GTEPS; "
+            f"whole protocol {r['protocol_s']:.2f} s; parity {r['parity']}")
         out = {"metric": METRIC, "value": r["edge_updates_per_s"], "unit": "edge-updates/s", "n_gpus": 1,
                "steps": r["steps"], "warmup": r["warmup"], "ms_per_step": r["ms_per_step"], "higher_is_better": True,
                "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
