@@ -16,6 +16,13 @@ the same timed region. The reference's own cpu_baseline (compiled from its
 sources into oracle/_ref) is timed on a bounded sample of the same workload
 on the host.
 
+Beside it, in the same JSON line: the batched form (8 fused uneven batches
+and a 64-source BFS per step, through the host path and as captured HIP
+graphs), the same steps in the plain form (each batch through update_gpma,
+then 64 single-source BFS), and the reference's GPMA CPU path (restated in
+oracle/) on every host core. --large, --serve and --batched-only run the
+other configurations instead.
+
 Prints progress on stderr and ONE JSON line on stdout.
 """
 import argparse
