@@ -148,9 +148,9 @@ __device__ __forceinline__ bool search_path_matches(const gkey *__restrict__ sep
 // One update per thread (gpma.cuh:204-226). Sorted input, so the last of
 // equal keys is the one applied (earlier duplicates are skipped).
 // With row_hint, the segment is first guessed from row_offset (the key lies
-// between the slots of guards src-1 and src) and the guess is checked against
-// the full bitwise search path; a failed check falls back to the walk, so the
-// result is always the reference search's.
+// between the slots of guards src-1 and src). Below 16 slots per segment the
+// guess is checked against the full bitwise search path and a failed check
+// falls back to the walk, so the result is always the reference search's.
 template <uint32_t S>
 __global__ __launch_bounds__(CMP_BLOCK) void k_locate(const gkey *__restrict__ ks, const gval *__restrict__ vs,
                                                       uint64_t n, const uint32_t *__restrict__ dn,
@@ -184,11 +184,15 @@ __global__ __launch_bounds__(CMP_BLOCK) void k_locate(const gkey *__restrict__ k
                 if (seps[j] <= k) p = j;
         }
         // the guessed leaf is loaded together with the path check; a failed
-        // check (or no hint) searches and loads again
+        // check (or no hint) searches and loads again. With 16+ slots per
+        // segment no segment is ever empty (every redistribution keeps the
+        // density above 1/16), so separators increase strictly, the bitwise
+        // search's answer is the last separator <= k, and it lies between the
+        // guards of rows src - 1 and src: the guess is the answer, unchecked.
         gkey lk[S];
         if (row_hint) {
             load_leaf_keys<S>(keys, p * S, lk);
-            have = search_path_matches(seps, nseg, k, p);
+            have = S >= 16 || search_path_matches(seps, nseg, k, p);
         }
         if (!have) {
             p = bitwise_search(seps, nseg, k);

@@ -61,3 +61,8 @@ def assert_invariants(g):
     expect = (np.arange(N, dtype=np.uint64) << np.uint64(32)) | np.uint64(0xFFFFFFFF)
     assert np.array_equal(k[guard_slots], expect), "row_offset[v+1] does not point at guard v"
     assert r[0] == 0
+    # with 16+ slots per segment every redistribution gives each segment at
+    # least one item (density >= 0.08 > 1/16), so segment first keys increase
+    S = g.stats()["segment_length"]
+    if S >= 16:
+        assert np.all(k[::S] != KEY_NONE), "a segment's first slot is empty"
