@@ -130,6 +130,11 @@ def test_live_set_closed_form_every_batch(N, scale, ndraws, seed):
         assert np.all((occ[1:] > occ[:-1]) | ((occ[1:] == occ[:-1]) & (occ[1:] == np.uint64(KMAX))))
         exp = (np.arange(N, dtype=np.uint64) << np.uint64(32)) | np.uint64(F)
         assert np.array_equal(k[r[1:].astype(np.int64)], exp)
+        # from 16 slots per segment no segment is left empty (segment first
+        # keys strictly increase), which the product's locate relies on
+        S = o.info()["segment_length"]
+        if S >= 16:
+            assert np.all(k[::S] != np.uint64(NONE)), f"slide {i}: empty segment at S={S}"
 
 
 def test_bfs_matches_scipy_shortest_paths():
