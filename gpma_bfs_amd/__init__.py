@@ -18,7 +18,12 @@ from ._lib import (  # noqa: F401
     COL_IDX_NONE,
     GPMA_ERR_ARG,
     GPMA_ERR_CAPACITY,
+    GPMA_ERR_HIP,
     GPMA_ERR_KEY_RANGE,
+    GPMA_ERR_NO_DEVICE,
+    GPMA_ERR_OOM,
+    GPMA_ERR_STOPPED,
+    GPMA_ERR_TIMEOUT,
     GPMA_OK,
     KEY_MAX,
     KEY_NONE,
@@ -133,6 +138,17 @@ class GPMA:
         out = np.empty(n.value, np.uint64)
         check(lib.gpma_export_live(self.handle, out.ctypes.data, n.value, ctypes.byref(n)), "gpma_export_live")
         return out[: n.value]
+
+    def live_digest(self):
+        """(count, sum mod 2^64, xor) of the live keys, computed on the device."""
+        c, sm, x = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib.gpma_live_digest(self.handle, ctypes.byref(c), ctypes.byref(sm), ctypes.byref(x)),
+              "gpma_live_digest")
+        return c.value, sm.value, x.value
+
+    def test_fail_allocs(self, skip, count):
+        """Make device allocations fail (testing the out-of-memory paths)."""
+        check(lib.gpma_test_fail_allocs(self.handle, skip, count), "gpma_test_fail_allocs")
 
     def sync(self):
         check(lib.gpma_sync(self.handle), "gpma_sync")
@@ -364,6 +380,21 @@ def gen_powerlaw(N, E, alpha=2.1, soft_cap=2800.0, p1=0.24, seed=0):
     check(lib.gpma_gen_powerlaw(N, E, alpha, soft_cap, p1, seed, src.ctypes.data, dst.ctypes.data),
           "gpma_gen_powerlaw")
     return src, dst
+
+
+def key_digest(keys):
+    """(count, sum mod 2^64, xor) of a u64 key array: what GPMA.live_digest()
+    returns for a live set equal to it (order and duplicates of distinct keys
+    do not matter; each key counts once per occurrence)."""
+    k = np.asarray(keys, np.uint64)
+    with np.errstate(over="ignore"):
+        return int(len(k)), int(k.sum(dtype=np.uint64)), int(np.bitwise_xor.reduce(k)) if len(k) else 0
+
+
+def combine_digests(*ds):
+    """Digest of the concatenation of key sets with the given digests."""
+    return (sum(d[0] for d in ds), sum(d[1] for d in ds) % (1 << 64),
+            __import__("functools").reduce(lambda a, b: a ^ b, [d[2] for d in ds], 0))
 
 
 def make_keys(src, dst):

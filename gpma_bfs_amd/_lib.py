@@ -159,6 +159,8 @@ SIGNATURES = {
     "gpma_copy_layout": (_I, [_P, _P, _P, _P]),
     "gpma_export_live": (_I, [_P, _P, _U64, _P]),
     "gpma_get_stats": (_I, [_P, _P]),
+    "gpma_live_digest": (_I, [_P, _P, _P, _P]),
+    "gpma_test_fail_allocs": (_I, [_P, _U32, _U32]),
     "gpma_sync": (_I, [_P]),
     "gpma_stream": (_P, [_P]),
     "gpma_last_error": (ctypes.c_char_p, []),

@@ -53,39 +53,68 @@ int fail(int code, const std::string &msg) {
         if (r_ != GPMA_OK) return r_; \
     } while (0)
 
+// Entry points on a handle whose array was lost to a failed allocation.
+#define CHECK_ALIVE(g, what)                                                                              \
+    do {                                                                                                  \
+        if ((g)->failed)                                                                                  \
+            return fail(GPMA_ERR_OOM, std::string(what) + ": the handle lost its array to a failed allocation"); \
+    } while (0)
+
 // gpma.cuh:104-126
 uint32_t fls32(uint32_t x) { return x ? 32u - (uint32_t)__builtin_clz(x) : 0u; }
+
+// Device memory held by one handle, and the allocation-failure injection of
+// gpma_test_fail_allocs (after `skip` more successful allocations, the next
+// `fail` ones report out of memory).
+struct Alloc {
+    uint64_t bytes = 0;
+    uint32_t skip = 0, fail = 0;
+    bool injected() {
+        if (!fail) return false;
+        if (skip) {
+            skip--;
+            return false;
+        }
+        fail--;
+        return true;
+    }
+};
 
 // Device buffer that only grows.
 struct Buf {
     void *p = nullptr;
     size_t bytes = 0;
-    int ensure(size_t need, uint64_t *total) {
+    int ensure(size_t need, Alloc *a) {
         if (need <= bytes && p) return GPMA_OK;
         size_t nb = std::max(need, bytes + bytes / 2);
         if (nb < 256) nb = 256;
         if (p) {
             (void)hipFree(p);
-            *total -= bytes;
+            a->bytes -= bytes;
             p = nullptr;
             bytes = 0;
         }
+        if (a->injected()) return fail(GPMA_ERR_OOM, "hipMalloc(" + std::to_string(nb) + "): injected failure");
         hipError_t e = hipMalloc(&p, nb);
         if (e != hipSuccess) {
             p = nullptr;
             return fail(GPMA_ERR_OOM, "hipMalloc(" + std::to_string(nb) + "): " + hipGetErrorString(e));
         }
         bytes = nb;
-        *total += nb;
+        a->bytes += nb;
         return GPMA_OK;
     }
-    void release(uint64_t *total) {
+    void release(Alloc *a) {
         if (p) {
             (void)hipFree(p);
-            *total -= bytes;
+            a->bytes -= bytes;
         }
         p = nullptr;
         bytes = 0;
+    }
+    void swap(Buf &o) {
+        std::swap(p, o.p);
+        std::swap(bytes, o.bytes);
     }
     template <typename T>
     T *as() const {
@@ -139,7 +168,8 @@ struct gpma_handle {
     uint32_t seg_override = 0;  // segment length after a resize (0: the reference formula)
     uint32_t spec_levels = SPEC_LEVELS;
     uint32_t sort_bits = 0;     // bucket bits of the batch sort (0: from the batch size)
-    uint64_t dev_bytes = 0;
+    Alloc mem;          // device bytes held, allocation-failure injection
+    bool failed = false;  // the array was lost to an allocation failure: every call returns GPMA_ERR_OOM
     // structure
     Buf keys, values, seps, row_offset;
     // update scratch
@@ -156,11 +186,14 @@ struct gpma_handle {
     // multi-source bfs scratch
     Buf ms_seen, ms_fr0, ms_fr1, ms_nx, ms_lv16, ms_q0, ms_q1, ms_qb0, ms_qb1, ms_touched, ms_cnt, ms_cntb, ms_tcnt,
         ms_partial, ms_starts, ms_out, ms_reach;
+    Buf ms_lv32;          // u32 level staging, allocated when a traversal reaches level 65,535
+    bool ms_wide = false; // the running multi-source traversal stages u32 levels
     unsigned long long *ms_hcnt = nullptr;
     unsigned long long *hcnt = nullptr;  // pinned mirror of one chunk of bcnt and bcntb
     uint32_t bfs_depth = 0, ms_depth = 0;  // depth of the last traversal (sizes the next first chunk)
     double pull_fraction = 0.25;  // pull when the frontier's slots exceed this share of C
     Buf ctr;
+    Buf digest;                // 3 u64: live-set digest
     uint32_t *hctr = nullptr;  // pinned mirror of ctr
     bool ctr_clean = false;    // ctr is all zero on the device (the last read cleared it)
     // completion signal (host-coherent pinned): [0] sequence, [1..] payload
@@ -331,11 +364,11 @@ int read_ctr(gpma_handle *g) {
 }
 
 int ensure_tmp(gpma_handle *g, uint64_t n) {
-    TRY(g->tmpk.ensure(n * 8, &g->dev_bytes));
-    TRY(g->tmpv.ensure(n, &g->dev_bytes));
+    TRY(g->tmpk.ensure(n * 8, &g->mem));
+    TRY(g->tmpv.ensure(n, &g->mem));
     uint64_t nch = (n + CHUNK - 1) / CHUNK + 1;
-    TRY(g->chunkcnt.ensure(nch * 4, &g->dev_bytes));
-    TRY(g->chunkoff.ensure(nch * 4, &g->dev_bytes));
+    TRY(g->chunkcnt.ensure(nch * 4, &g->mem));
+    TRY(g->chunkoff.ensure(nch * 4, &g->mem));
     return GPMA_OK;
 }
 
@@ -364,12 +397,75 @@ int count_live(gpma_handle *g, uint64_t w, uint64_t W, uint64_t *c) {
     return GPMA_OK;
 }
 
+// Array shape: capacity, segment length, height and level bounds.
+struct Shape {
+    uint64_t C;
+    uint32_t S, h;
+    std::vector<uint32_t> lower, upper;
+};
+Shape shape_of(const gpma_handle *g) { return {g->C, g->S, g->h, g->lower, g->upper}; }
+void set_shape(gpma_handle *g, const Shape &s) {
+    g->C = s.C;
+    g->S = s.S;
+    g->h = s.h;
+    g->lower = s.lower;
+    g->upper = s.upper;
+}
+
+// New keys / values / seps arrays of T slots (segment length g->S) for a
+// rebuild whose c live items wait in tmpk / tmpv. The new arrays are
+// allocated while the old ones still exist; if that fails, the old ones go
+// first (their items are in tmp) and the allocation is tried again. If that
+// fails too, the live items are put back into arrays of the old shape
+// (pending inserts are not applied) and GPMA_ERR_OOM is returned; if even
+// that fails, the handle is marked failed and every later call returns
+// GPMA_ERR_OOM without touching the device.
+int replace_arrays(gpma_handle *g, uint64_t T, uint64_t c, const Shape &old) {
+    auto alloc3 = [&](Buf &k, Buf &v, Buf &sp, uint64_t slots, uint32_t S) {
+        int rc = k.ensure(slots * 8, &g->mem);
+        if (rc == GPMA_OK) rc = v.ensure(slots, &g->mem);
+        if (rc == GPMA_OK) rc = sp.ensure((slots / S) * 8, &g->mem);
+        if (rc != GPMA_OK) {
+            k.release(&g->mem);
+            v.release(&g->mem);
+            sp.release(&g->mem);
+        }
+        return rc;
+    };
+    Buf nk, nv, ns;
+    int rc = alloc3(nk, nv, ns, T, g->S);
+    g->keys.release(&g->mem);
+    g->values.release(&g->mem);
+    g->seps.release(&g->mem);
+    if (rc != GPMA_OK) rc = alloc3(nk, nv, ns, T, g->S);
+    if (rc == GPMA_OK) {
+        g->keys.swap(nk);
+        g->values.swap(nv);
+        g->seps.swap(ns);
+        return GPMA_OK;
+    }
+    const std::string why = g_last_error;
+    set_shape(g, old);
+    if (alloc3(g->keys, g->values, g->seps, old.C, old.S) != GPMA_OK) {
+        g->failed = true;
+        return fail(GPMA_ERR_OOM, "rebuild to " + std::to_string(T) + " slots failed (" + why +
+                                      ") and the array could not be restored; the handle is unusable");
+    }
+    launch_merge_redistribute(g->tmpk.as<gkey>(), g->tmpv.as<gval>(), c, nullptr, nullptr, 0, g->keys.as<gkey>(),
+                              g->values.as<gval>(), g->seps.as<gkey>(), g->row_offset.as<uint32_t>(), 0, old.C, old.S,
+                              g->st);
+    (void)stream_sync(g);
+    return fail(GPMA_ERR_OOM, "rebuild to " + std::to_string(T) + " slots failed (" + why +
+                                  "); the live items were kept at the old capacity and the pending inserts dropped");
+}
+
 // resize_gpma + root rebalance (gpma.cuh:906-934,1034-1044) when
 // check_bounds, or significant_insert (gpma.cuh:936-966) when not: compact
 // every live item, size the array to the smallest power of two >= 4 with
 // floor(0.84 * C') >= live + pending, and redistribute the merge of the live
 // items and the pending inserts over the whole new array.
 int rebuild(gpma_handle *g, const gkey *pk, const gval *pv, uint64_t np, bool check_bounds) {
+    const Shape old = shape_of(g);
     uint64_t c = 0;
     phase_begin(g, P_REBUILD);
     TRY(count_live(g, 0, g->C, &c));
@@ -393,16 +489,10 @@ int rebuild(gpma_handle *g, const gkey *pk, const gval *pv, uint64_t np, bool ch
         }
     }
     if (T != g->C) {
-        // tmp already holds every live item, so the old arrays can go first
         phase_end(g, P_REBUILD, 1);
         TRY(stream_sync(g));
         phase_begin(g, P_REBUILD);
-        g->keys.release(&g->dev_bytes);
-        g->values.release(&g->dev_bytes);
-        g->seps.release(&g->dev_bytes);
-        TRY(g->keys.ensure(T * 8, &g->dev_bytes));
-        TRY(g->values.ensure(T, &g->dev_bytes));
-        TRY(g->seps.ensure((T / g->S) * 8, &g->dev_bytes));
+        TRY(replace_arrays(g, T, c, old));
     }
     g->C = T;
     if (check_bounds) g->resizes++;
@@ -420,10 +510,10 @@ int init_structure(gpma_handle *g) {
     g->S = 2;
     g->h = 1;
     recalc_density(g);
-    TRY(g->keys.ensure(4 * 8, &g->dev_bytes));
-    TRY(g->values.ensure(4, &g->dev_bytes));
-    TRY(g->seps.ensure(2 * 8, &g->dev_bytes));
-    TRY(g->row_offset.ensure(((uint64_t)g->N + 1) * 4, &g->dev_bytes));
+    TRY(g->keys.ensure(4 * 8, &g->mem));
+    TRY(g->values.ensure(4, &g->mem));
+    TRY(g->seps.ensure(2 * 8, &g->mem));
+    TRY(g->row_offset.ensure(((uint64_t)g->N + 1) * 4, &g->mem));
     gkey hk[4] = {KEY_MAX, KEY_NONE, KEY_MAX, KEY_NONE};
     gval hv[4] = {1, 0, 1, 0};
     gkey hs[2] = {KEY_MAX, KEY_MAX};
@@ -436,7 +526,7 @@ int init_structure(gpma_handle *g) {
 }
 
 int ensure_update_buffers(gpma_handle *g, uint64_t n) {
-    uint64_t *t = &g->dev_bytes;
+    Alloc *t = &g->mem;
     TRY(g->in_k.ensure(n * 8, t));
     TRY(g->in_v.ensure(n, t));
     TRY(g->sk0.ensure(n * 8, t));
@@ -495,7 +585,7 @@ void compact_step(gpma_handle *g, const gkey *ik, const gval *iv, const uint32_t
 }
 
 int ensure_msd(gpma_handle *g, uint64_t n, const MsdPlan &pl) {
-    uint64_t *t = &g->dev_bytes;
+    Alloc *t = &g->mem;
     TRY(g->msd_kp.ensure(n * 8, t));
     TRY(g->msd_rk.ensure(n * 4, t));
     TRY(g->msd_mat.ensure(2ull * pl.ntiles * pl.NB * 4, t));
@@ -727,7 +817,7 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
 
 int ensure_bfs(gpma_handle *g) {
     uint64_t n = (uint64_t)g->N + 1;
-    uint64_t *t = &g->dev_bytes;
+    Alloc *t = &g->mem;
     TRY(g->lev.ensure(n * 4, t));
     TRY(g->fr0.ensure(n * 4, t));
     TRY(g->fr1.ensure(n * 4, t));
@@ -810,7 +900,7 @@ int run_bfs(const gkey *keys, const gval *values, uint64_t C, const uint32_t *ro
 
 int ensure_msbfs(gpma_handle *g) {
     const uint64_t n = (uint64_t)g->N + 1;
-    uint64_t *t = &g->dev_bytes;
+    Alloc *t = &g->mem;
     TRY(g->ms_seen.ensure(n * 8, t));
     TRY(g->ms_fr0.ensure(n * 8, t));
     TRY(g->ms_fr1.ensure(n * 8, t));
@@ -839,6 +929,7 @@ MsState ms_state(gpma_handle *g) {
     m.fr[1] = g->ms_fr1.as<uint64_t>();
     m.nx = g->ms_nx.as<uint64_t>();
     m.lv16 = g->ms_lv16.as<uint16_t>();
+    m.lv32 = g->ms_wide ? g->ms_lv32.as<uint32_t>() : nullptr;
     m.q[0] = g->ms_q0.as<uint32_t>();
     m.q[1] = g->ms_q1.as<uint32_t>();
     m.qbig[0] = g->ms_qb0.as<uint32_t>();
@@ -863,9 +954,11 @@ unsigned long long ms_pull_threshold(const gpma_handle *g) {
 // Multi-source levels from `level` on, in chunks without a host round trip
 // per level (like run_bfs), until a level adds no vertex. cnt[level] must
 // hold the level's frontier counts and the touched count of `level` must be
-// clear. *depth: the last level run.
-int msbfs_levels(gpma_handle *g, const MsState &m, uint32_t level, uint32_t *depth) {
+// clear. *depth: the last level run. Widens the level staging to u32 before
+// the first level that stages a level above 65,535 (g->ms_wide).
+int msbfs_levels(gpma_handle *g, uint32_t level, uint32_t *depth) {
     hipStream_t st = g->st;
+    MsState m = ms_state(g);
     const uint32_t N = g->N;
     const unsigned long long thr = ms_pull_threshold(g);
     // a traversal from level 1 first covers the last one's depth + 2, then
@@ -882,9 +975,17 @@ int msbfs_levels(gpma_handle *g, const MsState &m, uint32_t level, uint32_t *dep
         // previous chunk)
         ms_clear(m, level + 1, level + chunk + 1, st);
         phase_begin(g, P_BFS);
-        for (uint32_t j = 0; j < chunk; j++)
+        for (uint32_t j = 0; j < chunk; j++) {
+            // level L stages level L + 1: past 65,535 the staging moves to u32 rows
+            if (!g->ms_wide && level + j + 1 > 0xFFFFu) {
+                TRY(g->ms_lv32.ensure((uint64_t)N * 256, &g->mem));
+                ms_widen(m, N, g->ms_lv32.as<uint32_t>(), st);
+                g->ms_wide = true;
+                m = ms_state(g);
+            }
             ms_level(g->keys.as<gkey>(), g->values.as<gval>(), g->C, N, g->row_offset.as<uint32_t>(), m, level + j,
                      thr, st);
+        }
         phase_end(g, P_BFS, 2 * chunk);
         {
             uint32_t w[4 * BFS_MAX_CHUNK];
@@ -911,13 +1012,13 @@ int msbfs_levels(gpma_handle *g, const MsState &m, uint32_t level, uint32_t *dep
 // reach[nsrc] (device).
 int run_msbfs(gpma_handle *g, const uint32_t *d_starts, uint32_t nsrc, uint32_t *out, uint32_t *reach,
               uint32_t *depth = nullptr) {
-    const MsState m = ms_state(g);
-    ms_init(m, g->N, d_starts, nsrc, g->row_offset.as<uint32_t>(), g->st);
+    g->ms_wide = false;
+    ms_init(ms_state(g), g->N, d_starts, nsrc, g->row_offset.as<uint32_t>(), g->st);
     uint32_t d = 0;
-    TRY(msbfs_levels(g, m, 1, &d));
+    TRY(msbfs_levels(g, 1, &d));
     if (depth) *depth = d;
     phase_begin(g, P_BFS);
-    ms_output(m, g->N, nsrc, out, reach, g->st);
+    ms_output(ms_state(g), g->N, nsrc, out, reach, g->st);
     phase_end(g, P_BFS, 2);
     if (g->prof) TRY(stream_sync(g));  // callers synchronise before reading results
     HIP_TRY(hipGetLastError());
@@ -969,6 +1070,7 @@ int capture_step(gpma_handle *g, uint32_t cls, const MsdPlan &pl, uint32_t uleve
         g->cc_epoch = 0;
         drop_graphs(g);
     }
+    g->ms_wide = false;  // graphs hold at most 64 BFS levels: u16 staging only
     HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     g->capturing = true;
     fill_u32(ctr, 0, C_COUNT, st);
@@ -1023,15 +1125,15 @@ int step_graph(gpma_handle *g, uint32_t cls, uint32_t nsrc, uint32_t *dlev, gpma
     TRY(ensure_update_buffers(g, cap));
     const MsdPlan pl = msd_plan(cap, g->N, g->sort_bits, true);
     TRY(ensure_msd(g, cap, pl));
-    TRY(g->step_n.ensure(256, &g->dev_bytes));
-    TRY(g->step_stage.ensure((4 + 64) * 4, &g->dev_bytes));
+    TRY(g->step_n.ensure(256, &g->mem));
+    TRY(g->step_stage.ensure((4 + 64) * 4, &g->mem));
     *out = nullptr;
     if (nsrc) {
         TRY(ensure_msbfs(g));
         if (dlev) {
             *out = dlev;
         } else {
-            TRY(g->ms_out.ensure((uint64_t)nsrc * g->N * 4, &g->dev_bytes));
+            TRY(g->ms_out.ensure((uint64_t)nsrc * g->N * 4, &g->mem));
             *out = g->ms_out.as<uint32_t>();
         }
     }
@@ -1087,6 +1189,7 @@ int run_step(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint64
     // reachable counts, then the step's sequence word
     *(volatile uint32_t *)g->sig = 0;
     std::atomic_thread_fence(std::memory_order_seq_cst);
+    g->ms_wide = false;  // the replay's traversal stages u16 levels
     HIP_TRY(hipGraphLaunch(sg->exec, st));
     g->sst.replays++;
     TRY(wait_signal(g, SIG_STEP));
@@ -1114,10 +1217,9 @@ int run_step(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint64
     } else if (rb[0] != 0 || rb[2] != 0) {  // vertices in the frontier after the last captured level
         // deeper than the captured levels: continue from the next one
         g->sst.bfs_tails++;
-        const MsState m = ms_state(g);
         uint32_t depth = 0;
-        TRY(msbfs_levels(g, m, blevels + 1, &depth));
-        ms_output(m, g->N, nsrc, out, g->ms_reach.as<uint32_t>(), st);
+        TRY(msbfs_levels(g, blevels + 1, &depth));
+        ms_output(ms_state(g), g->N, nsrc, out, g->ms_reach.as<uint32_t>(), st);
         if (depth + 2 > g->step_blevels) g->step_blevels = std::min(depth + 2, STEP_MAX_BLEVELS);
         reach_ready = false;
     }
@@ -1143,7 +1245,7 @@ int bfs_multi_impl(gpma_handle *g, const uint32_t *starts, uint32_t nsrc, uint32
     HIP_TRY(hipMemcpyAsync(g->ms_starts.p, starts, nsrc * 4, hipMemcpyHostToDevice, g->st));
     uint32_t *out = levels;
     if (!on_device || !levels) {
-        TRY(g->ms_out.ensure((uint64_t)nsrc * g->N * 4, &g->dev_bytes));
+        TRY(g->ms_out.ensure((uint64_t)nsrc * g->N * 4, &g->mem));
         out = g->ms_out.as<uint32_t>();
     }
     TRY(run_msbfs(g, g->ms_starts.as<uint32_t>(), nsrc, out, g->ms_reach.as<uint32_t>()));
@@ -1276,7 +1378,7 @@ int gpma_create(uint32_t row_num, const gpma_config *cfg, gpma_t *out) {
             rc = fail(GPMA_ERR_HIP, "hipStreamCreate failed");
             break;
         }
-        if ((rc = g->ctr.ensure(C_COUNT * 4, &g->dev_bytes)) != GPMA_OK) break;
+        if ((rc = g->ctr.ensure(C_COUNT * 4, &g->mem)) != GPMA_OK) break;
         if (hipHostMalloc((void **)&g->hctr, C_COUNT * 4, hipHostMallocDefault) != hipSuccess ||
             hipHostMalloc((void **)&g->sig, SIG_WORDS * 4, hipHostMallocCoherent) != hipSuccess) {
             rc = fail(GPMA_ERR_OOM, "hipHostMalloc failed");
@@ -1313,9 +1415,9 @@ int gpma_destroy(gpma_t g) {
                   &g->chunkoff, &g->lev, &g->fr0, &g->fr1, &g->fb0, &g->fb1, &g->bcnt, &g->bcntb,
                   &g->ms_seen, &g->ms_fr0, &g->ms_fr1, &g->ms_nx, &g->ms_lv16, &g->ms_q0, &g->ms_q1, &g->ms_qb0,
                   &g->ms_qb1, &g->ms_touched, &g->ms_cnt, &g->ms_cntb, &g->ms_tcnt, &g->ms_partial, &g->ms_starts,
-                  &g->ms_out, &g->ms_reach, &g->ctr, &g->step_n, &g->step_stage};
+                  &g->ms_out, &g->ms_reach, &g->ms_lv32, &g->ctr, &g->digest, &g->step_n, &g->step_stage};
     drop_graphs(g);
-    for (Buf *b : all) b->release(&g->dev_bytes);
+    for (Buf *b : all) b->release(&g->mem);
     if (g->hctr) (void)hipHostFree(g->hctr);
     if (g->sig) (void)hipHostFree(g->sig);
     if (g->hcnt) (void)hipHostFree(g->hcnt);
@@ -1329,12 +1431,14 @@ int gpma_destroy(gpma_t g) {
 
 int gpma_update(gpma_t g, const uint64_t *keys, const uint8_t *values, uint64_t n, int on_device) {
     if (!g) return fail(GPMA_ERR_ARG, "gpma_update: null handle");
+    CHECK_ALIVE(g, "gpma_update");
     return do_update(g, keys, values, n, on_device);
 }
 
 int gpma_update_batches(gpma_t g, const uint64_t *keys, const uint8_t *values, const uint64_t *offsets,
                         uint32_t nbatches, int on_device) {
     if (!g || !offsets) return fail(GPMA_ERR_ARG, "gpma_update_batches: null argument");
+    CHECK_ALIVE(g, "gpma_update_batches");
     if (nbatches == 0) return GPMA_OK;
     if (offsets[0] != 0) return fail(GPMA_ERR_ARG, "gpma_update_batches: offsets[0] must be 0");
     for (uint32_t b = 0; b < nbatches; b++)
@@ -1346,6 +1450,7 @@ int gpma_update_batches(gpma_t g, const uint64_t *keys, const uint8_t *values, c
 
 int gpma_bfs(gpma_t g, uint32_t start, uint32_t *levels, int on_device, uint32_t *reachable) {
     if (!g) return fail(GPMA_ERR_ARG, "gpma_bfs: null handle");
+    CHECK_ALIVE(g, "gpma_bfs");
     if (start >= g->N) return fail(GPMA_ERR_ARG, "gpma_bfs: start node >= row_num");
     HIP_TRY(hipSetDevice(g->device));
     TRY(ensure_bfs(g));
@@ -1364,6 +1469,7 @@ int gpma_bfs(gpma_t g, uint32_t start, uint32_t *levels, int on_device, uint32_t
 int gpma_bfs_multi(gpma_t g, const uint32_t *starts, uint32_t nsrc, uint32_t *levels, int on_device,
                    uint32_t *reachable) {
     if (!g || !starts) return fail(GPMA_ERR_ARG, "gpma_bfs_multi: null argument");
+    CHECK_ALIVE(g, "gpma_bfs_multi");
     if (nsrc == 0 || nsrc > 64) return fail(GPMA_ERR_ARG, "gpma_bfs_multi: nsrc must be in [1, 64]");
     for (uint32_t i = 0; i < nsrc; i++)
         if (starts[i] >= g->N) return fail(GPMA_ERR_ARG, "gpma_bfs_multi: start node >= row_num");
@@ -1374,6 +1480,7 @@ int gpma_step(gpma_t g, const uint64_t *keys, const uint8_t *values, const uint6
               int on_device, const uint32_t *starts, uint32_t nsrc, uint32_t *levels, int levels_on_device,
               uint32_t *reachable) {
     if (!g || !offsets) return fail(GPMA_ERR_ARG, "gpma_step: null argument");
+    CHECK_ALIVE(g, "gpma_step");
     if (offsets[0] != 0) return fail(GPMA_ERR_ARG, "gpma_step: offsets[0] must be 0");
     for (uint32_t b = 0; b < nbatches; b++)
         if (offsets[b + 1] < offsets[b]) return fail(GPMA_ERR_ARG, "gpma_step: offsets must not decrease");
@@ -1390,6 +1497,7 @@ int gpma_step(gpma_t g, const uint64_t *keys, const uint8_t *values, const uint6
 
 int gpma_step_prepare(gpma_t g, uint64_t max_records, uint32_t nsrc, uint32_t *levels, uint32_t bfs_levels) {
     if (!g) return fail(GPMA_ERR_ARG, "gpma_step_prepare: null handle");
+    CHECK_ALIVE(g, "gpma_step_prepare");
     if (nsrc > 64) return fail(GPMA_ERR_ARG, "gpma_step_prepare: nsrc must be in [0, 64]");
     if (bfs_levels > STEP_MAX_BLEVELS) return fail(GPMA_ERR_ARG, "gpma_step_prepare: bfs_levels above 64");
     HIP_TRY(hipSetDevice(g->device));
@@ -1404,7 +1512,7 @@ int gpma_step_prepare(gpma_t g, uint64_t max_records, uint32_t nsrc, uint32_t *l
     TRY(ensure_msd(g, 1ull << top_cls, msd_plan(1ull << top_cls, g->N, g->sort_bits, true)));
     if (nsrc) {
         TRY(ensure_msbfs(g));
-        if (!levels) TRY(g->ms_out.ensure((uint64_t)nsrc * g->N * 4, &g->dev_bytes));
+        if (!levels) TRY(g->ms_out.ensure((uint64_t)nsrc * g->N * 4, &g->mem));
     }
     for (uint32_t cls = STEP_MIN_CLASS;; cls++) {
         uint32_t *out = nullptr;
@@ -1472,6 +1580,7 @@ int gpma_csr_bfs(const uint64_t *keys, const uint8_t *values, const uint32_t *ro
 
 int gpma_get_csr(gpma_t g, gpma_csr *out) {
     if (!g || !out) return fail(GPMA_ERR_ARG, "gpma_get_csr: null argument");
+    CHECK_ALIVE(g, "gpma_get_csr");
     out->keys = g->keys.as<uint64_t>();
     out->values = g->values.as<uint8_t>();
     out->row_offset = g->row_offset.as<uint32_t>();
@@ -1485,6 +1594,7 @@ int gpma_get_csr(gpma_t g, gpma_csr *out) {
 
 int gpma_copy_layout(gpma_t g, uint64_t *keys, uint8_t *values, uint32_t *row_offset) {
     if (!g) return fail(GPMA_ERR_ARG, "gpma_copy_layout: null handle");
+    CHECK_ALIVE(g, "gpma_copy_layout");
     HIP_TRY(hipSetDevice(g->device));
     if (keys) HIP_TRY(hipMemcpyAsync(keys, g->keys.p, g->C * 8, hipMemcpyDeviceToHost, g->st));
     if (values) HIP_TRY(hipMemcpyAsync(values, g->values.p, g->C, hipMemcpyDeviceToHost, g->st));
@@ -1496,6 +1606,7 @@ int gpma_copy_layout(gpma_t g, uint64_t *keys, uint8_t *values, uint32_t *row_of
 
 int gpma_export_live(gpma_t g, uint64_t *out, uint64_t cap, uint64_t *n) {
     if (!g || !n) return fail(GPMA_ERR_ARG, "gpma_export_live: null argument");
+    CHECK_ALIVE(g, "gpma_export_live");
     HIP_TRY(hipSetDevice(g->device));
     uint64_t c = 0;
     TRY(count_live(g, 0, g->C, &c));
@@ -1511,6 +1622,29 @@ int gpma_export_live(gpma_t g, uint64_t *out, uint64_t cap, uint64_t *n) {
     return GPMA_OK;
 }
 
+int gpma_live_digest(gpma_t g, uint64_t *count, uint64_t *sum, uint64_t *xr) {
+    if (!g) return fail(GPMA_ERR_ARG, "gpma_live_digest: null handle");
+    CHECK_ALIVE(g, "gpma_live_digest");
+    HIP_TRY(hipSetDevice(g->device));
+    TRY(g->digest.ensure(3 * 8, &g->mem));
+    launch_live_digest(g->keys.as<gkey>(), g->values.as<gval>(), g->C, g->digest.as<unsigned long long>(), g->st);
+    uint64_t h[3] = {0, 0, 0};
+    HIP_TRY(hipMemcpyAsync(h, g->digest.p, sizeof(h), hipMemcpyDeviceToHost, g->st));
+    HIP_TRY(hipStreamSynchronize(g->st));
+    HIP_TRY(hipGetLastError());
+    if (count) *count = h[0];
+    if (sum) *sum = h[1];
+    if (xr) *xr = h[2];
+    return GPMA_OK;
+}
+
+int gpma_test_fail_allocs(gpma_t g, uint32_t skip, uint32_t count) {
+    if (!g) return fail(GPMA_ERR_ARG, "gpma_test_fail_allocs: null handle");
+    g->mem.skip = skip;
+    g->mem.fail = count;
+    return GPMA_OK;
+}
+
 int gpma_get_stats(gpma_t g, gpma_stats *out) {
     if (!g || !out) return fail(GPMA_ERR_ARG, "gpma_get_stats: null argument");
     out->capacity = g->C;
@@ -1523,7 +1657,7 @@ int gpma_get_stats(gpma_t g, gpma_stats *out) {
     out->dropped = g->dropped;
     out->resizes = g->resizes;
     out->bulk_loads = g->bulk;
-    out->device_bytes = g->dev_bytes;
+    out->device_bytes = g->mem.bytes;
     return GPMA_OK;
 }
 

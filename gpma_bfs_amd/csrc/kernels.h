@@ -139,6 +139,8 @@ void launch_merge_redistribute(const gkey *tk, const gval *tv, uint64_t c, const
                                uint64_t r, gkey *keys, gval *values, gkey *seps, uint32_t *row_offset,
                                uint64_t w, uint64_t W, uint32_t S, hipStream_t st);
 void launch_mark_done(uint8_t *pd, uint64_t a, uint64_t b, hipStream_t st);
+// out[0..3) = (count, wrapping sum, xor) of the live keys of keys[0 .. C).
+void launch_live_digest(const gkey *keys, const gval *values, uint64_t C, unsigned long long *out, hipStream_t st);
 // Write c compacted items to keys[0..c), KEY_NONE after; rebuild seps and
 // point row_offset at the compacted guards.
 void launch_write_compacted(const gkey *tk, const gval *tv, uint64_t c, gkey *keys, gval *values, uint64_t C,
@@ -167,6 +169,7 @@ void launch_bfs_level(const gkey *keys, const gval *values, uint64_t C, uint32_t
 struct MsState {
     uint64_t *seen, *fr[2], *nx;
     uint16_t *lv16;            // [N][64] levels, vertex-major
+    uint32_t *lv32;            // [N][64] u32 levels once the traversal passed level 65,534 (else null)
     uint32_t *q[2], *qbig[2];  // frontier lists by parity (short / long ranges)
     uint32_t *touched;         // vertices whose nx went non-zero this level
     unsigned long long *cnt, *cnt_big;  // per level: low = vertices, high = slots
@@ -183,5 +186,8 @@ void ms_level(const gkey *keys, const gval *values, uint64_t C, uint32_t N, cons
 // Source-major u32 levels out[nsrc][N] and reachable counts reach[nsrc].
 void ms_output(const MsState &m, uint32_t N, uint32_t nsrc, uint32_t *out, uint32_t *reach, hipStream_t st);
 uint32_t ms_output_blocks(uint32_t N);
+// Copy every staged level (where seen) into the u32 rows lv32[N][64]; the
+// levels after it then go to lv32 (MsState::lv32).
+void ms_widen(const MsState &m, uint32_t N, uint32_t *lv32, hipStream_t st);
 
 }  // namespace gpma

@@ -21,7 +21,9 @@
 // form's ORs wait for their old values) for a somewhat slower finalize.
 // Levels are kept as u16 in a vertex-major scratch (a vertex's 64 levels in
 // one 128-byte line) and transposed at the end into the caller's
-// source-major u32 arrays, with per-source reachable counts.
+// source-major u32 arrays, with per-source reachable counts. A traversal that
+// reaches level 65,535 first widens the staging to u32 (k_ms_widen; 256-byte
+// rows) and keeps writing u32 levels from there, so no depth wraps.
 //
 // The level of source s at w is 1 + hop distance, as in the single-source
 // BFS (gpma_bfs.cuh:446-509 semantics), so every source's array equals the
@@ -323,7 +325,8 @@ __global__ __launch_bounds__(T) void k_ms_finalize(const uint32_t *__restrict__ 
                                                    const unsigned long long *__restrict__ cnt_big,
                                                    uint64_t *__restrict__ fr_cur, uint64_t *__restrict__ fr_next,
                                                    uint64_t *__restrict__ seen, uint64_t *__restrict__ nx,
-                                                   uint16_t *__restrict__ lv16, const uint32_t *__restrict__ touched,
+                                                   uint16_t *__restrict__ lv16, uint32_t *__restrict__ lv32,
+                                                   const uint32_t *__restrict__ touched,
                                                    const uint32_t *__restrict__ tcnt, uint32_t *__restrict__ q2,
                                                    uint32_t *__restrict__ qbig2, unsigned long long *__restrict__ cnt2,
                                                    unsigned long long *__restrict__ cnt_big2,
@@ -345,7 +348,8 @@ __global__ __launch_bounds__(T) void k_ms_finalize(const uint32_t *__restrict__ 
         lq_work = 0;
     }
     __syncthreads();
-    const uint16_t newlev = (uint16_t)(level + 1);
+    // u16 staging while every level fits; lv32 (non-null once widened) past that
+    const uint32_t newlev = level + 1;
     // a workgroup whose whole share fits the LDS queue never flushes early
     const bool roomy = (((uint64_t)nt + stride - 1) / stride) * T <= LQ;
     for (uint64_t i0 = (uint64_t)blockIdx.x * T; i0 < nt; i0 += stride) {
@@ -374,7 +378,10 @@ __global__ __launch_bounds__(T) void k_ms_finalize(const uint32_t *__restrict__ 
                 const uint32_t wj = (uint32_t)__builtin_amdgcn_readlane((int)w, j);
                 const uint32_t flo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)fresh, j);
                 const uint32_t fhi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(fresh >> 32), j);
-                if ((((uint64_t)fhi << 32 | flo) >> lane) & 1ull) lv16[(uint64_t)wj * 64 + lane] = newlev;
+                if ((((uint64_t)fhi << 32 | flo) >> lane) & 1ull) {
+                    if (lv32) lv32[(uint64_t)wj * 64 + lane] = newlev;
+                    else lv16[(uint64_t)wj * 64 + lane] = (uint16_t)newlev;
+                }
             }
         }
         // short ranges through LDS, long ranges directly
@@ -427,9 +434,12 @@ __global__ __launch_bounds__(T) void k_ms_finalize(const uint32_t *__restrict__ 
 // levels as one coalesced store, and lane s counts source s with a ballot.
 // partial[wave][s]: per-wave counts. Workgroup 0 also zeroes reach[0 .. nsrc)
 // for k_ms_reach.
-__global__ __launch_bounds__(T) void k_ms_output(const uint16_t *__restrict__ lv16, const uint64_t *__restrict__ seen,
+// WIDE: the staging holds u32 levels (256-byte rows, read as 16 x 16 bytes).
+template <bool WIDE>
+__global__ __launch_bounds__(T) void k_ms_output(const void *__restrict__ lv, const uint64_t *__restrict__ seen,
                                                  uint32_t N, uint32_t nsrc, uint32_t *__restrict__ out,
                                                  uint32_t *__restrict__ partial, uint32_t *__restrict__ reach) {
+    constexpr uint32_t RW = WIDE ? 16 : 8;  // 16-byte words per vertex row
     const uint32_t lane = lane_id();
     if (blockIdx.x == 0 && threadIdx.x < nsrc) reach[threadIdx.x] = 0;  // k_ms_reach adds into it
     const uint32_t gw = (blockIdx.x * T + threadIdx.x) / WAVE;
@@ -439,21 +449,28 @@ __global__ __launch_bounds__(T) void k_ms_output(const uint16_t *__restrict__ lv
         const uint64_t v = v0 + lane;
         const bool in = v < N;
         const uint64_t mask = in ? seen[v] : 0;
-        uint4 row[8];
+        uint4 row[RW];
         if (mask) {
-            const uint4 *src = reinterpret_cast<const uint4 *>(lv16 + v * 64);
+            const uint4 *src = reinterpret_cast<const uint4 *>(lv) + v * RW;
 #pragma unroll
-            for (uint32_t q = 0; q < 8; q++) row[q] = src[q];
+            for (uint32_t q = 0; q < RW; q++) row[q] = src[q];
         } else {
 #pragma unroll
-            for (uint32_t q = 0; q < 8; q++) row[q] = make_uint4(0, 0, 0, 0);
+            for (uint32_t q = 0; q < RW; q++) row[q] = make_uint4(0, 0, 0, 0);
         }
 #pragma unroll
         for (uint32_t s = 0; s < 64; s++) {
-            const uint4 &r4 = row[s / 8];
-            const uint32_t word = (s % 8) < 2 ? r4.x : (s % 8) < 4 ? r4.y : (s % 8) < 6 ? r4.z : r4.w;
+            uint32_t level;
+            if (WIDE) {
+                const uint4 &r4 = row[s / 4];
+                level = (s % 4) == 0 ? r4.x : (s % 4) == 1 ? r4.y : (s % 4) == 2 ? r4.z : r4.w;
+            } else {
+                const uint4 &r4 = row[s / 8];
+                const uint32_t word = (s % 8) < 2 ? r4.x : (s % 8) < 4 ? r4.y : (s % 8) < 6 ? r4.z : r4.w;
+                level = (word >> (16 * (s & 1))) & 0xFFFFu;
+            }
             const bool hit = (mask >> s) & 1ull;
-            if (s < nsrc && in) out[(uint64_t)s * N + v] = hit ? (word >> (16 * (s & 1))) & 0xFFFFu : 0u;
+            if (s < nsrc && in) out[(uint64_t)s * N + v] = hit ? level : 0u;
             const uint32_t c = (uint32_t)__popcll(ballot(hit));
             if (lane == s) acc += c;
         }
@@ -479,6 +496,17 @@ __global__ __launch_bounds__(RT) void k_ms_reach(const uint32_t *__restrict__ pa
         uint32_t t = 0;
         for (uint32_t j = 0; j < RT / 64; j++) t += red[j][threadIdx.x];
         if (t) atomicAdd(reach + threadIdx.x, t);
+    }
+}
+
+// lv32[v][s] = lv16[v][s] wherever source s has reached v: the staging of a
+// traversal about to pass level 65,535 moves to u32 rows.
+__global__ void k_ms_widen(const uint16_t *__restrict__ lv16, const uint64_t *__restrict__ seen, uint32_t N,
+                           uint32_t *__restrict__ lv32) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)N * 64; i += stride) {
+        const uint64_t v = i >> 6;
+        if ((seen[v] >> (i & 63)) & 1ull) lv32[i] = lv16[i];
     }
 }
 
@@ -513,7 +541,7 @@ void ms_level(const gkey *keys, const gval *values, uint64_t C, uint32_t N, cons
                        m.qbig[cur], m.cnt, m.cnt_big, m.fr[cur], m.seen, m.nx, m.touched, m.tcnt + level, level,
                        pull_threshold, small_grid);
     hipLaunchKernelGGL(k_ms_finalize, dim3(1024), dim3(T), 0, st, m.q[cur], m.qbig[cur], m.cnt, m.cnt_big, m.fr[cur],
-                       m.fr[nxt], m.seen, m.nx, m.lv16, m.touched, m.tcnt + level, m.q[nxt], m.qbig[nxt],
+                       m.fr[nxt], m.seen, m.nx, m.lv16, m.lv32, m.touched, m.tcnt + level, m.q[nxt], m.qbig[nxt],
                        m.cnt + level + 1, m.cnt_big + level + 1, ro, N, level);
 }
 
@@ -524,10 +552,19 @@ static uint32_t ms_output_grid(uint32_t N) {
 
 void ms_output(const MsState &m, uint32_t N, uint32_t nsrc, uint32_t *out, uint32_t *reach, hipStream_t st) {
     const uint32_t nb = ms_output_grid(N);
-    hipLaunchKernelGGL(k_ms_output, dim3(nb), dim3(T), 0, st, m.lv16, m.seen, N, nsrc, out, m.partial, reach);
+    if (m.lv32)
+        hipLaunchKernelGGL(k_ms_output<true>, dim3(nb), dim3(T), 0, st, (const void *)m.lv32, m.seen, N, nsrc, out,
+                           m.partial, reach);
+    else
+        hipLaunchKernelGGL(k_ms_output<false>, dim3(nb), dim3(T), 0, st, (const void *)m.lv16, m.seen, N, nsrc, out,
+                           m.partial, reach);
     hipLaunchKernelGGL(k_ms_reach, dim3(RB), dim3(RT), 0, st, m.partial, nb * (T / WAVE), nsrc, reach);
 }
 
 uint32_t ms_output_blocks(uint32_t N) { return ms_output_grid(N) * (T / WAVE); }
+
+void ms_widen(const MsState &m, uint32_t N, uint32_t *lv32, hipStream_t st) {
+    hipLaunchKernelGGL(k_ms_widen, dim3(4096), dim3(256), 0, st, m.lv16, m.seen, N, lv32);
+}
 
 }  // namespace gpma

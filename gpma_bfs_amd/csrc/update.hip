@@ -793,6 +793,31 @@ __global__ __launch_bounds__(256) void k_fill_gaps(uint64_t m, gkey *__restrict_
     }
 }
 
+// (count, wrapping sum, xor) of the live keys of keys[0 .. C): a
+// size-independent digest of the live set. out[0..3) must be zero.
+__global__ __launch_bounds__(256) void k_live_digest(const gkey *__restrict__ keys, const gval *__restrict__ values,
+                                                     uint64_t C, unsigned long long *__restrict__ out) {
+    unsigned long long c = 0, sm = 0, x = 0;
+    for (uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x; p < C; p += (uint64_t)gridDim.x * 256) {
+        const gkey k = keys[p];
+        if (is_live(k, values[p])) {
+            c++;
+            sm += k;
+            x ^= k;
+        }
+    }
+    for (int d = WAVE / 2; d > 0; d >>= 1) {
+        c += __shfl_xor(c, d, WAVE);
+        sm += __shfl_xor(sm, d, WAVE);
+        x ^= __shfl_xor(x, d, WAVE);
+    }
+    if (lane_id() == 0) {
+        if (c) atomicAdd(out, c);
+        if (sm) atomicAdd(out + 1, sm);
+        if (x) atomicXor(out + 2, x);
+    }
+}
+
 // The reference keeps row_offset stale here (gpma.cuh:906-934 compacts
 // without touching it); pointing row_offset at the compacted guards instead
 // keeps every later BFS range inside the array.
@@ -949,6 +974,11 @@ void launch_merge_redistribute(const gkey *tk, const gval *tv, uint64_t c, const
     if (m) hipLaunchKernelGGL(k_merge_scatter, dim3(grid_for(m, 256, 8192)), dim3(256), 0, st, tk, tv, c, ik, iv, r,
                               keys, values, seps, row_offset, w, W, S);
     hipLaunchKernelGGL(k_fill_gaps, dim3(grid_for(W, 256, 8192)), dim3(256), 0, st, m, keys, values, seps, w, W, S);
+}
+
+void launch_live_digest(const gkey *keys, const gval *values, uint64_t C, unsigned long long *out, hipStream_t st) {
+    fill_u64((uint64_t *)out, 0, 3, st);
+    hipLaunchKernelGGL(k_live_digest, dim3(grid_for(C, 256, 4096)), dim3(256), 0, st, keys, values, C, out);
 }
 
 void launch_mark_done(uint8_t *pd, uint64_t a, uint64_t b, hipStream_t st) {

@@ -193,6 +193,21 @@ int gpma_export_live(gpma_t g, uint64_t *out, uint64_t cap, uint64_t *n);
 
 int gpma_get_stats(gpma_t g, gpma_stats *out);
 
+/* Digest of the live-key set (the keys gpma_export_live returns): their
+ * count, their sum modulo 2^64 and their xor, computed on the device. A
+ * size-independent parity check against a closed-form key set (no
+ * reference counterpart). Any output pointer may be NULL. */
+int gpma_live_digest(gpma_t g, uint64_t *count, uint64_t *sum, uint64_t *xr);
+
+/* Testing hook for the out-of-memory paths (no reference counterpart):
+ * after `skip` more successful device allocations of this handle, the next
+ * `count` ones fail as if the device were out of memory (count = 0: off).
+ * A rebuild (resize or bulk load) that cannot allocate its new arrays keeps
+ * the live items at the old capacity and returns GPMA_ERR_OOM without the
+ * pending inserts; if even that fails, every later call on the handle
+ * returns GPMA_ERR_OOM. */
+int gpma_test_fail_allocs(gpma_t g, uint32_t skip, uint32_t count);
+
 /* Wait for all work queued on the handle's stream. */
 int gpma_sync(gpma_t g);
 

@@ -556,6 +556,43 @@ uint32_t orc_bfs_edges(uint32_t N, const uint32_t *src, const uint32_t *dst, uin
     return reach;
 }
 
+// Textbook BFS (cpu_baseline/algorithms/bfs.hpp:7-29 semantics) from each of
+// nstart sources over one adjacency built once: levels[i * N + v]. Returns
+// the sum of the reachable counts.
+uint64_t orc_bfs_edges_multi(uint32_t N, const uint32_t *src, const uint32_t *dst, uint64_t E,
+                             const uint32_t *starts, uint32_t nstart, uint32_t *levels) {
+    std::vector<uint64_t> off((size_t)N + 1, 0);
+    for (uint64_t e = 0; e < E; e++) off[src[e] + 1]++;
+    for (uint32_t v = 0; v < N; v++) off[v + 1] += off[v];
+    std::vector<uint32_t> adj(E);
+    std::vector<uint64_t> pos(off.begin(), off.end() - 1);
+    for (uint64_t e = 0; e < E; e++) adj[pos[src[e]]++] = dst[e];
+    uint64_t total = 0;
+#pragma omp parallel for reduction(+ : total) schedule(dynamic, 1)
+    for (int64_t i = 0; i < (int64_t)nstart; i++) {
+        uint32_t *lv = levels + (uint64_t)i * N;
+        std::fill(lv, lv + N, 0u);
+        const uint32_t start = starts[i];
+        if (start >= N) continue;
+        std::vector<uint32_t> q;
+        q.reserve(1024);
+        q.push_back(start);
+        lv[start] = 1;
+        for (size_t head = 0; head < q.size(); head++) {
+            const uint32_t v = q[head];
+            for (uint64_t p = off[v]; p < off[v + 1]; p++) {
+                const uint32_t d = adj[p];
+                if (lv[d] == 0) {
+                    lv[d] = lv[v] + 1;
+                    q.push_back(d);
+                }
+            }
+        }
+        total += q.size();
+    }
+    return total;
+}
+
 // Redistribution slot of item i (gpma.cuh:636-654), exposed for brute-force checks.
 uint32_t orc_redispatch_slot(uint32_t width, uint32_t S, uint32_t m, uint32_t i) {
     uint64_t frac = width / S;

@@ -41,6 +41,8 @@ def lib():
         L.orc_bfs.restype = ctypes.c_uint32
         L.orc_bfs_edges.argtypes = [ctypes.c_uint32, _P, _P, ctypes.c_uint64, ctypes.c_uint32, _P]
         L.orc_bfs_edges.restype = ctypes.c_uint32
+        L.orc_bfs_edges_multi.argtypes = [ctypes.c_uint32, _P, _P, ctypes.c_uint64, _P, ctypes.c_uint32, _P]
+        L.orc_bfs_edges_multi.restype = ctypes.c_uint64
         L.orc_redispatch_slot.argtypes = [ctypes.c_uint32] * 4
         L.orc_redispatch_slot.restype = ctypes.c_uint32
         L.orc_fls.argtypes = [ctypes.c_uint32]
@@ -129,6 +131,18 @@ def bfs_edges(N, src, dst, start):
     lv = np.empty(N, np.uint32)
     reach = lib().orc_bfs_edges(N, src.ctypes.data, dst.ctypes.data, len(src), start, lv.ctypes.data)
     return lv, reach
+
+
+def bfs_edges_multi(N, src, dst, starts):
+    """Textbook BFS from each start over one edge list: levels[len(starts)][N]
+    and the reachable counts."""
+    src = np.ascontiguousarray(src, np.uint32)
+    dst = np.ascontiguousarray(dst, np.uint32)
+    st = np.ascontiguousarray(starts, np.uint32)
+    lv = np.empty((len(st), N), np.uint32)
+    lib().orc_bfs_edges_multi(N, src.ctypes.data, dst.ctypes.data, len(src), st.ctypes.data, len(st),
+                              lv.ctypes.data)
+    return lv, (lv != 0).sum(axis=1).astype(np.int64)
 
 
 def redispatch_slot(width, S, m, i):

@@ -381,3 +381,156 @@ def test_unsupported_tuning_is_rejected():
         with pytest.raises(G.GpmaError) as e:
             G.GPMA(64, tuning=bad)
         assert e.value.code == G.GPMA_ERR_ARG
+
+
+def _chain_graph():
+    """A 70,000-vertex chain 0 -> 1 -> ... -> 69,999 (depth 70,000 from 0)
+    with 2,000 side branches (chain vertex 35k -> side vertex 70,000 + k ->
+    chain vertex 35k + 1, never a shortcut) and 500 backward edges."""
+    n_chain, n_side = 70000, 2000
+    N = n_chain + n_side
+    c = np.arange(n_chain - 1, dtype=np.uint32)
+    k = np.arange(n_side, dtype=np.uint32)
+    rng = np.random.default_rng(70)
+    a = rng.integers(1, n_chain, 500).astype(np.uint32)
+    b = (a - rng.integers(1, 1000, 500).clip(max=a)).astype(np.uint32)
+    src = np.concatenate([c, 35 * k, n_chain + k, a])
+    dst = np.concatenate([c + 1, n_chain + k, np.minimum(35 * k + 1, n_chain - 1), b])
+    keys = np.unique(G.make_keys(src, dst))
+    keys = keys[(keys >> np.uint64(32)) != (keys & np.uint64(0xFFFFFFFF))]
+    return N, keys
+
+
+def _edges(keys):
+    return (keys >> np.uint64(32)).astype(np.uint32), (keys & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+
+
+@pytest.mark.timeout(600)
+def test_multi_source_bfs_deeper_than_u16_levels():
+    """Sources whose BFS depth passes 65,535 levels: the 64-source traversal
+    (gpma_bfs_multi), the captured step (gpma_step, whose graph holds at most
+    64 levels and continues on the host path) and a 2-request server group
+    must all equal the oracle's textbook BFS, and gpma_bfs for a few sources."""
+    N, keys = _chain_graph()
+    g = G.GPMA(N)
+    G.update_gpma(g, keys, np.ones(len(keys), np.uint8))
+    rng = np.random.default_rng(71)
+    starts = rng.integers(0, N, 64).astype(np.uint32)
+    starts[:6] = [0, 1, 70000, 35000, 69999, 71999]
+    starts[10] = starts[0]  # duplicate source
+    s, d = _edges(keys)
+    olv, oreach = O.bfs_edges_multi(N, s, d, starts)
+    assert olv[0].max() == 70000
+    lv, reach = G.gpma_bfs_multi(g, starts)
+    for i in range(64):
+        assert np.array_equal(lv[i], olv[i]) and reach[i] == oreach[i], f"source {i} ({starts[i]})"
+    for i in (0, 2, 5):
+        one, r1 = G.gpma_bfs(g, int(starts[i]))
+        assert np.array_equal(one, lv[i]) and r1 == reach[i], f"gpma_bfs source {i}"
+    # captured step: delete 100 backward edges, insert 100 new ones
+    back = keys[(keys >> np.uint64(32)) > (keys & np.uint64(0xFFFFFFFF))][:100]
+    a = rng.integers(2000, 69000, 100).astype(np.uint64)
+    new = np.setdiff1d((a << np.uint64(32)) | (a - np.uint64(7)), keys)
+    bk = np.concatenate([new, back])
+    bv = np.concatenate([np.ones(len(new), np.uint8), np.zeros(len(back), np.uint8)])
+    rc, lv2, reach2 = G.gpma_step(g, bk, bv, np.array([0, len(bk)], np.uint64), starts, return_levels=True)
+    assert rc == G.GPMA_OK
+    keys2 = np.union1d(np.setdiff1d(keys, back), new)
+    s2, d2 = _edges(keys2)
+    olv2, oreach2 = O.bfs_edges_multi(N, s2, d2, starts)
+    assert np.array_equal(lv2, olv2) and np.array_equal(reach2, oreach2)
+    assert G.gpma_step_stats(g)["bfs_tails"] + G.gpma_step_stats(g)["host_steps"] >= 1
+    # server: a 2-request BFS group behind a long no-op update (deletes of absent keys)
+    i = np.arange(1_000_000, dtype=np.uint64)
+    absent = np.setdiff1d(((i % np.uint64(N)) << np.uint64(32)) | (np.uint64(N - 1) - i // np.uint64(N)), keys2)
+    assert len(absent) > 900_000
+    grouped = False
+    for attempt in range(3):
+        with G.GpmaServer(g, max_bfs_group=2, order="snapshot") as srv:
+            tu = srv.submit_update(absent, np.zeros(len(absent), np.uint8))
+            tb = [srv.submit_bfs(0), srv.submit_bfs(1)]
+            assert srv.wait(tu, timeout_s=120)["status"] == G.GPMA_OK
+            res = [srv.wait(t, timeout_s=300) for t in tb]
+        for r, i in zip(res, (0, 1)):
+            assert r["status"] == G.GPMA_OK
+            assert np.array_equal(r["levels"], olv2[i]) and r["reachable"] == oreach2[i], f"server source {i}"
+        grouped = all(r["group"] == 2 for r in res)
+        if grouped:
+            break
+    assert grouped, "the two BFS requests never shared a traversal"
+    g.close()
+
+
+def test_rebuild_allocation_failure_keeps_the_handle_usable():
+    """A bulk rebuild whose new arrays cannot be allocated: (a) the new arrays
+    do not fit beside the old ones, so the old go first and the rebuild
+    succeeds; (b) they never fit: the live items stay at the old capacity,
+    GPMA_ERR_OOM is returned, and the handle stays usable (re-applying the
+    batch gives the closed form); (c) not even the old shape can be restored:
+    every later call returns GPMA_ERR_OOM."""
+    N = 3000
+    src, dst = G.gen_rmat(12, 120000, seed=29)
+    m = (src < N) & (dst < N)
+    keys = G.make_keys(src[m], dst[m])
+    guards = (np.arange(N, dtype=np.uint64) << np.uint64(32)) | np.uint64(0xFFFFFFFF)
+    kmax = np.array([G.KEY_MAX, G.KEY_MAX], np.uint64)
+
+    def expect(k):
+        return np.sort(np.concatenate([k, guards, kmax]))
+
+    def prepared(first, n_big):
+        g = G.GPMA(N, bulk_threshold=5000)
+        G.update_gpma(g, keys[:first], np.ones(first, np.uint8))
+        # size every scratch buffer the big batch needs (a no-op batch of absent deletes) and the
+        # rebuild's compaction buffer, so the injected failures hit the new arrays
+        absent = (np.arange(n_big, dtype=np.uint64) % np.uint64(N) << np.uint64(32)) | np.uint64(N - 1)
+        absent = np.setdiff1d(absent, keys)[:n_big]
+        absent = np.resize(absent, n_big)
+        G.update_gpma(g, absent, np.zeros(n_big, np.uint8))
+        g.export_live()
+        return g
+
+    first, big = 2000, 30000
+    batch = keys[first: first + big]
+    ones = np.ones(len(batch), np.uint8)
+    # (a)
+    g = prepared(first, len(batch))
+    c0 = g.get_size()
+    g.test_fail_allocs(0, 1)
+    G.update_gpma(g, batch, ones)
+    g.test_fail_allocs(0, 0)
+    assert g.get_size() > c0 and g.stats()["bulk_loads"] == 1
+    assert np.array_equal(g.export_live(), expect(keys[: first + big]))
+    assert_invariants(g)
+    g.close()
+    # (b)
+    g = prepared(first, len(batch))
+    g.test_fail_allocs(0, 2)
+    with pytest.raises(G.GpmaError) as e:
+        G.update_gpma(g, batch, ones)
+    assert e.value.code == G.GPMA_ERR_OOM
+    g.test_fail_allocs(0, 0)
+    assert g.get_size() == c0
+    live = g.export_live()
+    assert np.all(np.isin(expect(keys[:first]), live)), "a live key was lost"
+    assert_invariants(g)
+    G.update_gpma(g, batch, ones)
+    assert np.array_equal(g.export_live(), expect(keys[: first + big]))
+    lv, reach = G.gpma_bfs(g, 0)
+    s, d = _edges(keys[: first + big])
+    olv, oreach = O.bfs_edges(N, s, d, 0)
+    assert reach == oreach and np.array_equal(lv, olv)
+    g.close()
+    # (c)
+    g = prepared(first, len(batch))
+    g.test_fail_allocs(0, 3)
+    with pytest.raises(G.GpmaError) as e:
+        G.update_gpma(g, batch, ones)
+    assert e.value.code == G.GPMA_ERR_OOM
+    g.test_fail_allocs(0, 0)
+    for call in (lambda: G.update_gpma(g, batch[:10], ones[:10]), lambda: G.gpma_bfs(g, 0), g.export_live,
+                 g.live_digest):
+        with pytest.raises(G.GpmaError) as e:
+            call()
+        assert e.value.code == G.GPMA_ERR_OOM
+    g.close()

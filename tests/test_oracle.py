@@ -154,6 +154,26 @@ def test_bfs_matches_scipy_shortest_paths():
     assert np.array_equal(lv2, lv) and reach2 == reach
 
 
+def test_multi_start_bfs_matches_scipy_and_chain_depth():
+    """bfs_edges_multi (one adjacency, many starts) against scipy's shortest
+    paths for several starts, and the closed form of a chain deeper than
+    65,535 levels (level of chain vertex i from 0 is i + 1)."""
+    N = 3000
+    src, dst = _rmat(N, 12, 40000, 17)
+    starts = np.array([0, 5, 5, 2999, 1234], np.uint32)
+    lv, reach = O.bfs_edges_multi(N, src, dst, starts)
+    A = scipy.sparse.csr_matrix((np.ones(len(src)), (src, dst)), shape=(N, N))
+    d = scipy.sparse.csgraph.shortest_path(A, method="D", unweighted=True, indices=starts.astype(np.int64))
+    expect = np.where(np.isinf(d), 0, d + 1).astype(np.uint32)
+    assert np.array_equal(lv, expect)
+    assert reach.tolist() == np.count_nonzero(expect, axis=1).tolist()
+    n = 70000
+    c_src = np.arange(n - 1, dtype=np.uint32)
+    lvc, rc = O.bfs_edges_multi(n, c_src, c_src + 1, np.array([0, 60000], np.uint32))
+    assert np.array_equal(lvc[0], np.arange(1, n + 1, dtype=np.uint32)) and rc[0] == n
+    assert lvc[1][59999] == 0 and lvc[1][n - 1] == n - 60000 and rc[1] == n - 60000
+
+
 @pytest.mark.parametrize("name", ["tiny_rmat8_seed4.npz", "small_rmat12_seed2.npz", "c1_rmat16_seed1.npz"])
 def test_oracle_against_reference_cpu_baseline_golden(name):
     fx = np.load(os.path.join(GOLDEN, name))
