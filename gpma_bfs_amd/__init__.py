@@ -363,6 +363,20 @@ def gpma_csr_bfs(keys_ptr, values_ptr, row_offsets_ptr, node_size, edge_size, st
     return out
 
 
+NOALLOC_MAX_NODES = 1 << 21
+
+
+def gpma_csr_bfs_noalloc(keys_ptr, values_ptr, row_offsets_ptr, node_size, edge_size, start_node, levels_ptr,
+                         stream=None):
+    """gpma_csr_bfs that allocates nothing: levels go to the device buffer at
+    levels_ptr (node_size uint32), scratch is the library's static per-device
+    array (node_size <= NOALLOC_MAX_NODES). Returns the reachable count."""
+    reach = ctypes.c_uint32()
+    check(lib.gpma_csr_bfs_noalloc(keys_ptr, values_ptr, row_offsets_ptr, node_size, edge_size, start_node,
+                                   levels_ptr, ctypes.byref(reach), stream), "gpma_csr_bfs_noalloc")
+    return reach.value
+
+
 def gen_rmat(scale, ndraws, a=0.57, b=0.19, c=0.19, seed=1):
     """Seeded R-MAT edge list: unique, no self-loops, shuffled (src, dst uint32)."""
     n = ctypes.c_uint64()

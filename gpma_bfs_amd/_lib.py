@@ -155,6 +155,7 @@ SIGNATURES = {
     "gpma_get_step_stats": (_I, [_P, _P]),
     "gpma_step_prepare": (_I, [_P, _U64, _U32, _P, _U32]),
     "gpma_csr_bfs": (_I, [_P, _P, _P, _U32, _U32, _U32, _P]),
+    "gpma_csr_bfs_noalloc": (_I, [_P, _P, _P, _U32, _U32, _U32, _P, _P, _P]),
     "gpma_get_csr": (_I, [_P, _P]),
     "gpma_copy_layout": (_I, [_P, _P, _P, _P]),
     "gpma_export_live": (_I, [_P, _P, _U64, _P]),

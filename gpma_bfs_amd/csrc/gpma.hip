@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1266,6 +1267,51 @@ int step_host_path(gpma_handle *g, const uint64_t *keys, const uint8_t *values, 
     return urc;
 }
 
+// ---- BFS without allocation (gpma_csr_bfs_noalloc)
+//
+// The frontier queues and per-level counts of one traversal live in the
+// file-scope device array `scratch`, sized for node_size <= NOALLOC_MAX_NODES:
+// two count arrays of NA_CNT u64 (indexed by level), then four frontier
+// queues of NA_Q u32. Every GPU the library runs on holds its own copy (64 MiB)
+// from module load to process end. The traversal clears what it reads before
+// reading it (k_bfs_init: levels and the first counts; k_bfs_clear: each
+// chunk's counts), so the contents a previous call left do not matter. Calls
+// that overlap on one GPU share the array and race.
+constexpr uint32_t NOALLOC_MAX_NODES = 1u << 21;
+constexpr uint64_t NA_CNT = (uint64_t)NOALLOC_MAX_NODES + 1 + BFS_MAX_CHUNK + 2;
+constexpr uint64_t NA_Q = (uint64_t)NOALLOC_MAX_NODES + 1;
+constexpr uint64_t NA_WORDS64 = 2 * NA_CNT + (4 * NA_Q + 1) / 2;
+
+}  // namespace
+
+static __device__ __align__(256) unsigned long long scratch[NA_WORDS64];
+
+namespace {
+
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_dev[NA_MAX_DEVICES] = {};  // device address of `scratch` per device, once looked up
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current. Host code uses only this address: the
+// array's own name is a host-side stand-in.
+int noalloc_scratch(void **out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NA_MAX_DEVICES)
+        return fail(GPMA_ERR_NO_DEVICE, "gpma_csr_bfs_noalloc: device ordinal " + std::to_string(dev) + " too large");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(GPMA_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return GPMA_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1576,6 +1622,37 @@ int gpma_csr_bfs(const uint64_t *keys, const uint8_t *values, const uint32_t *ro
     if (hc) (void)hipHostFree(hc);
     (void)hipStreamDestroy(st);
     return rc;
+}
+
+int gpma_csr_bfs_noalloc(const uint64_t *keys, const uint8_t *values, const uint32_t *row_offsets, uint32_t node_size,
+                         uint32_t edge_size, uint32_t start_node, uint32_t *levels, uint32_t *reachable,
+                         void *stream) {
+    (void)edge_size;
+    if (!keys || !values || !row_offsets || !levels)
+        return fail(GPMA_ERR_ARG, "gpma_csr_bfs_noalloc: null pointer");
+    if (start_node >= node_size) return fail(GPMA_ERR_ARG, "gpma_csr_bfs_noalloc: start_node >= node_size");
+    if (node_size > NOALLOC_MAX_NODES)
+        return fail(GPMA_ERR_CAPACITY, "gpma_csr_bfs_noalloc: node_size " + std::to_string(node_size) +
+                                           " exceeds the preallocated maximum of " +
+                                           std::to_string(NOALLOC_MAX_NODES) + " nodes");
+    void *base = nullptr;
+    TRY(noalloc_scratch(&base));
+    unsigned long long *cnt = (unsigned long long *)base;
+    uint32_t *q = (uint32_t *)(cnt + 2 * NA_CNT);
+    BfsState b;
+    b.levels = levels;  // the caller's output doubles as the claim words
+    b.cnt = cnt;
+    b.cnt_big = cnt + NA_CNT;
+    b.q[0] = q;
+    b.q[1] = q + NA_Q;
+    b.qbig[0] = q + 2 * NA_Q;
+    b.qbig[1] = q + 3 * NA_Q;
+    unsigned long long hc[2 * (BFS_MAX_CHUNK + 1)];  // per-chunk counts read back into host stack memory
+    uint32_t reach = 0;
+    // the caller's array length is unknown here, so only the push kernel runs
+    TRY(run_bfs(keys, values, 0, row_offsets, node_size, start_node, b, hc, (hipStream_t)stream, &reach, 0.0));
+    if (reachable) *reachable = reach;
+    return GPMA_OK;
 }
 
 int gpma_get_csr(gpma_t g, gpma_csr *out) {

@@ -178,6 +178,21 @@ int gpma_step_prepare(gpma_t g, uint64_t max_records, uint32_t nsrc, uint32_t *l
 int gpma_csr_bfs(const uint64_t *keys, const uint8_t *values, const uint32_t *row_offsets, uint32_t node_size,
                  uint32_t edge_size, uint32_t start_node, uint32_t *results);
 
+/* gpma_csr_bfs without any allocation (no reference counterpart: for
+ * callers that cannot allocate device or pinned memory, such as tight
+ * serving loops and harnesses that forbid hipMalloc). Same traversal and
+ * results as gpma_csr_bfs, but levels is a DEVICE array of node_size entries
+ * written on `stream` (a hipStream_t; NULL = the default stream), and
+ * *reachable (host, may be NULL) gets the reachable count. The frontier
+ * queues and level counts live in a static per-device array sized for at
+ * most 2,097,152 nodes (64 MiB, held by every GPU the library runs on until
+ * the process ends); larger node_size returns GPMA_ERR_CAPACITY before any
+ * launch. The call returns after the traversal finished. Calls that overlap
+ * on one GPU share that array and race: serialise them. */
+int gpma_csr_bfs_noalloc(const uint64_t *keys, const uint8_t *values, const uint32_t *row_offsets,
+                         uint32_t node_size, uint32_t edge_size, uint32_t start_node, uint32_t *levels,
+                         uint32_t *reachable, void *stream);
+
 /* Public members of the reference's GPMA (gpma.cuh:31-52): device pointers
  * to keys, values, row_offset plus segment_length, tree_height, row_num. */
 int gpma_get_csr(gpma_t g, gpma_csr *out);
