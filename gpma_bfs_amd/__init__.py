@@ -11,6 +11,7 @@ may be numpy arrays (host) or torch tensors on the GPU (passed by device
 pointer). Everything runs through libgpma.so's C ABI; there is no CPU path.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -409,6 +410,29 @@ def combine_digests(*ds):
     """Digest of the concatenation of key sets with the given digests."""
     return (sum(d[0] for d in ds), sum(d[1] for d in ds) % (1 << 64),
             __import__("functools").reduce(lambda a, b: a ^ b, [d[2] for d in ds], 0))
+
+
+def read_edge_text(path):
+    """(N, src, dst) from an "N E" + "src dst" text file (the reference's
+    dataset format)."""
+    n, e = ctypes.c_uint32(), ctypes.c_uint64()
+    p = os.fsencode(path)
+    check(lib.gpma_read_edge_text(p, ctypes.byref(n), ctypes.byref(e), None, None, 0), "gpma_read_edge_text")
+    src = np.empty(e.value, np.uint32)
+    dst = np.empty(e.value, np.uint32)
+    check(lib.gpma_read_edge_text(p, ctypes.byref(n), ctypes.byref(e), src.ctypes.data, dst.ctypes.data, e.value),
+          "gpma_read_edge_text")
+    return n.value, src, dst
+
+
+def write_edge_text(path, N, src, dst):
+    """Write an edge list in the reference's "N E" + "src dst" text format."""
+    src = np.ascontiguousarray(src, np.uint32)
+    dst = np.ascontiguousarray(dst, np.uint32)
+    if len(src) != len(dst):
+        raise GpmaError(GPMA_ERR_ARG, "src and dst differ in length")
+    check(lib.gpma_write_edge_text(os.fsencode(path), N, len(src), src.ctypes.data, dst.ctypes.data),
+          "gpma_write_edge_text")
 
 
 def make_keys(src, dst):

@@ -183,6 +183,8 @@ SIGNATURES = {
     "gpma_server_get_stats": (_I, [_P, _P]),
     "gpma_gen_rmat": (_I, [_U32, _U64, ctypes.c_double, ctypes.c_double, ctypes.c_double, _U64, _P, _P, _U64, _P]),
     "gpma_gen_powerlaw": (_I, [_U32, _U64, ctypes.c_double, ctypes.c_double, ctypes.c_double, _U64, _P, _P]),
+    "gpma_read_edge_text": (_I, [ctypes.c_char_p, _P, _P, _P, _P, _U64]),
+    "gpma_write_edge_text": (_I, [ctypes.c_char_p, _U32, _U64, _P, _P]),
 }
 
 

@@ -12,6 +12,8 @@
 #include <cstdint>
 #include <cstring>
 #include <atomic>
+#include <cstdio>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -195,9 +197,127 @@ void shuffle_binned(std::vector<uint64_t> &keys, uint64_t seed) {
     keys.swap(tmp);
 }
 
+// Parser of whitespace-separated unsigned decimal integers over a buffer.
+struct Scanner {
+    const char *p, *e;
+    // next integer (false at the end of the buffer or on a non-digit token)
+    bool next(uint64_t *out, bool *bad) {
+        while (p < e && (*p == ' ' || *p == '\t' || *p == '\n' || *p == '\r')) p++;
+        if (p == e) return false;
+        if (*p < '0' || *p > '9') {
+            *bad = true;
+            return false;
+        }
+        uint64_t v = 0;
+        while (p < e && *p >= '0' && *p <= '9') {
+            v = v * 10 + (uint64_t)(*p - '0');
+            if (v > 0xFFFFFFFFFFull) {
+                *bad = true;
+                return false;
+            }
+            p++;
+        }
+        if (p < e && !(*p == ' ' || *p == '\t' || *p == '\n' || *p == '\r')) {
+            *bad = true;
+            return false;
+        }
+        *out = v;
+        return true;
+    }
+};
+
+bool read_file(const char *path, std::string *buf) {
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return false;
+    char tmp[1 << 16];
+    size_t n;
+    while ((n = std::fread(tmp, 1, sizeof(tmp), f)) > 0) buf->append(tmp, n);
+    const bool ok = !std::ferror(f);
+    std::fclose(f);
+    return ok;
+}
+
 }  // namespace
 
+namespace gpma {
+void set_last_error(const std::string &msg);  // gpma.hip
+}
+
 extern "C" {
+
+// The reference's graph text format (dataset/prepare_dataset.sh:34-37, read
+// by gpma_bfs_demo.cu:6-30): a header "N E", then E lines "src dst" with
+// 0-based node ids. Reads N and E, and with src and dst given (cap >= E
+// entries) the edges in file order. Ids must be < 2^32.
+int gpma_read_edge_text(const char *path, uint32_t *N, uint64_t *E, uint32_t *src, uint32_t *dst, uint64_t cap) {
+    if (!path || !N || !E) {
+        gpma::set_last_error("gpma_read_edge_text: null argument");
+        return GPMA_ERR_ARG;
+    }
+    std::string buf;
+    if (!read_file(path, &buf)) {
+        gpma::set_last_error(std::string("gpma_read_edge_text: cannot read ") + path);
+        return GPMA_ERR_ARG;
+    }
+    Scanner sc{buf.data(), buf.data() + buf.size()};
+    bool bad = false;
+    uint64_t n = 0, e = 0;
+    if (!sc.next(&n, &bad) || !sc.next(&e, &bad) || n > 0xFFFFFFFFull) {
+        gpma::set_last_error("gpma_read_edge_text: missing or bad \"N E\" header");
+        return GPMA_ERR_ARG;
+    }
+    *N = (uint32_t)n;
+    *E = e;
+    if (!src || !dst) return GPMA_OK;
+    if (cap < e) {
+        gpma::set_last_error("gpma_read_edge_text: buffers hold " + std::to_string(cap) + " edges, file has " +
+                             std::to_string(e));
+        return GPMA_ERR_CAPACITY;
+    }
+    for (uint64_t i = 0; i < e; i++) {
+        uint64_t a = 0, b = 0;
+        if (!sc.next(&a, &bad) || !sc.next(&b, &bad) || a > 0xFFFFFFFFull || b > 0xFFFFFFFFull) {
+            gpma::set_last_error("gpma_read_edge_text: edge " + std::to_string(i) + " missing or malformed");
+            return GPMA_ERR_ARG;
+        }
+        src[i] = (uint32_t)a;
+        dst[i] = (uint32_t)b;
+    }
+    return GPMA_OK;
+}
+
+// Writes the same format: "N E" then one "src dst" line per edge.
+int gpma_write_edge_text(const char *path, uint32_t N, uint64_t E, const uint32_t *src, const uint32_t *dst) {
+    if (!path || (E && (!src || !dst))) {
+        gpma::set_last_error("gpma_write_edge_text: null argument");
+        return GPMA_ERR_ARG;
+    }
+    FILE *f = std::fopen(path, "wb");
+    if (!f) {
+        gpma::set_last_error(std::string("gpma_write_edge_text: cannot open ") + path);
+        return GPMA_ERR_ARG;
+    }
+    std::string out;
+    out.reserve(1 << 20);
+    out += std::to_string(N) + " " + std::to_string(E) + "\n";
+    bool ok = true;
+    for (uint64_t i = 0; i < E && ok; i++) {
+        char line[32];
+        const int k = std::snprintf(line, sizeof(line), "%u %u\n", src[i], dst[i]);
+        out.append(line, (size_t)k);
+        if (out.size() >= (1u << 20)) {
+            ok = std::fwrite(out.data(), 1, out.size(), f) == out.size();
+            out.clear();
+        }
+    }
+    if (ok && !out.empty()) ok = std::fwrite(out.data(), 1, out.size(), f) == out.size();
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) {
+        gpma::set_last_error(std::string("gpma_write_edge_text: write failed: ") + path);
+        return GPMA_ERR_ARG;
+    }
+    return GPMA_OK;
+}
 
 // R-MAT (Chakrabarti, Zhan, Faloutsos 2004) without noise: each of `ndraws`
 // draws descends `scale` levels, MSB first, picking quadrant (0,0) with

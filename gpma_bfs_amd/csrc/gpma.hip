@@ -158,6 +158,11 @@ constexpr uint32_t SPEC_LEVELS = 3;
 
 }  // namespace
 
+namespace gpma {
+// Sets gpma_last_error() for this thread (host code outside this file).
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+}  // namespace gpma
+
 struct gpma_handle {
     int device = 0;
     hipStream_t st = nullptr;

@@ -12,21 +12,25 @@
 
 #include "gpma.hpp"
 
+// gpma_bfs_demo.cu:6-30, through the library's reader of the same format
 static void load_data(const char *path, std::vector<int> &xs, std::vector<int> &ys, int &node_size, int &edge_size) {
-    FILE *fp = fopen(path, "r");
-    if (!fp) {
+    uint32_t n = 0;
+    uint64_t e = 0;
+    if (gpma_read_edge_text(path, &n, &e, nullptr, nullptr, 0) != GPMA_OK) {
         printf("Open graph file failed.\n");
         exit(0);
     }
-    if (fscanf(fp, "%d %d", &node_size, &edge_size) != 2) exit(1);
+    node_size = (int)n;
+    edge_size = (int)e;
     printf("node_num: %d, edge_num: %d\n", node_size, edge_size);
-    xs.resize(edge_size);
-    ys.resize(edge_size);
-    for (int i = 0; i < edge_size; i++) {
-        if (fscanf(fp, "%d %d", &xs[i], &ys[i]) != 2) exit(1);
+    std::vector<uint32_t> s(e), d(e);
+    if (gpma_read_edge_text(path, &n, &e, s.data(), d.data(), e) != GPMA_OK) {
+        fprintf(stderr, "%s\n", gpma_last_error());
+        exit(1);
     }
+    xs.assign(s.begin(), s.end());
+    ys.assign(d.begin(), d.end());
     printf("Graph file is loaded.\n");
-    fclose(fp);
 }
 
 static long count_reach(const std::vector<SIZE_TYPE> &r) {

@@ -273,6 +273,14 @@ int gpma_gen_rmat(uint32_t scale, uint64_t ndraws, double a, double b, double c,
 int gpma_gen_powerlaw(uint32_t N, uint64_t E, double alpha, double soft_cap, double p1, uint64_t seed, uint32_t *src,
                       uint32_t *dst);
 
+/* The reference's graph text format (dataset/prepare_dataset.sh:34-37, read
+ * by gpma_bfs_demo.cu:6-30): a header "N E", then E lines "src dst" with
+ * 0-based node ids. gpma_read_edge_text stores N and E, and with src and dst
+ * non-NULL (cap >= E entries each) the edges in file order;
+ * gpma_write_edge_text writes that format. */
+int gpma_read_edge_text(const char *path, uint32_t *N, uint64_t *E, uint32_t *src, uint32_t *dst, uint64_t cap);
+int gpma_write_edge_text(const char *path, uint32_t N, uint64_t E, const uint32_t *src, const uint32_t *dst);
+
 #ifdef __cplusplus
 }
 #endif

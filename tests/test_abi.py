@@ -112,3 +112,29 @@ def test_bench_rejects_multi_gpu():
     assert p.returncode == 2
     assert "one GPU" in p.stderr
 
+
+
+def test_edge_text_format_round_trip_and_errors(tmp_path):
+    """The reference's "N E" + "src dst" text format: a hand-written file
+    (tabs, CRLF, blank lines) reads back as written; the writer's output
+    equals numpy's own rendering and reads back; malformed files and short
+    buffers are errors."""
+    p = tmp_path / "g.txt"
+    p.write_bytes(b"5 4\r\n0 1\r\n1\t2\n\n3 4\n  4 0\n")
+    n, s, d = G.read_edge_text(p)
+    assert n == 5 and s.tolist() == [0, 1, 3, 4] and d.tolist() == [1, 2, 4, 0]
+    src, dst = G.gen_rmat(10, 3000, seed=11)
+    q = tmp_path / "r.txt"
+    G.write_edge_text(q, 1024, src, dst)
+    lines = q.read_text().splitlines()
+    assert lines[0] == f"1024 {len(src)}"
+    assert lines[1:] == [f"{a} {b}" for a, b in zip(src.tolist(), dst.tolist())]
+    n, s2, d2 = G.read_edge_text(q)
+    assert n == 1024 and np.array_equal(s2, src) and np.array_equal(d2, dst)
+    for bad in (b"", b"5", b"x 3\n", b"5 3\n0 1\n1 2\n", b"5 2\n0 1\n1 -2\n", b"5 1\n0 99999999999\n"):
+        p.write_bytes(bad)
+        with pytest.raises(G.GpmaError) as e:
+            G.read_edge_text(p)
+        assert e.value.code == G.GPMA_ERR_ARG
+    with pytest.raises(G.GpmaError):
+        G.read_edge_text(tmp_path / "missing.txt")

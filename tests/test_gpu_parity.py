@@ -6,6 +6,7 @@ Live-key sets equal the closed form of the sliding window; BFS levels equal
 the oracle and the reference's own cpu_baseline fixtures.
 """
 import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -93,6 +94,24 @@ def test_c1_protocol():
     assert_invariants(g)
     lv, reach = G.gpma_bfs(g, 0)
     assert np.array_equal(lv, fx["levels_after"]) and reach == int(fx["reach_after"])
+
+
+def test_reference_demo_protocol_through_the_facade(tmp_path):
+    """The reference demo's protocol (gpma_bfs_demo.cu:6-111), compiled against
+    the C++ facade (GPMA_multidev<c,1>::update_batch, gpma_bfs(multidev, ...))
+    and run on C1's graph written as an "N E" text file: its printed
+    reachable counts before and after the 100 slides equal the reference
+    cpu_baseline fixture."""
+    demo = os.path.join(os.path.dirname(GOLDEN), "..", "gpma_bfs_amd", "demo", "gpma_bfs_demo")
+    src, dst = G.gen_rmat(16, 1 << 20, seed=1)
+    f = tmp_path / "c1.txt"
+    G.write_edge_text(f, 1 << 16, src, dst)
+    p = subprocess.run([demo, str(f), "0"], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=300)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert f"node_num: {1 << 16}, edge_num: {len(src)}" in p.stdout
+    reach = [int(line.rsplit(":", 1)[1]) for line in p.stdout.splitlines() if "number of reachable nodes" in line]
+    fx = np.load(os.path.join(GOLDEN, "c1_rmat16_seed1.npz"))
+    assert reach == [int(fx["reach_before"]), int(fx["reach_after"])], p.stdout
 
 
 def test_update_semantics_edge_cases():
