@@ -24,8 +24,11 @@ Beside it, in the same JSON line: the batched form (8 fused uneven batches
 and a 64-source BFS per step, through the host path and as captured HIP
 graphs), the same steps in the plain form (each batch through update_gpma,
 then 64 single-source BFS), and the reference's GPMA CPU path (restated in
-oracle/) on every host core. --large, --serve and --batched-only run the
-other configurations instead.
+oracle/) on every host core, and a serving leg (R-MAT 2^24 nodes / 2^28
+draws through the request server: 10^4 requests open-loop at 5,000/s, then
+10^4 submitted at once for the saturation rate; p50/p99 latency per request
+class). --large, --serve and --batched-only run the other configurations
+instead.
 
 Prints progress on stderr and ONE JSON line on stdout.
 """
@@ -487,19 +490,24 @@ def run_large(G, args):
     }
 
 
-def run_serve(G, args):
+def run_serve(G, requests, rate, order="snapshot", saturation=True, seed=4):
     """Serving: R-MAT scale 24 (2^24 nodes), 2^28 draws (seed 4), first half
-    loaded; then an open-loop Poisson stream of requests through the request
-    server: 70% update batches (log-uniform 1..1e5 records, half inserts of
-    new edges, half deletes of the oldest, a circular sliding window over the
+    loaded; then streams of `requests` requests through the request server:
+    70% update batches (log-uniform 1..1e5 records, half inserts of new
+    edges, half deletes of the oldest, a circular sliding window over the
     edge list, staged in HBM beforehand), 30% single-source BFS from uniform
-    sources. Reports p50/p99 latency per request class and sustained rates."""
+    sources. The first stream arrives open-loop as a Poisson process at
+    `rate` requests/s; with `saturation` a second stream (continuing the
+    window) is submitted all at once, which measures the sustained rate the
+    server reaches when it is never idle. Reports p50/p99 latency per request
+    class and sustained rates; parity: every status, the live set against
+    the closed form (device digest) and the last BFS answers against gpma_bfs."""
     import threading
 
     scale, draws = 24, 1 << 28
     N = 1 << scale
     t0 = time.time()
-    src, dst = G.gen_rmat(scale, draws, seed=4)
+    src, dst = G.gen_rmat(scale, draws, seed=seed)
     keys = G.make_keys(src, dst)
     del src, dst
     E = len(keys)
@@ -508,106 +516,116 @@ def run_serve(G, args):
     g = G.GPMA(N)
     G.update_gpma(g, keys[:half], np.ones(half, np.uint8))
     g.sync()
-    rng = np.random.default_rng(4)
-    R = args.serve_requests
-    is_upd = rng.random(R) < 0.7
-    sizes = np.where(is_upd, np.exp(rng.uniform(0, np.log(1e5), R)).astype(np.int64), 0)
-    sizes = (sizes // 2) * 2
-    sizes[is_upd & (sizes == 0)] = 2
-    starts = rng.integers(0, N, R).astype(np.uint32)
-    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, R))
-    total = int(sizes.sum())
-    # update stream: request j inserts edges[ins : ins + h] and deletes edges[dele : dele + h] (indices mod E)
-    stream = np.empty(total, np.uint64)
-    vals = np.empty(total, np.uint8)
-    offs = np.zeros(R + 1, np.int64)
-    offs[1:] = np.cumsum(sizes)
+    rng = np.random.default_rng(seed)
+    passes = [("open_loop", rate)] + ([("saturation", None)] if saturation else [])
+    R = requests
+
     def circ(a, h):  # keys[a .. a + h) over the circular edge list
         a %= E
         return keys[a: a + h] if a + h <= E else np.concatenate([keys[a:], keys[: a + h - E]])
 
     ins, dele = half, 0
-    for j in np.nonzero(is_upd)[0]:
-        h = int(sizes[j] // 2)
-        a = int(offs[j])
-        stream[a: a + h] = circ(ins, h)
-        stream[a + h: a + 2 * h] = circ(dele, h)
-        vals[a: a + h] = 1
-        vals[a + h: a + 2 * h] = 0
-        ins += h
-        dele += h
-    dk = G.DeviceBuffer.from_numpy(stream)
-    dv = G.DeviceBuffer.from_numpy(vals)
-    del stream, vals
-    log(f"serve: {R} requests ({int(is_upd.sum())} updates, {total} records staged in HBM), "
-        f"open loop at {args.serve_rate:.0f} req/s")
-    tickets = [None] * R
-    results = [None] * R
-    srv = G.GpmaServer(g, order=args.serve_order)
-    done_evt = threading.Event()
+    out = {"workload": f"R-MAT scale 24 (N=2^24), 2^28 draws (seed {seed}) -> {E} edges, first half loaded; "
+                       f"{R} requests per stream: 70% update batches log-uniform 1..1e5 records (1:1 insert:delete, "
+                       "circular window, staged in HBM), 30% single-source BFS (uniform sources); server order "
+                       f"{order}; stream 1 Poisson arrivals at {rate:.0f}/s"
+                       + ("; stream 2 submitted at once (saturation)" if saturation else ""),
+           "requests_per_stream": R}
+    last_bfs = []
+    ok_all = True
+    for name, prate in passes:
+        is_upd = rng.random(R) < 0.7
+        sizes = np.where(is_upd, np.exp(rng.uniform(0, np.log(1e5), R)).astype(np.int64), 0)
+        sizes = (sizes // 2) * 2
+        sizes[is_upd & (sizes == 0)] = 2
+        starts = rng.integers(0, N, R).astype(np.uint32)
+        arrivals = np.cumsum(rng.exponential(1.0 / prate, R)) if prate else np.zeros(R)
+        total = int(sizes.sum())
+        stream = np.empty(total, np.uint64)
+        vals = np.empty(total, np.uint8)
+        offs = np.zeros(R + 1, np.int64)
+        offs[1:] = np.cumsum(sizes)
+        for j in np.nonzero(is_upd)[0]:
+            h = int(sizes[j] // 2)
+            a = int(offs[j])
+            stream[a: a + h] = circ(ins, h)
+            stream[a + h: a + 2 * h] = circ(dele, h)
+            vals[a: a + h] = 1
+            vals[a + h: a + 2 * h] = 0
+            ins += h
+            dele += h
+        dk = G.DeviceBuffer.from_numpy(stream)
+        dv = G.DeviceBuffer.from_numpy(vals)
+        del stream, vals
+        log(f"serve/{name}: {R} requests ({int(is_upd.sum())} updates, {total} records staged in HBM)")
+        tickets = [None] * R
+        results = [None] * R
+        ready = threading.Semaphore(0)
+        srv = G.GpmaServer(g, order=order)
 
-    def waiter():
+        def waiter():
+            for j in range(R):
+                ready.acquire()
+                results[j] = srv.wait(tickets[j])
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        lag = np.zeros(R)
+        t_start = time.perf_counter()
+        last_log = t_start
         for j in range(R):
-            while tickets[j] is None:
-                time.sleep(0.0002)
-            results[j] = srv.wait(tickets[j])
-        done_evt.set()
+            now = time.perf_counter() - t_start
+            if arrivals[j] > now:
+                time.sleep(arrivals[j] - now)
+            lag[j] = time.perf_counter() - t_start - arrivals[j]
+            if is_upd[j]:
+                tickets[j] = srv.submit_update(dk.ptr + int(offs[j]) * 8, dv.ptr + int(offs[j]), int(sizes[j]))
+            else:
+                tickets[j] = srv.submit_bfs(int(starts[j]), levels=False)
+            ready.release()
+            if time.perf_counter() - last_log > 20:
+                last_log = time.perf_counter()
+                log(f"serve/{name}: {j + 1}/{R} submitted")
+        th.join()
+        t_total = time.perf_counter() - t_start
+        sst = srv.stats()
+        srv.close()
+        dk.free()
+        dv.free()
+        lat = np.array([r["latency_us"] for r in results]) / 1e3
+        ok_all &= all(r["status"] == G.GPMA_OK for r in results)
+        upd_lat, bfs_lat = lat[is_upd], lat[~is_upd]
 
-    th = threading.Thread(target=waiter)
-    th.start()
-    lag = np.zeros(R)
-    t_start = time.perf_counter()
-    last_log = t_start
-    for j in range(R):
-        now = time.perf_counter() - t_start
-        if arrivals[j] > now:
-            time.sleep(arrivals[j] - now)
-        lag[j] = time.perf_counter() - t_start - arrivals[j]
-        if is_upd[j]:
-            tickets[j] = srv.submit_update(dk.ptr + int(offs[j]) * 8, dv.ptr + int(offs[j]), int(sizes[j]))
-        else:
-            tickets[j] = srv.submit_bfs(int(starts[j]), levels=False)
-        if time.perf_counter() - last_log > 20:
-            last_log = time.perf_counter()
-            log(f"serve: {j + 1}/{R} submitted")
-    th.join()
-    t_total = time.perf_counter() - t_start
-    sst = srv.stats()
-    srv.close()
-    dk.free()
-    dv.free()
-    lat = np.array([r["latency_us"] for r in results]) / 1e3
-    st_ok = all(r["status"] == G.GPMA_OK for r in results)
-    # parity: live set == circular window edges[dele .. ins) + guards + 2 KEY_MAX (count / sum / xor)
+        def pct(x, q):
+            return float(np.percentile(x, q)) if len(x) else None
+
+        out[name] = {
+            "offered_rate": prate, "seconds": t_total, "sustained_req_per_s": R / t_total,
+            "update_records_per_s": total / t_total, "bfs_per_s": int((~is_upd).sum()) / t_total,
+            "latency_ms": {"update_p50": pct(upd_lat, 50), "update_p99": pct(upd_lat, 99),
+                           "bfs_p50": pct(bfs_lat, 50), "bfs_p99": pct(bfs_lat, 99),
+                           "all_p50": pct(lat, 50), "all_p99": pct(lat, 99)},
+            "submit_lag_ms": {"p50": pct(lag * 1e3, 50), "p99": pct(lag * 1e3, 99)},
+            "server": sst,
+            "mean_update_fusion": sst["update_records"] / max(sst["update_launches"], 1),
+            "mean_bfs_group": sst["bfs_requests"] / max(sst["bfs_launches"], 1)}
+        log(f"serve/{name}: {R / t_total:.0f} req/s sustained, update p50/p99 "
+            f"{out[name]['latency_ms']['update_p50']:.2f}/{out[name]['latency_ms']['update_p99']:.2f} ms, BFS p50/p99 "
+            f"{out[name]['latency_ms']['bfs_p50']:.2f}/{out[name]['latency_ms']['bfs_p99']:.2f} ms; deferred "
+            f"{sst['deferred']}, rounds staged ahead {sst['staged_ahead']}")
+        # BFS requests after the stream's last update see the final graph
+        tail = [j for j in range(R) if not is_upd[j] and not is_upd[j:].any()]
+        last_bfs = [(int(starts[j]), results[j]["reachable"]) for j in tail[:8]]
     win = circ(dele, ins - dele)
     guards = (np.arange(N, dtype=np.uint64) << np.uint64(32)) | np.uint64(0xFFFFFFFF)
-    ce = [live_checksum(win), live_checksum(guards), live_checksum(np.array([G.KEY_MAX, G.KEY_MAX], np.uint64))]
-    expect = (sum(c[0] for c in ce), sum(c[1] for c in ce) % (1 << 64), ce[0][2] ^ ce[1][2] ^ ce[2][2])
-    ok_live = live_checksum(g.export_live()) == expect
-    # spot check: BFS requests after the last update see the final graph
-    tail = [j for j in range(R) if not is_upd[j] and not is_upd[j:].any()]
-    ok_bfs = all(G.gpma_bfs(g, int(starts[j]))[1] == results[j]["reachable"] for j in tail[:8]) if tail else None
+    expect = G.combine_digests(G.key_digest(win), G.key_digest(guards),
+                               G.key_digest(np.array([G.KEY_MAX, G.KEY_MAX], np.uint64)))
+    ok_live = g.live_digest() == expect
+    ok_bfs = all(G.gpma_bfs_count(g, s0) == r0 for s0, r0 in last_bfs) if last_bfs else None
     g.close()
-
-    def pct(x, q):
-        return float(np.percentile(x, q)) if len(x) else None
-
-    upd_lat, bfs_lat = lat[is_upd], lat[~is_upd]
-    return {
-        "workload": f"R-MAT scale 24 (N=2^24), 2^28 draws (seed 4) -> {E} edges, first half loaded; {R} requests, "
-                    f"Poisson arrivals at {args.serve_rate:.0f}/s: 70% update batches log-uniform 1..1e5 records "
-                    "(1:1 insert:delete, circular window, staged in HBM), 30% single-source BFS (uniform sources); "
-                    f"server order {args.serve_order}",
-        "requests": R, "seconds": t_total, "offered_rate": args.serve_rate, "sustained_req_per_s": R / t_total,
-        "update_records_per_s": total / t_total, "bfs_per_s": int((~is_upd).sum()) / t_total,
-        "latency_ms": {"update_p50": pct(upd_lat, 50), "update_p99": pct(upd_lat, 99),
-                       "bfs_p50": pct(bfs_lat, 50), "bfs_p99": pct(bfs_lat, 99), "all_p99": pct(lat, 99)},
-        "submit_lag_ms": {"p50": pct(lag * 1e3, 50), "p99": pct(lag * 1e3, 99)},
-        "server": sst,
-        "mean_update_fusion": sst["update_records"] / max(sst["update_launches"], 1),
-        "mean_bfs_group": sst["bfs_requests"] / max(sst["bfs_launches"], 1),
-        "parity": {"all_ok": st_ok, "live_set_count_sum_xor": ok_live, "last_bfs_reach": ok_bfs},
-    }
+    out["parity"] = {"all_ok": bool(ok_all), "live_set_digest_closed_form": bool(ok_live),
+                     "last_bfs_reach_equals_gpma_bfs": ok_bfs}
+    return out
 
 
 def main():
@@ -634,6 +652,8 @@ def main():
     ap.add_argument("--serve", action="store_true",
                     help="request server under an open-loop Poisson stream on R-MAT 2^24 nodes / 2^28 draws")
     ap.add_argument("--serve-requests", type=int, default=100000)
+    ap.add_argument("--serve-leg-requests", type=int, default=10000,
+                    help="requests per stream of the serving leg in the default run (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=5000.0)
     ap.add_argument("--serve-order", default="snapshot", choices=("strict", "snapshot"))
     args = ap.parse_args()
@@ -648,12 +668,12 @@ def main():
     import gpma_bfs_amd as G
 
     if args.serve:
-        r = run_serve(G, args)
-        log(f"serve: {r['sustained_req_per_s']:.0f} req/s sustained, update p50/p99 "
-            f"{r['latency_ms']['update_p50']:.2f}/{r['latency_ms']['update_p99']:.2f} ms, BFS p50/p99 "
-            f"{r['latency_ms']['bfs_p50']:.2f}/{r['latency_ms']['bfs_p99']:.2f} ms, parity {r['parity']}")
-        out = {"metric": METRIC, "value": r["update_records_per_s"], "unit": "edge-updates/s", "n_gpus": 1,
-               "steps": r["requests"], "warmup": 0, "ms_per_step": r["seconds"] / r["requests"] * 1e3,
+        r = run_serve(G, args.serve_requests, args.serve_rate, args.serve_order)
+        log(f"serve: parity {r['parity']}")
+        o = r["open_loop"]
+        out = {"metric": METRIC, "value": o["update_records_per_s"], "unit": "edge-updates/s", "n_gpus": 1,
+               "steps": r["requests_per_stream"], "warmup": 0,
+               "ms_per_step": o["seconds"] / r["requests_per_stream"] * 1e3,
                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
                "data": "synthetic", "config": {"workload": r["workload"], "form": "serving"}, "serve": r}
         print(json.dumps(out), flush=True)
@@ -942,6 +962,13 @@ def main():
         log(f"plain form on the same steps: {pf['edge_updates_per_s'] / 1e9:.3f} G edge-updates/s, step "
             f"{pf['step_ms']:.2f} ms; batched graph {sp['step']:.1f}x per step, {sp['update']:.2f}x on updates")
 
+    serve_leg = None
+    if args.serve_leg_requests > 0 and not args.small:
+        g.close()
+        log(f"serving leg: {args.serve_leg_requests} requests open-loop at {args.serve_rate:.0f}/s, then as many at once")
+        serve_leg = run_serve(G, args.serve_leg_requests, args.serve_rate, args.serve_order)
+        log(f"serving leg parity: {serve_leg['parity']}")
+
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -985,6 +1012,7 @@ def main():
         "bfs": bfs_info,
         "tuned_plain": tuned_plain,
         "batched": batched,
+        "serve": serve_leg,
         "latency_ms": {"p50": float(np.percentile(per_ms, 50)), "p99": float(np.percentile(per_ms, 99))},
         "protocol": protocol,
         "parity": parity,

@@ -136,6 +136,10 @@ class GpmaServerStats(ctypes.Structure):
         ("max_fused_records", ctypes.c_uint64),
         ("max_bfs_group", ctypes.c_uint32),
         ("reserved", ctypes.c_uint32),
+        ("deferred", ctypes.c_uint64),
+        ("staged_ahead", ctypes.c_uint64),
+        ("budget_bytes", ctypes.c_uint64),
+        ("growth_reserve_bytes", ctypes.c_uint64),
     ]
 
 

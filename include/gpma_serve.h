@@ -8,9 +8,12 @@
  * gets a ticket, and the ticket order is the order the requests take effect
  * in (exactly, or for BFS at least, depending on the order chosen below).
  *
- * A worker thread owns the handle and serves the queue in rounds; requests
- * that arrive while a round runs are served by the next one (continuous
- * batching). Updates are always applied in ticket order, fused into one
+ * Requests are served in rounds by a two-stage pipeline (continuous
+ * batching): a planner thread takes the next round from the queue and copies
+ * its update records into a device staging area on the server's own stream
+ * while an executor thread runs the round before on the handle's stream; two
+ * staging areas alternate. Requests that arrive while a round runs join a
+ * later one. Updates are always applied in ticket order, fused into one
  * gpma_update_batches launch per round; BFS requests of a round share one
  * multi-source gpma_bfs_multi traversal (up to 64 sources). Two orders:
  *
@@ -26,11 +29,18 @@
  * Each result carries state_ticket, the ticket of the last update applied
  * before the request was served, so a caller can replay exactly what it saw.
  *
- * Admission: the server sizes its fused launches from a device-memory
- * budget (default: a quarter of the device's free memory at creation);
- * an update request larger than one launch may hold is rejected with
- * GPMA_ERR_CAPACITY at submission, and submitters block while more than
- * max_queued_records update records wait.
+ * Admission: the server plans within a device-memory budget (default: half
+ * the device's free memory at creation). Out of it come a reserve for a
+ * rebuild of the packed array (its compaction copy and new arrays at twice
+ * the current capacity, which exist beside the old arrays during a resize;
+ * 28 bytes per slot), the BFS group's level buffer and traversal scratch,
+ * and two staging areas; the rest, at 146 bytes per record (library scratch
+ * and staging), sets how many update records one round may hold. The limit
+ * is recomputed after every round as the array grows. A round always takes
+ * the first queued update; later ones that do not fit wait for a later
+ * round (counted as deferred). An update request larger than the limit at
+ * creation is rejected with GPMA_ERR_CAPACITY at submission; submitters
+ * block while more than max_queued_records update records wait.
  *
  * While a server exists, the handle must not be used directly.
  */
@@ -51,7 +61,7 @@ typedef struct gpma_server *gpma_server_t;
 typedef struct gpma_server_config {
     uint32_t struct_size;        /* sizeof(gpma_server_config) */
     uint32_t max_bfs_group;      /* BFS requests per traversal, 1..64 (0 = 64) */
-    uint64_t hbm_budget_bytes;   /* device memory the server may plan for (0 = free / 4) */
+    uint64_t hbm_budget_bytes;   /* device memory the server may plan for (0 = free / 2) */
     uint64_t max_fused_records;  /* update records per fused launch (0 = from the budget) */
     uint64_t max_queued_records; /* update records queued before submitters block (0 = 2^26) */
     uint32_t order;              /* GPMA_SERVE_STRICT or GPMA_SERVE_SNAPSHOT */
@@ -77,9 +87,13 @@ typedef struct gpma_server_stats {
     uint64_t bfs_launches;
     uint64_t bfs_requests;
     uint64_t rejected;
-    uint64_t max_fused_records;  /* limits in effect */
+    uint64_t max_fused_records;    /* update records one round may hold now */
     uint32_t max_bfs_group;
     uint32_t reserved;
+    uint64_t deferred;             /* update requests left for a later round by the records limit */
+    uint64_t staged_ahead;         /* rounds whose records were staged while the round before ran */
+    uint64_t budget_bytes;         /* device memory the server plans within */
+    uint64_t growth_reserve_bytes; /* part of it held for a rebuild of the array */
 } gpma_server_stats;
 
 /* Start a server (and its worker thread) over handle g. */

@@ -53,7 +53,7 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(_lib.GpmaStats) == 8 + 4 * 4 + 6 * 8
     assert ctypes.sizeof(_lib.GpmaServerConfig) == 4 + 4 + 3 * 8 + 4 * 4
     assert ctypes.sizeof(_lib.GpmaRequestResult) == 4 * 4 + 4 * 8
-    assert ctypes.sizeof(_lib.GpmaServerStats) == 8 * 8 + 2 * 4
+    assert ctypes.sizeof(_lib.GpmaServerStats) == 8 * 8 + 2 * 4 + 4 * 8
     assert ctypes.sizeof(_lib.GpmaStepStats) == 6 * 8 + 2 * 4
 
 
