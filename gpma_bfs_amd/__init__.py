@@ -58,6 +58,13 @@ def _ptr(a, dtype):
     raise TypeError(f"unsupported array type {type(a)}")
 
 
+# gpma_config knobs a tuning dict may set (0 = default); the density
+# thresholds are in thousandths (reference 80 / 420 / 920 / 840, gpma.cuh:40-43)
+DENSITY_KNOBS = ("lower_leaf_permille", "lower_root_permille", "upper_leaf_permille", "upper_root_permille")
+TUNING_KNOBS = ("pull_permille", "segment_length", "device_levels", "sort_bucket_bits") + DENSITY_KNOBS + (
+    "sort_items_per_lane", "push_tile", "ms_tile")
+
+
 class GPMA:
     """One device-resident GPMA (reference: class GPMA, gpma.cuh:31-102)."""
 
@@ -71,10 +78,18 @@ class GPMA:
         for k, v in kw.items():
             if v:
                 t[k] = v
-        self.tuning = {k: int(t.get(k, 0)) for k in kw}
+        unknown = set(t) - set(TUNING_KNOBS)
+        if unknown:
+            raise GpmaError(GPMA_ERR_ARG, f"unknown tuning knobs {sorted(unknown)}")
+        self.tuning = {k: int(t.get(k, 0)) for k in TUNING_KNOBS}
         cfg = GpmaConfig(ctypes.sizeof(GpmaConfig), device, bulk_threshold, self.tuning["pull_permille"],
                          self.tuning["segment_length"], self.tuning["device_levels"],
                          self.tuning["sort_bucket_bits"])
+        for i, k in enumerate(DENSITY_KNOBS):
+            cfg.density_permille[i] = self.tuning[k]
+        cfg.sort_items_per_lane = self.tuning["sort_items_per_lane"]
+        cfg.push_tile = self.tuning["push_tile"]
+        cfg.ms_tile = self.tuning["ms_tile"]
         h = ctypes.c_void_p()
         check(lib.gpma_create(row_num, ctypes.byref(cfg), ctypes.byref(h)), "gpma_create")
         self._h = h

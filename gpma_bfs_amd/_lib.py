@@ -54,6 +54,11 @@ class GpmaConfig(ctypes.Structure):
         ("device_levels", ctypes.c_uint32),
         ("sort_bucket_bits", ctypes.c_uint32),
         ("reserved", ctypes.c_uint32),
+        ("density_permille", ctypes.c_uint16 * 4),
+        ("sort_items_per_lane", ctypes.c_uint32),
+        ("push_tile", ctypes.c_uint32),
+        ("ms_tile", ctypes.c_uint32),
+        ("reserved2", ctypes.c_uint32),
     ]
 
 

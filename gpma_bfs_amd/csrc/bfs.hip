@@ -192,7 +192,7 @@ __device__ __forceinline__ void visit_batch(const gkey (&k)[U], const gval (&v)[
 //   walked PUSH_UNROLL x 256 slots per round (owner by binary search).
 //   Workgroups [small_grid, gridDim): BIG_CHUNK-slot pieces of the big
 //   queue's ranges, so one hub is spread over many workgroups.
-constexpr uint32_t PUSH_TILE = 64;      // most frontier vertices per tile
+constexpr uint32_t PUSH_TILE = 64;      // most frontier vertices per tile (the tile_cap knob lowers it)
 constexpr uint32_t PUSH_MIN_TILE = 16;  // fewest (short frontiers)
 constexpr uint32_t PUSH_UNROLL = 4;
 
@@ -201,7 +201,7 @@ __device__ __forceinline__ void push_body(const gkey *__restrict__ keys, const g
                                           const uint32_t *__restrict__ front, const uint32_t *__restrict__ front_big,
                                           const NextLevel &nx, const unsigned long long *__restrict__ cnt,
                                           const unsigned long long *__restrict__ cnt_big, uint32_t *__restrict__ levels,
-                                          uint32_t level, uint32_t small_grid,
+                                          uint32_t level, uint32_t small_grid, uint32_t tile_cap,
                                           uint32_t *s_beg, uint32_t *s_ofs, uint32_t *scratch, const LdsQueue &lq) {
     const uint32_t tid = threadIdx.x;
     const uint32_t newlev = level + 1;
@@ -209,7 +209,7 @@ __device__ __forceinline__ void push_body(const gkey *__restrict__ keys, const g
         const uint32_t nf = pk_count(cnt[level]);
         // vertices per tile: a short frontier is spread over every workgroup
         const uint32_t tn =
-            (uint32_t)min<uint64_t>(PUSH_TILE, max<uint64_t>(PUSH_MIN_TILE, (nf + small_grid - 1) / small_grid));
+            (uint32_t)min<uint64_t>(tile_cap, max<uint64_t>(PUSH_MIN_TILE, (nf + small_grid - 1) / small_grid));
         for (uint64_t tile = (uint64_t)blockIdx.x * tn; tile < nf; tile += (uint64_t)small_grid * tn) {
             const uint64_t fi = tile + tid;
             uint32_t b = 0, len = 0;
@@ -349,7 +349,7 @@ __global__ __launch_bounds__(BFS_THREADS) void k_bfs_step(const gkey *__restrict
                                                           const unsigned long long *__restrict__ cnt_big,
                                                           uint32_t *__restrict__ levels, uint32_t level,
                                                           unsigned long long pull_threshold,
-                                                          uint32_t small_grid) {
+                                                          uint32_t small_grid, uint32_t tile_cap) {
     __shared__ uint32_t s_beg[BIG_LIST_MAX];
     __shared__ uint32_t s_ofs[BIG_LIST_MAX + 1];
     __shared__ uint32_t scratch[BFS_THREADS / WAVE + 1];
@@ -367,8 +367,8 @@ __global__ __launch_bounds__(BFS_THREADS) void k_bfs_step(const gkey *__restrict
     if (C && work > pull_threshold)
         pull_body(keys, values, C, N, row_offset, nx, levels, level, lq);
     else
-        push_body(keys, values, row_offset, front, front_big, nx, cnt, cnt_big, levels, level, small_grid, s_beg, s_ofs,
-                  scratch, lq);
+        push_body(keys, values, row_offset, front, front_big, nx, cnt, cnt_big, levels, level, small_grid, tile_cap,
+                  s_beg, s_ofs, scratch, lq);
 }
 
 __global__ void k_bfs_clear(unsigned long long *__restrict__ cnt, unsigned long long *__restrict__ cnt_big,
@@ -401,7 +401,7 @@ void launch_bfs_level(const gkey *keys, const gval *values, uint64_t C, uint32_t
     const uint32_t small_grid = 1536, big_grid = 512;
     hipLaunchKernelGGL(k_bfs_step, dim3(small_grid + big_grid), dim3(BFS_THREADS), 0, st, keys, values, C, N,
                        row_offset, b.q[cur], b.qbig[cur], nx, b.cnt, b.cnt_big, b.levels, level, pull_threshold,
-                       small_grid);
+                       small_grid, std::min(PUSH_TILE, std::max(PUSH_MIN_TILE, b.tile_cap)));
 }
 
 }  // namespace gpma

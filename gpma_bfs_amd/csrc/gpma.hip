@@ -174,6 +174,11 @@ struct gpma_handle {
     uint32_t seg_override = 0;  // segment length after a resize (0: the reference formula)
     uint32_t spec_levels = SPEC_LEVELS;
     uint32_t sort_bits = 0;     // bucket bits of the batch sort (0: from the batch size)
+    uint32_t sort_ipt = 0;      // items per lane of the sort's tiles (0: from the batch size)
+    uint32_t push_tile = 64;    // most frontier vertices per BFS push tile
+    uint32_t ms_tile = 64;      // the same for the multi-source BFS
+    // density thresholds (gpma.cuh:40-43): lower / upper at the leaves and at the root
+    double lower_leaf = 0.08, lower_root = 0.42, upper_leaf = 0.92, upper_root = 0.84;
     Alloc mem;          // device bytes held, allocation-failure injection
     bool failed = false;  // the array was lost to an allocation failure: every call returns GPMA_ERR_OOM
     // structure
@@ -243,9 +248,11 @@ struct gpma_handle {
 
 namespace {
 
-// gpma.cuh:147-168
+// gpma.cuh:147-168, with the handle's thresholds (the reference's
+// 0.08 / 0.42 / 0.92 / 0.84 unless configured)
 void recalc_density(gpma_handle *g) {
-    const double lower_leaf = 0.08, lower_root = 0.42, upper_leaf = 0.92, upper_root = 0.84;
+    const double lower_leaf = g->lower_leaf, lower_root = g->lower_root, upper_leaf = g->upper_leaf,
+                 upper_root = g->upper_root;
     g->lower.assign(g->h + 1, 0);
     g->upper.assign(g->h + 1, 0);
     uint64_t len = g->S;
@@ -468,7 +475,8 @@ int replace_arrays(gpma_handle *g, uint64_t T, uint64_t c, const Shape &old) {
 // resize_gpma + root rebalance (gpma.cuh:906-934,1034-1044) when
 // check_bounds, or significant_insert (gpma.cuh:936-966) when not: compact
 // every live item, size the array to the smallest power of two >= 4 with
-// floor(0.84 * C') >= live + pending, and redistribute the merge of the live
+// floor(upper_root * C') >= live + pending (0.84 in the reference), and
+// redistribute the merge of the live
 // items and the pending inserts over the whole new array.
 int rebuild(gpma_handle *g, const gkey *pk, const gval *pv, uint64_t np, bool check_bounds) {
     const Shape old = shape_of(g);
@@ -481,7 +489,7 @@ int rebuild(gpma_handle *g, const gkey *pk, const gval *pv, uint64_t np, bool ch
                         g->tmpk.as<gkey>(), g->tmpv.as<gval>(), g->st);
     uint64_t merge = c + np;
     uint64_t T = 4;
-    while (std::floor(0.84 * (double)T) < (double)merge) T *= 2;
+    while (std::floor(g->upper_root * (double)T) < (double)merge) T *= 2;
     if (T > (1ull << 31)) return fail(GPMA_ERR_CAPACITY, "packed array would exceed 2^31 slots");
     size_tree(g, T);
     if (check_bounds) {
@@ -801,7 +809,7 @@ int do_update(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint6
     if (g->msd_cool) g->msd_cool--;
     MsdPlan pl{};
     if (use_msd) {
-        pl = msd_plan(n, g->N, g->sort_bits, !guards_ok);
+        pl = msd_plan(n, g->N, g->sort_bits, !guards_ok, g->sort_ipt);
         TRY(ensure_msd(g, n, pl));
     }
     if (!g->ctr_clean) fill_u32(g->ctr.as<uint32_t>(), 0, C_COUNT, st);
@@ -837,6 +845,7 @@ int ensure_bfs(gpma_handle *g) {
 
 BfsState bfs_state(gpma_handle *g) {
     BfsState b;
+    b.tile_cap = g->push_tile;
     b.levels = g->lev.as<uint32_t>();
     b.q[0] = g->fr0.as<uint32_t>();
     b.q[1] = g->fr1.as<uint32_t>();
@@ -930,6 +939,7 @@ int ensure_msbfs(gpma_handle *g) {
 
 MsState ms_state(gpma_handle *g) {
     MsState m;
+    m.tile_cap = g->ms_tile;
     m.seen = g->ms_seen.as<uint64_t>();
     m.fr[0] = g->ms_fr0.as<uint64_t>();
     m.fr[1] = g->ms_fr1.as<uint64_t>();
@@ -1047,7 +1057,8 @@ void drop_graphs(gpma_handle *g) {
 std::vector<uint64_t> step_signature(const gpma_handle *g, uint32_t cls, uint32_t nsrc, const uint32_t *out,
                                      uint32_t ulevels, uint32_t blevels) {
     std::vector<uint64_t> v = {cls, nsrc, (uint64_t)(uintptr_t)out, ulevels, blevels, g->N, g->C, g->S, g->h,
-                               g->bulk_threshold, g->sort_bits, (uint64_t)(g->pull_fraction * 1e9)};
+                               g->bulk_threshold, g->sort_bits, (uint64_t)(g->pull_fraction * 1e9), g->sort_ipt,
+                               g->ms_tile};
     const Buf *bufs[] = {&g->keys, &g->values, &g->seps, &g->row_offset, &g->ctr, &g->step_n, &g->step_stage,
                          &g->in_k, &g->in_v,
                          &g->sk0, &g->sv0, &g->sk1, &g->win, &g->pk0, &g->pv0, &g->pw0, &g->pd0, &g->pk1, &g->pv1,
@@ -1129,7 +1140,7 @@ int step_graph(gpma_handle *g, uint32_t cls, uint32_t nsrc, uint32_t *dlev, gpma
                uint32_t **out, uint32_t *blevels) {
     const uint64_t cap = 1ull << cls;
     TRY(ensure_update_buffers(g, cap));
-    const MsdPlan pl = msd_plan(cap, g->N, g->sort_bits, true);
+    const MsdPlan pl = msd_plan(cap, g->N, g->sort_bits, true, g->sort_ipt);
     TRY(ensure_msd(g, cap, pl));
     TRY(g->step_n.ensure(256, &g->mem));
     TRY(g->step_stage.ensure((4 + 64) * 4, &g->mem));
@@ -1398,7 +1409,8 @@ int gpma_create(uint32_t row_num, const gpma_config *cfg, gpma_t *out) {
     if (row_num >= 0xFFFFFFFFu) return fail(GPMA_ERR_ARG, "gpma_create: row_num must be < 2^32-1");
     gpma_handle *g = new gpma_handle();
     g->N = row_num;
-    if (cfg && cfg->struct_size >= sizeof(gpma_config)) {
+    // callers built against the first 32-byte layout (up to `reserved`) keep working
+    if (cfg && cfg->struct_size >= GPMA_CONFIG_V1_SIZE) {
         if (cfg->device >= 0) g->device = cfg->device;
         else hipGetDevice(&g->device);
         if (cfg->bulk_threshold) g->bulk_threshold = cfg->bulk_threshold;
@@ -1416,6 +1428,31 @@ int gpma_create(uint32_t row_num, const gpma_config *cfg, gpma_t *out) {
         g->seg_override = sl;
         if (cfg->device_levels) g->spec_levels = cfg->device_levels;
         g->sort_bits = cfg->sort_bucket_bits;
+        if (cfg->struct_size >= sizeof(gpma_config)) {
+            const uint16_t *dp = cfg->density_permille;
+            if (dp[0]) g->lower_leaf = dp[0] / 1000.0;
+            if (dp[1]) g->lower_root = dp[1] / 1000.0;
+            if (dp[2]) g->upper_leaf = dp[2] / 1000.0;
+            if (dp[3]) g->upper_root = dp[3] / 1000.0;
+            // every redistributed window keeps at least one item per segment
+            // (lower_leaf * 16 >= 1), bounds nest, and a resized root (density
+            // above upper_root / 2) never starts below its lower bound
+            if (g->lower_leaf < 0.0625 || g->lower_leaf > g->lower_root || g->lower_root * 2 > g->upper_root ||
+                g->upper_root > g->upper_leaf || g->upper_leaf > 1.0) {
+                delete g;
+                return fail(GPMA_ERR_ARG, "gpma_create: density thresholds need 0.0625 <= lower_leaf <= lower_root, "
+                                          "2 * lower_root <= upper_root <= upper_leaf <= 1");
+            }
+            const uint32_t ipt = cfg->sort_items_per_lane, pt = cfg->push_tile, mt = cfg->ms_tile;
+            if ((ipt && ipt != 4 && ipt != 8 && ipt != 16) || (pt && (pt < 16 || pt > 64)) || (mt && (mt < 16 || mt > 64))) {
+                delete g;
+                return fail(GPMA_ERR_ARG, "gpma_create: sort_items_per_lane must be 4, 8 or 16; push_tile and "
+                                          "ms_tile 16..64");
+            }
+            g->sort_ipt = ipt;
+            if (pt) g->push_tile = pt;
+            if (mt) g->ms_tile = mt;
+        }
     } else {
         hipGetDevice(&g->device);
     }
@@ -1560,7 +1597,7 @@ int gpma_step_prepare(gpma_t g, uint64_t max_records, uint32_t nsrc, uint32_t *l
     uint32_t top_cls = STEP_MIN_CLASS;
     while ((1ull << top_cls) < top) top_cls++;
     TRY(ensure_update_buffers(g, 1ull << top_cls));
-    TRY(ensure_msd(g, 1ull << top_cls, msd_plan(1ull << top_cls, g->N, g->sort_bits, true)));
+    TRY(ensure_msd(g, 1ull << top_cls, msd_plan(1ull << top_cls, g->N, g->sort_bits, true, g->sort_ipt)));
     if (nsrc) {
         TRY(ensure_msbfs(g));
         if (!levels) TRY(g->ms_out.ensure((uint64_t)nsrc * g->N * 4, &g->mem));

@@ -46,7 +46,8 @@ struct MsdPlan {
 // bucket_bits 0: from n. by_row: buckets by the row's slot position
 // (row_offset valid, capacity C), balanced for skewed sources; else by the
 // key's top bits (the guard insertion, before row_offset exists).
-MsdPlan msd_plan(uint64_t n, uint32_t N, uint32_t bucket_bits, bool by_row);
+// items_per_lane: 4, 8 or 16 items per lane of the rank and scatter tiles (0: from n).
+MsdPlan msd_plan(uint64_t n, uint32_t N, uint32_t bucket_bits, bool by_row, uint32_t items_per_lane = 0);
 struct MsdWork {
     uint64_t *kp;               // n packed keys
     uint32_t *rk;               // n (bucket << 16 | rank in (tile, bucket))
@@ -155,6 +156,7 @@ struct BfsState {
     uint32_t *q[2], *qbig[2];
     // per level: low 32 bits = queued vertices, high 32 bits = their slot count
     unsigned long long *cnt, *cnt_big;
+    uint32_t tile_cap = 64;  // most frontier vertices per push tile (16..64)
 };
 // levels = 0 except start (1); start queued for level 1.
 void launch_bfs_init(const BfsState &b, uint32_t N, uint32_t start, const uint32_t *row_offset, hipStream_t st);
@@ -175,6 +177,7 @@ struct MsState {
     unsigned long long *cnt, *cnt_big;  // per level: low = vertices, high = slots
     uint32_t *tcnt;            // per level touched count
     uint32_t *partial;         // [ms_output_blocks(N)][64] reachable partials
+    uint32_t tile_cap = 64;    // most frontier vertices per push tile (16..64)
 };
 // gate (optional): when *gate != 0 no source is queued, so every level finds
 // an empty frontier.

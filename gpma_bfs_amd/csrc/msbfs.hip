@@ -36,7 +36,7 @@ namespace gpma {
 namespace {
 
 constexpr uint32_t T = 256;
-constexpr uint32_t MS_TILE = 64;      // most frontier vertices per tile
+constexpr uint32_t MS_TILE = 64;      // most frontier vertices per tile (the tile_cap knob lowers it)
 constexpr uint32_t MS_MIN_TILE = 16;  // fewest (short frontiers)
 constexpr uint32_t MS_UNROLL = 4;
 constexpr uint32_t MS_BIG_RANGE = 2048;
@@ -175,7 +175,8 @@ __global__ __launch_bounds__(T) void k_ms_expand(const gkey *__restrict__ keys, 
                                                  const uint64_t *__restrict__ fr, const uint64_t *__restrict__ seen,
                                                  uint64_t *__restrict__ nx, uint32_t *__restrict__ touched,
                                                  uint32_t *__restrict__ tcnt, uint32_t level,
-                                                 unsigned long long pull_threshold, uint32_t small_grid) {
+                                                 unsigned long long pull_threshold, uint32_t small_grid,
+                                                 uint32_t tile_cap) {
     __shared__ uint32_t s_beg[MS_BIG_LIST];
     __shared__ uint32_t s_ofs[MS_BIG_LIST + 1];
     __shared__ uint32_t scratch[T / WAVE + 1];
@@ -215,7 +216,7 @@ __global__ __launch_bounds__(T) void k_ms_expand(const gkey *__restrict__ keys, 
         }
     } else if (blockIdx.x < small_grid) {
         // vertices per tile: a short frontier is spread over every workgroup
-        const uint32_t tn = (uint32_t)min<uint64_t>(MS_TILE, max<uint64_t>(MS_MIN_TILE, (nf + small_grid - 1) / small_grid));
+        const uint32_t tn = (uint32_t)min<uint64_t>(tile_cap, max<uint64_t>(MS_MIN_TILE, (nf + small_grid - 1) / small_grid));
         for (uint64_t tile = (uint64_t)blockIdx.x * tn; tile < nf; tile += (uint64_t)small_grid * tn) {
             const uint64_t fi = tile + tid;
             uint32_t bb = 0, len = 0;
@@ -539,7 +540,7 @@ void ms_level(const gkey *keys, const gval *values, uint64_t C, uint32_t N, cons
     const uint32_t small_grid = 1536, big_grid = 512;
     hipLaunchKernelGGL(k_ms_expand, dim3(small_grid + big_grid), dim3(T), 0, st, keys, values, C, N, ro, m.q[cur],
                        m.qbig[cur], m.cnt, m.cnt_big, m.fr[cur], m.seen, m.nx, m.touched, m.tcnt + level, level,
-                       pull_threshold, small_grid);
+                       pull_threshold, small_grid, std::min(MS_TILE, std::max(MS_MIN_TILE, m.tile_cap)));
     hipLaunchKernelGGL(k_ms_finalize, dim3(1024), dim3(T), 0, st, m.q[cur], m.qbig[cur], m.cnt, m.cnt_big, m.fr[cur],
                        m.fr[nxt], m.seen, m.nx, m.lv16, m.lv32, m.touched, m.tcnt + level, m.q[nxt], m.qbig[nxt],
                        m.cnt + level + 1, m.cnt_big + level + 1, ro, N, level);

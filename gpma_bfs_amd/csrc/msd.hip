@@ -416,7 +416,7 @@ __global__ __launch_bounds__(MT) void k_msd_local(uint64_t *__restrict__ kb, uin
 
 bool msd_sort_fits(uint32_t N) { return N > 0 && N < (1u << 28); }
 
-MsdPlan msd_plan(uint64_t n, uint32_t N, uint32_t bucket_bits, bool by_row) {
+MsdPlan msd_plan(uint64_t n, uint32_t N, uint32_t bucket_bits, bool by_row, uint32_t items_per_lane) {
     MsdPlan pl{};
     uint32_t bs = 0;
     while (bs < 32 && (N >> bs) != 0) bs++;  // bit width of N itself
@@ -432,6 +432,7 @@ MsdPlan msd_plan(uint64_t n, uint32_t N, uint32_t bucket_bits, bool by_row) {
     pl.bshift = 8 + B - D;
     // items per lane: enough tiles (about 400) to occupy every CU
     pl.ipt = n >= 400ull * MT * 16 ? 16 : n >= 400ull * MT * 8 ? 8 : 4;
+    if (items_per_lane == 4 || items_per_lane == 8 || items_per_lane == 16) pl.ipt = items_per_lane;
     pl.ntiles = (uint32_t)((n + MT * pl.ipt - 1) / (MT * pl.ipt));
     pl.NB = 1u << D;
     pl.nscan = (pl.NB + SCAN_B - 1) / SCAN_B;

@@ -1,5 +1,8 @@
-"""Autotuner for the library's tuning knobs (gpma_config: segment_length,
-device_levels, sort_bucket_bits, pull_permille).
+"""Autotuner for the library's tuning knobs (gpma_config): segment length,
+device-driven rebalance levels, the batch sort's bucket bits and items per
+lane, the four density thresholds (reference gpma.cuh:40-43), the BFS
+push/pull threshold and the push tile sizes of the single- and multi-source
+BFS.
 
 None of the knobs changes results: every candidate is checked against the
 closed-form live-key set of the sliding window it ran (and BFS levels against
@@ -23,16 +26,32 @@ import time
 
 import numpy as np
 
+# 0 is the default of every knob: the reference formula for the segment
+# length (16 at these sizes), the reference thresholds (80 / 420 / 920 / 840
+# thousandths), sizes from the batch for the sort, 64-vertex push tiles.
+# Threshold combinations the library refuses (bounds that do not nest) are
+# skipped by the search.
+DENSITY_SPACE = {
+    "lower_leaf_permille": [0, 120, 200],
+    "upper_leaf_permille": [0, 850, 980],
+    "lower_root_permille": [0, 300],
+    "upper_root_permille": [0, 700, 900],
+}
 UPDATE_SPACE = {
-    "segment_length": [0, 8, 32],  # 0: the reference formula (16 at these sizes)
+    "segment_length": [0, 8, 32],
     "device_levels": [1, 2, 3, 4, 5, 6],
     "sort_bucket_bits": [0, 9, 10, 11, 12],
+    "sort_items_per_lane": [0, 4, 8, 16],
+    **DENSITY_SPACE,
 }
-BFS_SPACE = {"pull_permille": [0, 50, 100, 500]}
+BFS_SPACE = {"pull_permille": [0, 50, 100, 500], "push_tile": [0, 16, 32]}
 BATCHED_SPACE = {
     "segment_length": [0, 8, 32],
     "sort_bucket_bits": [0, 10, 11, 12],
+    "sort_items_per_lane": [0, 4, 8, 16],
     "pull_permille": [0, 500],
+    "ms_tile": [0, 16, 32],
+    **DENSITY_SPACE,
 }
 
 
@@ -131,7 +150,10 @@ def tune_bfs(G, N, keys, knobs, space=BFS_SPACE, reps=3, log=print):
     g0.close()
 
     def measure(cfg):
-        g = G.GPMA(N, tuning=dict(knobs, **cfg))
+        try:
+            g = G.GPMA(N, tuning=dict(knobs, **cfg))
+        except G.GpmaError:
+            return None
         try:
             G.update_gpma(g, keys[:half], np.ones(half, np.uint8))
             lv, _ = G.gpma_bfs(g, src)

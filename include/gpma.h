@@ -55,7 +55,20 @@ typedef struct gpma_config {
     uint32_t device_levels;    /* rebalance levels 1..k queued without a host round trip (1..8, 0 = 3) */
     uint32_t sort_bucket_bits; /* bucket bits of the batch sort (4..12, 0 = from the batch size) */
     uint32_t reserved;
+    /* Fields from here on are read when struct_size >= sizeof(gpma_config);
+     * a caller built against the first layout (struct_size 32) gets their
+     * defaults. None changes the live-key set or BFS results. */
+    uint16_t density_permille[4]; /* density thresholds in thousandths (reference gpma.cuh:40-43;
+                                     0 = the reference's value): [0] lower bound at the leaves (80),
+                                     [1] lower at the root (420), [2] upper at the leaves (920),
+                                     [3] upper at the root (840, also the resize target). Needs
+                                     62.5 <= [0] <= [1], 2 * [1] <= [3] <= [2] <= 1000 */
+    uint32_t sort_items_per_lane; /* items per lane of the batch sort's tiles: 4, 8 or 16 (0 = from the batch size) */
+    uint32_t push_tile;           /* most frontier vertices per BFS push tile, 16..64 (0 = 64) */
+    uint32_t ms_tile;             /* the same for the multi-source BFS, 16..64 (0 = 64) */
+    uint32_t reserved2;
 } gpma_config;
+#define GPMA_CONFIG_V1_SIZE 32
 
 /* State summary. Pointers are device pointers owned by the handle and are
  * valid until the next gpma_update / gpma_destroy. */

@@ -48,7 +48,7 @@ def test_python_binding_covers_header():
 
 
 def test_struct_sizes_match_header_layout():
-    assert ctypes.sizeof(_lib.GpmaConfig) == 32
+    assert ctypes.sizeof(_lib.GpmaConfig) == 32 + 8 + 4 * 4
     assert ctypes.sizeof(_lib.GpmaCsr) == 3 * 8 + 8 + 4 * 4
     assert ctypes.sizeof(_lib.GpmaStats) == 8 + 4 * 4 + 6 * 8
     assert ctypes.sizeof(_lib.GpmaServerConfig) == 4 + 4 + 3 * 8 + 4 * 4

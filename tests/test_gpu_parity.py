@@ -366,12 +366,21 @@ def test_skewed_batches_sort_paths():
 
 @pytest.mark.parametrize("knobs", [{"segment_length": 8}, {"segment_length": 32}, {"segment_length": 4},
                                    {"device_levels": 1}, {"device_levels": 8},
-                                   {"sort_bucket_bits": 4}, {"sort_bucket_bits": 12}])
+                                   {"sort_bucket_bits": 4}, {"sort_bucket_bits": 12},
+                                   {"lower_leaf_permille": 150},
+                                   {"upper_leaf_permille": 800, "upper_root_permille": 700, "lower_root_permille": 350},
+                                   {"lower_leaf_permille": 63, "lower_root_permille": 250, "upper_root_permille": 600,
+                                    "upper_leaf_permille": 980},
+                                   {"lower_leaf_permille": 100, "lower_root_permille": 450, "upper_root_permille": 900,
+                                    "upper_leaf_permille": 950, "segment_length": 32},
+                                   {"sort_items_per_lane": 4}, {"sort_items_per_lane": 16},
+                                   {"push_tile": 16}, {"ms_tile": 16}, {"push_tile": 32, "ms_tile": 40}])
 def test_tuning_knobs_keep_results(knobs):
-    """Tuning knobs change timing (and, for segment_length, the layout) but
-    never the live-key set or BFS levels: 20 protocol slides on an R-MAT
-    graph with the closed-form live set after every slide, and BFS equal to
-    a textbook BFS over the final window."""
+    """Tuning knobs change timing (and, for segment_length and the density
+    thresholds, the layout) but never the live-key set or BFS levels: 20
+    protocol slides on an R-MAT graph with the closed-form live set after
+    every slide, and single- and 64-source BFS equal to a textbook BFS over
+    the final window."""
     src, dst = G.gen_rmat(14, 1 << 17, seed=5)
     N = 1 << 14
     keys = G.make_keys(src, dst)
@@ -384,7 +393,7 @@ def test_tuning_knobs_keep_results(knobs):
         G.update_gpma(g, b, v)
         assert np.array_equal(g.export_live(), closed_form_live(keys, N, i + 1, num_slide=20)), f"slide {i}"
     assert_invariants(g)
-    if "segment_length" in knobs:
+    if knobs.get("segment_length"):
         assert g.segment_length == knobs["segment_length"]
     step = half // 20
     win = keys[20 * step: half + 20 * step]
@@ -393,10 +402,28 @@ def test_tuning_knobs_keep_results(knobs):
         lv, reach = G.gpma_bfs(g, s0)
         elv, ereach = O.bfs_edges(N, ws, wd, s0)
         assert np.array_equal(lv, elv) and reach == ereach, f"start {s0}"
+    starts = np.random.default_rng(9).integers(0, N, 64).astype(np.uint32)
+    mlv, mreach = G.gpma_bfs_multi(g, starts)
+    elv, ereach = O.bfs_edges_multi(N, ws, wd, starts)
+    assert np.array_equal(mlv, elv) and np.array_equal(mreach, ereach)
+
+
+@pytest.mark.parametrize("name", ["mi355x_powerlaw_1p6m.json", "mi355x_powerlaw_1p6m_batched.json"])
+def test_committed_tuning_files_keep_results(name):
+    """The autotuner's committed results (segment length, device levels, sort
+    shape, density thresholds, BFS tiles) keep the closed-form live set over
+    20 slides and equal textbook BFS, single- and 64-source."""
+    from gpma_bfs_amd import tune
+
+    knobs = tune.load(os.path.join(os.path.dirname(GOLDEN), "..", "gpma_bfs_amd", "tuned", name))
+    test_tuning_knobs_keep_results(knobs)
 
 
 def test_unsupported_tuning_is_rejected():
-    for bad in ({"segment_length": 2}, {"segment_length": 12}, {"device_levels": 9}, {"sort_bucket_bits": 3}):
+    for bad in ({"segment_length": 2}, {"segment_length": 12}, {"device_levels": 9}, {"sort_bucket_bits": 3},
+                {"lower_leaf_permille": 60}, {"lower_root_permille": 500}, {"upper_root_permille": 950},
+                {"upper_leaf_permille": 1001}, {"sort_items_per_lane": 5}, {"push_tile": 8}, {"ms_tile": 65},
+                {"no_such_knob": 1}):
         with pytest.raises(G.GpmaError) as e:
             G.GPMA(64, tuning=bad)
         assert e.value.code == G.GPMA_ERR_ARG
