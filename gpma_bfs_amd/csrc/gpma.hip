@@ -187,6 +187,7 @@ struct gpma_handle {
     Buf in_k, in_v, sk0, sv0, sk1, sv1, win, blkcnt, blkoff, sortwork;
     Buf pk0, pv0, pw0, pd0, pk1, pv1, pw1, pd1;
     Buf tmpk, tmpv, chunkcnt, chunkoff;
+    Buf wide;                // per-window scratch of the wide rebalance levels
     Buf ccstat;              // look-back status words of the chained compaction
     Buf msd_kp, msd_rk, msd_mat, msd_bstart, msd_status;  // bucket sort scratch
     uint32_t msd_cool = 0;   // batches left on the LSD sort after a skewed batch
@@ -741,35 +742,29 @@ int finish_update(gpma_handle *g, uint64_t n, uint32_t spec, bool used_msd) {
                           nullptr, 0, st);
             phase_end(g, P_UPPER, 1);
         } else {
-            // wide windows: runs found on the host from the window ids
-            std::vector<uint32_t> hw(np);
-            HIP_TRY(hipMemcpyAsync(hw.data(), pw, np * 4, hipMemcpyDeviceToHost, st));
-            phase_end(g, P_UPPER, 0);
-            TRY(stream_sync(g));
-            uint64_t a = 0;
-            while (a < np) {
-                uint64_t b = a + 1;
-                while (b < np && hw[b] == hw[a]) b++;
-                const uint64_t r = b - a;
-                uint64_t c = 0;
-                TRY(count_live(g, hw[a], W, &c));
-                const uint64_t m = c + r;
-                phase_begin(g, P_UPPER);
-                if (g->lower[level] <= m && m <= g->upper[level]) {
-                    launch_compact_live(g->keys.as<gkey>(), g->values.as<gval>(), hw[a], W,
-                                        g->chunkoff.as<uint32_t>(), g->tmpk.as<gkey>(), g->tmpv.as<gval>(), st);
-                    launch_merge_redistribute(g->tmpk.as<gkey>(), g->tmpv.as<gval>(), c, pk + a, pv + a, r,
-                                              g->keys.as<gkey>(), g->values.as<gval>(), g->seps.as<gkey>(),
-                                              g->row_offset.as<uint32_t>(), hw[a], W, g->S, st);
-                    launch_mark_done(pd, a, b, st);
-                }
-                phase_end(g, P_UPPER, 3);
-                a = b;
-                TRY(stream_sync(g));
-            }
+            // windows wider than a workgroup: every window of the level in
+            // one set of device-wide passes (no host round trip per window)
+            TRY(ensure_tmp(g, g->C));
+            TRY(g->wide.ensure((np + 1) * 24, &g->mem));
+            WideWork w;
+            w.blkcnt = g->blkcnt.as<uint32_t>();
+            w.blkoff = g->blkoff.as<uint32_t>();
+            w.heads = g->win.as<uint32_t>();  // free once the level-0 compaction ran
+            w.d_nruns = ctr + C_NRUNS;
+            w.chunkcnt = g->chunkcnt.as<uint32_t>();
+            w.chunkoff = g->chunkoff.as<uint32_t>();
+            w.mpre = g->wide.as<unsigned long long>();
+            w.live = (uint32_t *)(w.mpre + np + 1);
+            w.merge = w.live + np + 1;
+            w.ok = (uint8_t *)(w.merge + np + 1);
+            w.tk = g->tmpk.as<gkey>();
+            w.tv = g->tmpv.as<gval>();
+            launch_wide_level(pk, pv, pw, pd, np, d_np, W, g->S, g->lower[level], g->upper[level],
+                              g->keys.as<gkey>(), g->values.as<gval>(), g->seps.as<gkey>(),
+                              g->row_offset.as<uint32_t>(), w, st);
+            phase_end(g, P_UPPER, 9);
         }
         const uint32_t mask = ~(uint32_t)(((uint64_t)g->S << (level + 1)) - 1);
-        if (W > WIN_MAX) fill_u32(d_np, (uint32_t)np, 1, st);  // count_live's reads cleared it
         g->ctr_clean = false;
         phase_begin(g, P_COMPACT);
         compact_step(g, pk, pv, pw, pd, np, d_np, PK[pc ^ 1]->as<gkey>(), PV[pc ^ 1]->as<gval>(),
@@ -1503,7 +1498,8 @@ int gpma_destroy(gpma_t g) {
                   &g->chunkoff, &g->lev, &g->fr0, &g->fr1, &g->fb0, &g->fb1, &g->bcnt, &g->bcntb,
                   &g->ms_seen, &g->ms_fr0, &g->ms_fr1, &g->ms_nx, &g->ms_lv16, &g->ms_q0, &g->ms_q1, &g->ms_qb0,
                   &g->ms_qb1, &g->ms_touched, &g->ms_cnt, &g->ms_cntb, &g->ms_tcnt, &g->ms_partial, &g->ms_starts,
-                  &g->ms_out, &g->ms_reach, &g->ms_lv32, &g->ctr, &g->digest, &g->step_n, &g->step_stage};
+                  &g->ms_out, &g->ms_reach, &g->ms_lv32, &g->ctr, &g->digest, &g->step_n, &g->step_stage,
+                  &g->wide};
     drop_graphs(g);
     for (Buf *b : all) b->release(&g->mem);
     if (g->hctr) (void)hipHostFree(g->hctr);

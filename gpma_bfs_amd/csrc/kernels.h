@@ -140,6 +140,29 @@ void launch_merge_redistribute(const gkey *tk, const gval *tv, uint64_t c, const
                                uint64_t r, gkey *keys, gval *values, gkey *seps, uint32_t *row_offset,
                                uint64_t w, uint64_t W, uint32_t S, hipStream_t st);
 void launch_mark_done(uint8_t *pd, uint64_t a, uint64_t b, hipStream_t st);
+// Scratch of one wide rebalance level (launch_wide_level): heads and counts
+// of the pending list's runs, per-chunk live counts and offsets (up to
+// C / CHUNK + 1 entries), per-window live count, merge size, pass flag and
+// merge prefix (up to the pending count + 1 entries), the compaction buffer
+// (C items).
+struct WideWork {
+    uint32_t *blkcnt, *blkoff;  // run-head counting (pending count / CMP_BLOCK + 1 words each)
+    uint32_t *heads, *d_nruns;
+    uint32_t *chunkcnt, *chunkoff;
+    uint32_t *live, *merge;
+    uint8_t *ok;
+    unsigned long long *mpre;
+    gkey *tk;
+    gval *tv;
+};
+// One rebalance level whose windows (W = S << level > WIN_MAX slots) are
+// wider than a workgroup, over every window the pending list names, with no
+// host round trip: windows whose merge size lies in [lower, upper] are
+// compacted, merged with their pending inserts and redistributed, and those
+// inserts marked done (pd). The pending count is *d_np (<= n_ub).
+void launch_wide_level(const gkey *pk, const gval *pv, const uint32_t *pw, uint8_t *pd, uint64_t n_ub,
+                       const uint32_t *d_np, uint64_t W, uint32_t S, uint32_t lower, uint32_t upper, gkey *keys,
+                       gval *values, gkey *seps, uint32_t *row_offset, const WideWork &w, hipStream_t st);
 // out[0..3) = (count, wrapping sum, xor) of the live keys of keys[0 .. C).
 void launch_live_digest(const gkey *keys, const gval *values, uint64_t C, unsigned long long *out, hipStream_t st);
 // Write c compacted items to keys[0..c), KEY_NONE after; rebuild seps and

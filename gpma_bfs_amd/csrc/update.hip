@@ -793,6 +793,235 @@ __global__ __launch_bounds__(256) void k_fill_gaps(uint64_t m, gkey *__restrict_
     }
 }
 
+// ---- one rebalance level of windows wider than one workgroup (W > WIN_MAX)
+//
+// All windows of the level are handled together (rebalancing_kernel's level
+// pass, gpma.cuh:727-794): the pending list's runs (heads[0 .. *d_nruns), one
+// per window) name the windows; every window is CPW = W / CHUNK chunks. The
+// passes: live count per chunk (k_wide_count), one planning workgroup
+// (k_wide_plan: chunk offsets, each window's live count c, merge size m =
+// c + pending run length, the bounds check, and the prefix of m over the
+// windows that pass), compaction of the passing windows' live items into tmp
+// (k_wide_compact), the merge of compacted items and pending inserts at the
+// reference's slot formula (k_wide_merge), the gaps (k_wide_fill) and the
+// done flags of the placed inserts (k_wide_done). Windows are disjoint, so
+// the result equals handling them one after another.
+
+__device__ __forceinline__ uint64_t wide_run_len(const uint32_t *heads, uint32_t nruns, uint64_t np, uint32_t j) {
+    return (j + 1 < nruns ? (uint64_t)heads[j + 1] : np) - heads[j];
+}
+
+// last j in [0, n) with a[j] <= x (a[0] <= x)
+template <typename T>
+__device__ __forceinline__ uint32_t last_le(const T *a, uint32_t n, uint64_t x) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if ((uint64_t)a[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_wide_count(const gkey *__restrict__ keys, const gval *__restrict__ values,
+                                                    const uint32_t *__restrict__ pw,
+                                                    const uint32_t *__restrict__ heads,
+                                                    const uint32_t *__restrict__ d_nruns, uint32_t cpw,
+                                                    uint32_t *__restrict__ chunkcnt) {
+    __shared__ uint32_t wcnt[256 / WAVE];
+    const uint64_t nb = (uint64_t)*d_nruns * cpw;
+    for (uint64_t g = blockIdx.x; g < nb; g += gridDim.x) {
+        const uint64_t base = (uint64_t)pw[heads[g / cpw]] + (g % cpw) * CHUNK;
+        uint32_t c = 0;
+        for (uint32_t j = threadIdx.x; j < CHUNK; j += 256) c += is_live(keys[base + j], values[base + j]) ? 1u : 0u;
+        for (int d = WAVE / 2; d > 0; d >>= 1) c += __shfl_down(c, d, WAVE);
+        if (lane_id() == 0) wcnt[threadIdx.x / WAVE] = c;
+        __syncthreads();
+        if (threadIdx.x == 0) chunkcnt[g] = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+        __syncthreads();
+    }
+}
+
+// One workgroup: chunkoff[0 .. nb] = exclusive scan of chunkcnt (nb =
+// nruns * cpw; chunkoff[nb] = total), live[j] and merge[j] per window,
+// ok[j] = lower <= merge <= upper, mpre[0 .. nruns] = exclusive scan of
+// merge over the windows that pass.
+__global__ __launch_bounds__(1024) void k_wide_plan(const uint32_t *__restrict__ chunkcnt,
+                                                   const uint32_t *__restrict__ heads,
+                                                   const uint32_t *__restrict__ d_nruns,
+                                                   const uint32_t *__restrict__ d_np, uint32_t cpw, uint32_t lower,
+                                                   uint32_t upper, uint32_t *__restrict__ chunkoff,
+                                                   uint32_t *__restrict__ live, uint32_t *__restrict__ merge,
+                                                   uint8_t *__restrict__ ok, unsigned long long *__restrict__ mpre) {
+    __shared__ uint32_t scratch[1024 / WAVE + 1];
+    __shared__ unsigned long long sc64[1024 / WAVE + 1];
+    const uint32_t nruns = *d_nruns;
+    const uint64_t np = *d_np;
+    const uint64_t nb = (uint64_t)nruns * cpw;
+    {
+        const uint64_t per = (nb + 1023) / 1024;
+        const uint64_t a = min<uint64_t>(nb, threadIdx.x * per), b = min<uint64_t>(nb, a + per);
+        uint32_t sum = 0;
+        for (uint64_t j = a; j < b; j++) sum += chunkcnt[j];
+        uint32_t tot;
+        uint32_t pre = block_exclusive_scan(sum, scratch, &tot);
+        for (uint64_t j = a; j < b; j++) {
+            const uint32_t c = chunkcnt[j];
+            chunkoff[j] = pre;
+            pre += c;
+        }
+        if (threadIdx.x == 0) chunkoff[nb] = tot;
+    }
+    __syncthreads();
+    const uint32_t per = (nruns + 1023) / 1024;
+    const uint32_t a = min(nruns, threadIdx.x * per), b = min(nruns, a + per);
+    unsigned long long sum = 0;
+    for (uint32_t j = a; j < b; j++) {
+        const uint32_t c = chunkoff[(uint64_t)(j + 1) * cpw] - chunkoff[(uint64_t)j * cpw];
+        const uint64_t m = c + wide_run_len(heads, nruns, np, j);
+        const bool pass = lower <= m && m <= upper;
+        live[j] = c;
+        merge[j] = (uint32_t)m;
+        ok[j] = pass ? 1 : 0;
+        sum += pass ? m : 0;
+    }
+    // block exclusive scan of the 64-bit sums
+    const uint32_t lane = lane_id(), wid = threadIdx.x / WAVE;
+    unsigned long long v = sum;
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const unsigned long long y = __shfl_up(v, d, WAVE);
+        if (lane >= (uint32_t)d) v += y;
+    }
+    if (lane == WAVE - 1) sc64[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long run = 0;
+        for (uint32_t w = 0; w < 1024 / WAVE; w++) {
+            const unsigned long long t = sc64[w];
+            sc64[w] = run;
+            run += t;
+        }
+        sc64[1024 / WAVE] = run;
+    }
+    __syncthreads();
+    unsigned long long pre = sc64[wid] + v - sum;
+    for (uint32_t j = a; j < b; j++) {
+        mpre[j] = pre;
+        pre += ok[j] ? merge[j] : 0;
+    }
+    if (threadIdx.x == 0) mpre[nruns] = sc64[1024 / WAVE];
+}
+
+// Live items of every chunk of a passing window into tk/tv at chunkoff.
+__global__ __launch_bounds__(256) void k_wide_compact(const gkey *__restrict__ keys, const gval *__restrict__ values,
+                                                      const uint32_t *__restrict__ pw,
+                                                      const uint32_t *__restrict__ heads,
+                                                      const uint32_t *__restrict__ d_nruns, uint32_t cpw,
+                                                      const uint32_t *__restrict__ chunkoff,
+                                                      const uint8_t *__restrict__ ok, gkey *__restrict__ tk,
+                                                      gval *__restrict__ tv) {
+    __shared__ uint32_t scratch[256 / WAVE + 1];
+    const uint64_t nb = (uint64_t)*d_nruns * cpw;
+    for (uint64_t g = blockIdx.x; g < nb; g += gridDim.x) {
+        const uint32_t j = (uint32_t)(g / cpw);
+        if (!ok[j]) continue;  // uniform over the workgroup
+        const uint64_t base = (uint64_t)pw[heads[j]] + (g % cpw) * CHUNK;
+        uint64_t o = chunkoff[g];
+        for (uint32_t c = 0; c < CHUNK; c += 256) {
+            const gkey k = keys[base + c + threadIdx.x];
+            const gval v = values[base + c + threadIdx.x];
+            const bool lv = is_live(k, v);
+            uint32_t tot;
+            const uint32_t pre = block_exclusive_scan(lv ? 1u : 0u, scratch, &tot);
+            if (lv) {
+                tk[o + pre] = k;
+                tv[o + pre] = v;
+            }
+            o += tot;
+        }
+    }
+}
+
+// Merged item t of the passing windows (t < mpre[nruns]): its window j,
+// then the reference's slot formula as in k_merge_scatter.
+__global__ __launch_bounds__(256) void k_wide_merge(const gkey *__restrict__ tk, const gval *__restrict__ tv,
+                                                    const gkey *__restrict__ pk, const gval *__restrict__ pv,
+                                                    const uint32_t *__restrict__ pw,
+                                                    const uint32_t *__restrict__ heads,
+                                                    const uint32_t *__restrict__ d_nruns, uint32_t cpw,
+                                                    const uint32_t *__restrict__ chunkoff,
+                                                    const uint32_t *__restrict__ live,
+                                                    const uint32_t *__restrict__ merge,
+                                                    const unsigned long long *__restrict__ mpre, uint64_t W,
+                                                    uint32_t S, gkey *__restrict__ keys, gval *__restrict__ values,
+                                                    gkey *__restrict__ seps, uint32_t *__restrict__ row_offset) {
+    const uint32_t nruns = *d_nruns;
+    const uint64_t total = mpre[nruns];
+    const uint64_t f = W / S;
+    for (uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (uint64_t)gridDim.x * 256) {
+        // a window that failed adds nothing to the prefix, so the last j
+        // with mpre[j] <= t is the passing window that holds item t
+        const uint32_t j = last_le(mpre, nruns, t);
+        const uint64_t u = t - mpre[j];
+        const uint64_t c = live[j], m = merge[j], r = m - c;
+        const gkey *tw = tk + chunkoff[(uint64_t)j * cpw];
+        const gval *tvw = tv + chunkoff[(uint64_t)j * cpw];
+        const uint64_t h = heads[j];
+        gkey k;
+        gval v;
+        uint64_t mi;
+        if (u < c) {
+            k = tw[u];
+            v = tvw[u];
+            mi = u + lower_bound_g(pk + h, r, k);
+        } else {
+            const uint64_t u2 = u - c;
+            k = pk[h + u2];
+            v = pv[h + u2];
+            mi = u2 + lower_bound_g(tw, c, k);
+        }
+        const uint64_t wb = pw[h];
+        const uint32_t slot = redistribute_slot(f, S, m, mi);
+        keys[wb + slot] = k;
+        values[wb + slot] = v;
+        if (is_guard(k)) row_offset[(k >> 32) + 1] = (uint32_t)(wb + slot);
+        if (slot % S == 0) seps[(wb + slot) / S] = k;
+    }
+}
+
+// Gap slots of the passing windows (the slots the merge left free).
+__global__ __launch_bounds__(256) void k_wide_fill(const uint32_t *__restrict__ pw, const uint32_t *__restrict__ heads,
+                                                   const uint32_t *__restrict__ d_nruns,
+                                                   const uint32_t *__restrict__ merge, const uint8_t *__restrict__ ok,
+                                                   uint32_t wshift, uint32_t S, gkey *__restrict__ keys,
+                                                   gval *__restrict__ values, gkey *__restrict__ seps) {
+    const uint64_t W = 1ull << wshift;
+    const uint64_t n = (uint64_t)*d_nruns << wshift;
+    const uint64_t f = W / S;
+    for (uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x; p < n; p += (uint64_t)gridDim.x * 256) {
+        const uint32_t j = (uint32_t)(p >> wshift);
+        if (!ok[j]) continue;
+        const uint64_t l = p & (W - 1), s = l / S, within = l % S, m = merge[j];
+        if (within >= seg_first_item(f, m, s + 1) - seg_first_item(f, m, s)) {
+            const uint64_t q = (uint64_t)pw[heads[j]] + l;
+            keys[q] = KEY_NONE;
+            values[q] = 0;
+            if (within == 0) seps[q / S] = KEY_NONE;
+        }
+    }
+}
+
+// Done flags of the pending entries of the passing windows.
+__global__ __launch_bounds__(256) void k_wide_done(const uint32_t *__restrict__ heads,
+                                                   const uint32_t *__restrict__ d_nruns,
+                                                   const uint32_t *__restrict__ d_np, const uint8_t *__restrict__ ok,
+                                                   uint8_t *__restrict__ pd) {
+    const uint32_t nruns = *d_nruns;
+    const uint64_t np = *d_np;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < np; i += (uint64_t)gridDim.x * 256)
+        if (ok[last_le(heads, nruns, i)]) pd[i] = 1;
+}
+
 // (count, wrapping sum, xor) of the live keys of keys[0 .. C): a
 // size-independent digest of the live set. out[0..3) must be zero.
 __global__ __launch_bounds__(256) void k_live_digest(const gkey *__restrict__ keys, const gval *__restrict__ values,
@@ -974,6 +1203,27 @@ void launch_merge_redistribute(const gkey *tk, const gval *tv, uint64_t c, const
     if (m) hipLaunchKernelGGL(k_merge_scatter, dim3(grid_for(m, 256, 8192)), dim3(256), 0, st, tk, tv, c, ik, iv, r,
                               keys, values, seps, row_offset, w, W, S);
     hipLaunchKernelGGL(k_fill_gaps, dim3(grid_for(W, 256, 8192)), dim3(256), 0, st, m, keys, values, seps, w, W, S);
+}
+
+void launch_wide_level(const gkey *pk, const gval *pv, const uint32_t *pw, uint8_t *pd, uint64_t n_ub,
+                       const uint32_t *d_np, uint64_t W, uint32_t S, uint32_t lower, uint32_t upper, gkey *keys,
+                       gval *values, gkey *seps, uint32_t *row_offset, const WideWork &w, hipStream_t st) {
+    if (!n_ub) return;
+    const uint32_t cpw = (uint32_t)(W / CHUNK);
+    uint32_t wshift = 0;
+    while ((1ull << wshift) < W) wshift++;
+    launch_run_heads(pw, n_ub, d_np, w.blkcnt, w.blkoff, w.heads, w.d_nruns, st);
+    const dim3 grid(4096), blk(256);
+    hipLaunchKernelGGL(k_wide_count, grid, blk, 0, st, keys, values, pw, w.heads, w.d_nruns, cpw, w.chunkcnt);
+    hipLaunchKernelGGL(k_wide_plan, dim3(1), dim3(1024), 0, st, w.chunkcnt, w.heads, w.d_nruns, d_np, cpw, lower,
+                       upper, w.chunkoff, w.live, w.merge, w.ok, w.mpre);
+    hipLaunchKernelGGL(k_wide_compact, grid, blk, 0, st, keys, values, pw, w.heads, w.d_nruns, cpw, w.chunkoff, w.ok,
+                       w.tk, w.tv);
+    hipLaunchKernelGGL(k_wide_merge, grid, blk, 0, st, w.tk, w.tv, pk, pv, pw, w.heads, w.d_nruns, cpw, w.chunkoff,
+                       w.live, w.merge, w.mpre, W, S, keys, values, seps, row_offset);
+    hipLaunchKernelGGL(k_wide_fill, grid, blk, 0, st, pw, w.heads, w.d_nruns, w.merge, w.ok, wshift, S, keys, values,
+                       seps);
+    hipLaunchKernelGGL(k_wide_done, grid, blk, 0, st, w.heads, w.d_nruns, d_np, w.ok, pd);
 }
 
 void launch_live_digest(const gkey *keys, const gval *values, uint64_t C, unsigned long long *out, hipStream_t st) {

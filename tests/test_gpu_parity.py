@@ -214,6 +214,38 @@ def test_million_record_batches():
     assert reach == oreach and np.array_equal(lv, olv)
 
 
+def test_wide_window_levels_match_oracle_layout():
+    """The 4.2M-record window on 2^20 vertices (under the bulk threshold)
+    climbs through rebalance levels whose windows (8,192 slots and more) are
+    wider than one workgroup; all windows of such a level go through one set
+    of device-wide passes. The layout (keys, live flags, row_offset) equals
+    the oracle slot for slot after the window and after a 2M-record slide,
+    and the oracle's level statistics confirm windows tried at every level
+    9..17 and placed at some of them."""
+    N = 1 << 20
+    src, dst = G.gen_rmat(20, 1 << 23, seed=17)
+    keys = G.make_keys(src, dst)
+    half = len(keys) // 2
+    g = G.GPMA(N)
+    o = O.OracleGPMA(N)
+    ones = np.ones(half, np.uint8)
+    G.update_gpma(g, keys[:half], ones)
+    o.update(keys[:half], ones)
+    assert g.stats()["bulk_loads"] == 0
+    assert_layout_equal(g, o, "window")
+    st = o.level_stats()
+    assert all(st[L][0] > 0 for L in range(9, 18)), st[9:18]
+    assert st[9:18, 1].sum() > 0, st[9:18]
+    step = half // 4
+    b = np.concatenate([keys[half: half + step], keys[:step]])
+    v = np.concatenate([np.ones(step, np.uint8), np.zeros(step, np.uint8)])
+    G.update_gpma(g, b, v)
+    o.update(b, v)
+    assert_layout_equal(g, o, "slide")
+    assert_invariants(g)
+    g.close()
+
+
 @pytest.mark.parametrize("pull_permille", [0, 1, 1000])
 def test_bfs_hubs_and_direction_modes(pull_permille):
     """Hub vertices (ranges far above the 2048-slot split) and both expansion
