@@ -460,8 +460,10 @@ def run_large(G, args):
     ok_ms = bool(np.array_equal(lv_all[0], G.gpma_bfs(g, int(starts[0]))[0]))
     mb, me = ms_bfs_bytes(N, outdeg, lv_all, 64)
     del lv_all
-    dlev.free()
     ms_s = float(np.median(ms_t)) if ms_t else float("nan")
+    batched = run_large_batched(G, g, keys, N, half, nsl * step, half + nsl * step, dlev, args.large_batched_steps,
+                                mb, me) if args.large_batched_steps else None
+    dlev.free()
     per_ms = np.array(per) * 1e3
     st = g.stats()
     g.close()
@@ -487,7 +489,77 @@ def run_large(G, args):
         "layout": {"capacity": st["capacity"], "segment_length": st["segment_length"],
                    "tree_height": st["tree_height"], "device_bytes": st.get("device_bytes")},
         "parity": {"live_set_count_sum_xor": ok_live, "msbfs_row0_equals_bfs": ok_ms},
+        "batched": batched,
     }
+
+
+def run_large_batched(G, g, keys, N, half, dele, ins, dlev, steps, ms_bytes, ms_edges, fused=8, hb=1 << 18,
+                      bfs_every=5):
+    """Batched form of the large configuration on the handle the plain slides
+    left: captured steps (gpma_step) of `fused` batches of hb inserts + hb
+    deletes each (4M records, two plain slides' worth), the window sliding
+    on from where the plain slides stopped, with a 64-source BFS in every
+    `bfs_every`-th step (once per ten slide-equivalents, as in the plain
+    protocol). Update rate from the update-only steps; the in-step BFS time
+    is the BFS steps' mean minus the update-only steps' mean."""
+    per = fused * 2 * hb
+    total = steps + 2
+    if ins + total * fused * hb > len(keys):
+        log("large batched: not enough edges left, skipped")
+        return None
+    st_keys = np.empty(total * per, np.uint64)
+    for i in range(total):
+        for b in range(fused):
+            a0 = i * per + b * 2 * hb
+            st_keys[a0: a0 + hb] = keys[ins: ins + hb]
+            st_keys[a0 + hb: a0 + 2 * hb] = keys[dele: dele + hb]
+            ins += hb
+            dele += hb
+    vals = np.tile(np.concatenate([np.ones(hb, np.uint8), np.zeros(hb, np.uint8)]), total * fused)
+    dk = G.DeviceBuffer.from_numpy(st_keys)
+    dv = G.DeviceBuffer.from_numpy(vals)
+    del st_keys, vals
+    offs = np.arange(fused + 1, dtype=np.uint64) * np.uint64(2 * hb)
+    rng = np.random.default_rng(30)
+    starts = [rng.integers(0, N, 64).astype(np.uint32) for _ in range(total)]
+    G.gpma_step_prepare(g, per, 0, None)
+    G.gpma_step_prepare(g, per, 64, dlev.ptr)
+    for i in range(2):  # warm-up: one update-only step, one with the BFS
+        G.gpma_step(g, dk.ptr + i * per * 8, dv.ptr + i * per, offs, starts[i] if i else (),
+                    levels_device_ptr=dlev.ptr)
+    g.sync()
+    upd_t, bfs_t = [], []
+    t0 = time.perf_counter()
+    for i in range(2, total):
+        with_bfs = (i - 1) % bfs_every == 0
+        a = time.perf_counter()
+        G.gpma_step(g, dk.ptr + i * per * 8, dv.ptr + i * per, offs, starts[i] if with_bfs else (),
+                    levels_device_ptr=dlev.ptr)
+        (bfs_t if with_bfs else upd_t).append(time.perf_counter() - a)
+    t_all = time.perf_counter() - t0
+    dk.free()
+    dv.free()
+    guards = (np.arange(N, dtype=np.uint64) << np.uint64(32)) | np.uint64(0xFFFFFFFF)
+    expect = G.combine_digests(G.key_digest(keys[dele: ins]), G.key_digest(guards),
+                               G.key_digest(np.array([G.KEY_MAX, G.KEY_MAX], np.uint64)))
+    ok_live = g.live_digest() == expect
+    sst = G.gpma_step_stats(g)
+    u = float(np.mean(upd_t)) if upd_t else float("nan")
+    b = float(np.mean(bfs_t)) if bfs_t else float("nan")
+    ub = update_bytes(fused * hb, fused * hb)["total"]
+    ms_in_step = b - u
+    out = {"sample": f"{steps} captured steps after the plain slides, each {fused} fused batches of {hb} inserts + "
+                     f"{hb} deletes ({per} records), a 64-source BFS in every {bfs_every}th step",
+           "steps": steps, "records_per_step": per, "edge_updates_per_s": per / u,
+           "update_step_ms": u * 1e3, "bfs_step_ms": b * 1e3, "msbfs_in_step_ms": ms_in_step * 1e3,
+           "msbfs_gteps_est": ms_edges / ms_in_step / 1e9 if ms_in_step > 0 else None,
+           "update_frac": ub / u / 1e9 / HBM_PEAK_GBS,
+           "protocol_edge_updates_per_s": steps * per / t_all,
+           "note": "BFS GTEPS from the plain protocol's last 64-source traversal's edge count",
+           "step_stats": sst, "parity": {"live_set_count_sum_xor": bool(ok_live)}}
+    log(f"large batched: {per / u / 1e9:.3f} G edge-updates/s on update-only steps ({u * 1e3:.2f} ms per "
+        f"{per}-record step), BFS step {b * 1e3:.2f} ms; replays {sst['replays']}; live set ok {ok_live}")
+    return out
 
 
 def run_serve(G, requests, rate, order="snapshot", saturation=True, seed=4):
@@ -647,6 +719,8 @@ def main():
     ap.add_argument("--large", action="store_true",
                     help="R-MAT 2^25 nodes / 2^29 draws instead (bulk load, 2^21-update slides, 64-source BFS)")
     ap.add_argument("--large-steps", type=int, default=200)
+    ap.add_argument("--large-batched-steps", type=int, default=20,
+                    help="captured batched steps after the plain slides of --large (0 = skip)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="end with one device-to-device copy of this many MiB (counter calibration)")
     ap.add_argument("--serve", action="store_true",
