@@ -199,6 +199,7 @@ struct gpma_handle {
     Buf ms_seen, ms_fr0, ms_fr1, ms_nx, ms_lv16, ms_q0, ms_q1, ms_qb0, ms_qb1, ms_touched, ms_cnt, ms_cntb, ms_tcnt,
         ms_partial, ms_starts, ms_out, ms_reach;
     Buf ms_lv32;          // u32 level staging, allocated when a traversal reaches level 65,535
+    Buf ms_ev, ms_evn;    // level events per vertex, and their counts
     bool ms_wide = false; // the running multi-source traversal stages u32 levels
     unsigned long long *ms_hcnt = nullptr;
     unsigned long long *hcnt = nullptr;  // pinned mirror of one chunk of bcnt and bcntb
@@ -916,6 +917,8 @@ int ensure_msbfs(gpma_handle *g) {
     TRY(g->ms_fr1.ensure(n * 8, t));
     TRY(g->ms_nx.ensure(n * 8, t));
     TRY(g->ms_lv16.ensure(n * 128, t));
+    TRY(g->ms_ev.ensure(n * 128, t));
+    TRY(g->ms_evn.ensure((n + 3) / 4 * 4, t));
     TRY(g->ms_q0.ensure(n * 4, t));
     TRY(g->ms_q1.ensure(n * 4, t));
     TRY(g->ms_qb0.ensure(n * 4, t));
@@ -940,6 +943,8 @@ MsState ms_state(gpma_handle *g) {
     m.fr[1] = g->ms_fr1.as<uint64_t>();
     m.nx = g->ms_nx.as<uint64_t>();
     m.lv16 = g->ms_lv16.as<uint16_t>();
+    m.ev = g->ms_ev.as<uint4>();
+    m.evn = g->ms_evn.as<uint8_t>();
     m.lv32 = g->ms_wide ? g->ms_lv32.as<uint32_t>() : nullptr;
     m.q[0] = g->ms_q0.as<uint32_t>();
     m.q[1] = g->ms_q1.as<uint32_t>();
@@ -1060,7 +1065,7 @@ std::vector<uint64_t> step_signature(const gpma_handle *g, uint32_t cls, uint32_
                          &g->pw1, &g->pd1, &g->ccstat, &g->msd_kp, &g->msd_rk, &g->msd_mat, &g->msd_bstart,
                          &g->msd_status, &g->ms_seen, &g->ms_fr0, &g->ms_fr1, &g->ms_nx, &g->ms_lv16, &g->ms_q0,
                          &g->ms_q1, &g->ms_qb0, &g->ms_qb1, &g->ms_touched, &g->ms_cnt, &g->ms_cntb, &g->ms_tcnt,
-                         &g->ms_partial, &g->ms_starts, &g->ms_reach};
+                         &g->ms_partial, &g->ms_starts, &g->ms_reach, &g->ms_ev, &g->ms_evn};
     for (const Buf *b : bufs) {
         v.push_back((uint64_t)(uintptr_t)b->p);
         v.push_back(b->bytes);
@@ -1498,7 +1503,8 @@ int gpma_destroy(gpma_t g) {
                   &g->chunkoff, &g->lev, &g->fr0, &g->fr1, &g->fb0, &g->fb1, &g->bcnt, &g->bcntb,
                   &g->ms_seen, &g->ms_fr0, &g->ms_fr1, &g->ms_nx, &g->ms_lv16, &g->ms_q0, &g->ms_q1, &g->ms_qb0,
                   &g->ms_qb1, &g->ms_touched, &g->ms_cnt, &g->ms_cntb, &g->ms_tcnt, &g->ms_partial, &g->ms_starts,
-                  &g->ms_out, &g->ms_reach, &g->ms_lv32, &g->ctr, &g->digest, &g->step_n, &g->step_stage,
+                  &g->ms_out, &g->ms_reach, &g->ms_lv32, &g->ms_ev, &g->ms_evn, &g->ctr, &g->digest, &g->step_n,
+                  &g->step_stage,
                   &g->wide};
     drop_graphs(g);
     for (Buf *b : all) b->release(&g->mem);

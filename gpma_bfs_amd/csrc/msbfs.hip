@@ -19,11 +19,17 @@
 // over every vertex in order. On the power-law 1.63M-node graph with 64
 // random sources this halves the widest levels' expand time (the queued
 // form's ORs wait for their old values) for a somewhat slower finalize.
-// Levels are kept as u16 in a vertex-major scratch (a vertex's 64 levels in
-// one 128-byte line) and transposed at the end into the caller's
-// source-major u32 arrays, with per-source reachable counts. A traversal that
-// reaches level 65,535 first widens the staging to u32 (k_ms_widen; 256-byte
-// rows) and keeps writing u32 levels from there, so no depth wraps.
+// Levels are recorded as events: each time a vertex w gets newly seen
+// sources, the lane that owns w appends one 16-byte event (the fresh source
+// mask and the level) to w's own list of MS_EVENTS slots, one vector store
+// for the whole wave. The output pass rebuilds every source's level from a
+// vertex's events (the sources in one event share its level) and transposes
+// them into the caller's source-major u32 arrays, with per-source reachable
+// counts. A vertex reached at more than MS_EVENTS distinct levels records
+// the rest in a vertex-major row (a vertex's 64 levels as u16 in one 128-byte
+// line, written by a loop over the wave's overflowing lanes). A traversal
+// that reaches level 65,535 first widens those rows to u32 (k_ms_widen;
+// 256-byte rows), so no depth wraps.
 //
 // The level of source s at w is 1 + hop distance, as in the single-source
 // BFS (gpma_bfs.cuh:446-509 semantics), so every source's array equals the
@@ -44,6 +50,8 @@ constexpr uint32_t MS_BIG_CHUNK = 2048;
 constexpr uint32_t MS_BIG_LIST = 2048;
 constexpr uint32_t MS_PULL_PER_THREAD = 16;
 constexpr uint32_t LQ = 2048;
+constexpr uint32_t MS_EVENTS = 8;     // event slots per vertex (16 bytes each)
+constexpr uint8_t EV_OVF = 0x80;      // event count flag: levels also in the overflow row
 
 __device__ __forceinline__ uint32_t rlen(const uint32_t *ro, uint32_t v) {
     const uint32_t b = ro[v], e = ro[v + 1];
@@ -54,7 +62,7 @@ __device__ __forceinline__ uint32_t rlen(const uint32_t *ro, uint32_t v) {
 // reads an entry only where the source's bit is set in seen[v], and every
 // such entry was written when that bit was set.
 __global__ void k_ms_init(uint64_t *__restrict__ seen, uint64_t *__restrict__ fr0, uint64_t *__restrict__ fr1,
-                          uint64_t *__restrict__ nx, uint32_t N) {
+                          uint64_t *__restrict__ nx, uint8_t *__restrict__ evn, uint32_t N) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     for (uint64_t i = t; i < N; i += stride) {
@@ -63,11 +71,14 @@ __global__ void k_ms_init(uint64_t *__restrict__ seen, uint64_t *__restrict__ fr
         fr1[i] = 0;
         nx[i] = 0;
     }
+    uint32_t *e4 = reinterpret_cast<uint32_t *>(evn);  // N rounded up to 4 bytes
+    for (uint64_t i = t; i < ((uint64_t)N + 3) / 4; i += stride) e4[i] = 0;
 }
 // One lane per source: mark its bit, stage level 1, and queue the first
 // occurrence of each start vertex (short or long range list).
 __global__ void k_ms_sources(const uint32_t *__restrict__ starts, uint32_t nsrc, uint64_t *__restrict__ seen,
-                             uint64_t *__restrict__ fr, uint16_t *__restrict__ lv16, uint32_t *__restrict__ q,
+                             uint64_t *__restrict__ fr, uint4 *__restrict__ ev, uint8_t *__restrict__ evn,
+                             uint32_t *__restrict__ q,
                              uint32_t *__restrict__ qbig, unsigned long long *__restrict__ cnt,
                              unsigned long long *__restrict__ cnt_big, uint32_t *__restrict__ tcnt,
                              const uint32_t *__restrict__ ro, const uint32_t *__restrict__ gate) {
@@ -84,7 +95,13 @@ __global__ void k_ms_sources(const uint32_t *__restrict__ starts, uint32_t nsrc,
     if (valid) {
         atomicOr((unsigned long long *)(seen + s), 1ull << lane);
         atomicOr((unsigned long long *)(fr + s), 1ull << lane);
-        lv16[(uint64_t)s * 64 + lane] = 1;
+    }
+    if (first) {  // level 1: one event per distinct start, all of its sources
+        uint64_t m = 0;
+        for (uint32_t j = lane; j < nsrc; j++)
+            if (ss[j] == s) m |= 1ull << j;
+        ev[(uint64_t)s * MS_EVENTS] = make_uint4((uint32_t)m, (uint32_t)(m >> 32), 1u, 0u);
+        evn[s] = 1;
     }
     const uint64_t len = first ? rlen(ro, s) : 0;
     const bool big = first && len > MS_BIG_RANGE, small = first && !big;
@@ -327,6 +344,7 @@ __global__ __launch_bounds__(T) void k_ms_finalize(const uint32_t *__restrict__ 
                                                    uint64_t *__restrict__ fr_cur, uint64_t *__restrict__ fr_next,
                                                    uint64_t *__restrict__ seen, uint64_t *__restrict__ nx,
                                                    uint16_t *__restrict__ lv16, uint32_t *__restrict__ lv32,
+                                                   uint4 *__restrict__ ev, uint8_t *__restrict__ evn,
                                                    const uint32_t *__restrict__ touched,
                                                    const uint32_t *__restrict__ tcnt, uint32_t *__restrict__ q2,
                                                    uint32_t *__restrict__ qbig2, unsigned long long *__restrict__ cnt2,
@@ -355,7 +373,7 @@ __global__ __launch_bounds__(T) void k_ms_finalize(const uint32_t *__restrict__ 
     const bool roomy = (((uint64_t)nt + stride - 1) / stride) * T <= LQ;
     for (uint64_t i0 = (uint64_t)blockIdx.x * T; i0 < nt; i0 += stride) {
         const uint64_t i = i0 + threadIdx.x;
-        bool small = false, big = false;
+        bool small = false, big = false, ovf = false;
         uint32_t w = 0, len = 0;
         uint64_t fresh = 0;
         if (i < nt) {
@@ -363,6 +381,7 @@ __global__ __launch_bounds__(T) void k_ms_finalize(const uint32_t *__restrict__ 
             const uint64_t acc = nx[w];
             if (acc) nx[w] = 0;
             const uint64_t sn = acc ? seen[w] : ~0ull;
+            const uint8_t en = acc ? evn[w] : 0;
             fresh = acc & ~sn;
             if (fresh) {
                 seen[w] = sn | fresh;
@@ -370,11 +389,20 @@ __global__ __launch_bounds__(T) void k_ms_finalize(const uint32_t *__restrict__ 
                 len = rlen(ro, w);
                 small = len <= MS_BIG_RANGE;
                 big = !small;
+                // the level of the fresh sources: one event, or the overflow row
+                const uint32_t k = en & ~EV_OVF;
+                if (k < MS_EVENTS) {
+                    ev[(uint64_t)w * MS_EVENTS + k] = make_uint4((uint32_t)fresh, (uint32_t)(fresh >> 32), newlev, 0u);
+                    evn[w] = (uint8_t)(en + 1);
+                } else {
+                    ovf = true;
+                    if (!(en & EV_OVF)) evn[w] = (uint8_t)(en | EV_OVF);
+                }
             }
         }
         {
             const uint32_t lane = lane_id();
-            for (uint64_t todo = __ballot(fresh != 0); todo; todo &= todo - 1) {
+            for (uint64_t todo = __ballot(ovf); todo; todo &= todo - 1) {
                 const int j = __ffsll((unsigned long long)todo) - 1;
                 const uint32_t wj = (uint32_t)__builtin_amdgcn_readlane((int)w, j);
                 const uint32_t flo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)fresh, j);
@@ -427,20 +455,20 @@ __global__ __launch_bounds__(T) void k_ms_finalize(const uint32_t *__restrict__ 
     }
 }
 
-// Transpose the vertex-major u16 levels into source-major u32 arrays (an
-// entry counts only where the source's bit is set in seen[v]) and count
-// reachable vertices per source. One wave per 64-vertex group, no LDS and no
-// barrier: each lane loads its vertex's 128-byte level row (only if some
-// source reached it), then for every source the wave writes the group's 64
-// levels as one coalesced store, and lane s counts source s with a ballot.
-// partial[wave][s]: per-wave counts. Workgroup 0 also zeroes reach[0 .. nsrc)
-// for k_ms_reach.
-// WIDE: the staging holds u32 levels (256-byte rows, read as 16 x 16 bytes).
+// Source-major u32 levels from the events (and the overflow rows), with
+// per-source reachable counts. One wave per 64-vertex group, no LDS and no
+// barrier: each lane loads its vertex's events (only if some source reached
+// it): source s's level is the level of the event whose mask holds bit s, or
+// the overflow row's entry if none does. Then for every source the wave
+// writes the group's 64 levels as one coalesced store, and lane s counts
+// source s with a ballot. partial[wave][s]: per-wave counts. Workgroup 0 also
+// zeroes reach[0 .. nsrc) for k_ms_reach.
+// WIDE: the overflow rows hold u32 levels, else u16.
 template <bool WIDE>
-__global__ __launch_bounds__(T) void k_ms_output(const void *__restrict__ lv, const uint64_t *__restrict__ seen,
+__global__ __launch_bounds__(T) void k_ms_output(const uint4 *__restrict__ ev, const uint8_t *__restrict__ evn,
+                                                 const void *__restrict__ lv, const uint64_t *__restrict__ seen,
                                                  uint32_t N, uint32_t nsrc, uint32_t *__restrict__ out,
                                                  uint32_t *__restrict__ partial, uint32_t *__restrict__ reach) {
-    constexpr uint32_t RW = WIDE ? 16 : 8;  // 16-byte words per vertex row
     const uint32_t lane = lane_id();
     if (blockIdx.x == 0 && threadIdx.x < nsrc) reach[threadIdx.x] = 0;  // k_ms_reach adds into it
     const uint32_t gw = (blockIdx.x * T + threadIdx.x) / WAVE;
@@ -450,27 +478,22 @@ __global__ __launch_bounds__(T) void k_ms_output(const void *__restrict__ lv, co
         const uint64_t v = v0 + lane;
         const bool in = v < N;
         const uint64_t mask = in ? seen[v] : 0;
-        uint4 row[RW];
-        if (mask) {
-            const uint4 *src = reinterpret_cast<const uint4 *>(lv) + v * RW;
+        const uint8_t en = mask ? evn[v] : 0;
+        const uint32_t ne = en & ~EV_OVF;
+        uint4 e[MS_EVENTS];
 #pragma unroll
-            for (uint32_t q = 0; q < RW; q++) row[q] = src[q];
-        } else {
-#pragma unroll
-            for (uint32_t q = 0; q < RW; q++) row[q] = make_uint4(0, 0, 0, 0);
-        }
+        for (uint32_t k = 0; k < MS_EVENTS; k++)
+            e[k] = k < ne ? ev[v * MS_EVENTS + k] : make_uint4(0, 0, 0, 0);
 #pragma unroll
         for (uint32_t s = 0; s < 64; s++) {
-            uint32_t level;
-            if (WIDE) {
-                const uint4 &r4 = row[s / 4];
-                level = (s % 4) == 0 ? r4.x : (s % 4) == 1 ? r4.y : (s % 4) == 2 ? r4.z : r4.w;
-            } else {
-                const uint4 &r4 = row[s / 8];
-                const uint32_t word = (s % 8) < 2 ? r4.x : (s % 8) < 4 ? r4.y : (s % 8) < 6 ? r4.z : r4.w;
-                level = (word >> (16 * (s & 1))) & 0xFFFFu;
-            }
+            uint32_t level = 0;
+#pragma unroll
+            for (uint32_t k = 0; k < MS_EVENTS; k++)
+                if ((((s < 32 ? e[k].x : e[k].y) >> (s & 31)) & 1u)) level = e[k].z;
             const bool hit = (mask >> s) & 1ull;
+            if (hit && level == 0 && (en & EV_OVF))
+                level = WIDE ? reinterpret_cast<const uint32_t *>(lv)[v * 64 + s]
+                             : reinterpret_cast<const uint16_t *>(lv)[v * 64 + s];
             if (s < nsrc && in) out[(uint64_t)s * N + v] = hit ? level : 0u;
             const uint32_t c = (uint32_t)__popcll(ballot(hit));
             if (lane == s) acc += c;
@@ -524,8 +547,8 @@ __global__ void k_ms_clear(unsigned long long *__restrict__ cnt, unsigned long l
 
 void ms_init(const MsState &m, uint32_t N, const uint32_t *starts, uint32_t nsrc, const uint32_t *ro, hipStream_t st,
              const uint32_t *gate) {
-    hipLaunchKernelGGL(k_ms_init, dim3(2048), dim3(256), 0, st, m.seen, m.fr[0], m.fr[1], m.nx, N);
-    hipLaunchKernelGGL(k_ms_sources, dim3(1), dim3(64), 0, st, starts, nsrc, m.seen, m.fr[1], m.lv16, m.q[1],
+    hipLaunchKernelGGL(k_ms_init, dim3(2048), dim3(256), 0, st, m.seen, m.fr[0], m.fr[1], m.nx, m.evn, N);
+    hipLaunchKernelGGL(k_ms_sources, dim3(1), dim3(64), 0, st, starts, nsrc, m.seen, m.fr[1], m.ev, m.evn, m.q[1],
                        m.qbig[1], m.cnt, m.cnt_big, m.tcnt, ro, gate);
 }
 
@@ -542,7 +565,8 @@ void ms_level(const gkey *keys, const gval *values, uint64_t C, uint32_t N, cons
                        m.qbig[cur], m.cnt, m.cnt_big, m.fr[cur], m.seen, m.nx, m.touched, m.tcnt + level, level,
                        pull_threshold, small_grid, std::min(MS_TILE, std::max(MS_MIN_TILE, m.tile_cap)));
     hipLaunchKernelGGL(k_ms_finalize, dim3(1024), dim3(T), 0, st, m.q[cur], m.qbig[cur], m.cnt, m.cnt_big, m.fr[cur],
-                       m.fr[nxt], m.seen, m.nx, m.lv16, m.lv32, m.touched, m.tcnt + level, m.q[nxt], m.qbig[nxt],
+                       m.fr[nxt], m.seen, m.nx, m.lv16, m.lv32, m.ev, m.evn, m.touched, m.tcnt + level, m.q[nxt],
+                       m.qbig[nxt],
                        m.cnt + level + 1, m.cnt_big + level + 1, ro, N, level);
 }
 
@@ -554,11 +578,11 @@ static uint32_t ms_output_grid(uint32_t N) {
 void ms_output(const MsState &m, uint32_t N, uint32_t nsrc, uint32_t *out, uint32_t *reach, hipStream_t st) {
     const uint32_t nb = ms_output_grid(N);
     if (m.lv32)
-        hipLaunchKernelGGL(k_ms_output<true>, dim3(nb), dim3(T), 0, st, (const void *)m.lv32, m.seen, N, nsrc, out,
-                           m.partial, reach);
+        hipLaunchKernelGGL(k_ms_output<true>, dim3(nb), dim3(T), 0, st, m.ev, m.evn, (const void *)m.lv32, m.seen, N,
+                           nsrc, out, m.partial, reach);
     else
-        hipLaunchKernelGGL(k_ms_output<false>, dim3(nb), dim3(T), 0, st, (const void *)m.lv16, m.seen, N, nsrc, out,
-                           m.partial, reach);
+        hipLaunchKernelGGL(k_ms_output<false>, dim3(nb), dim3(T), 0, st, m.ev, m.evn, (const void *)m.lv16, m.seen, N,
+                           nsrc, out, m.partial, reach);
     hipLaunchKernelGGL(k_ms_reach, dim3(RB), dim3(RT), 0, st, m.partial, nb * (T / WAVE), nsrc, reach);
 }
 

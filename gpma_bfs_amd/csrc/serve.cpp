@@ -58,7 +58,7 @@ enum Kind : uint32_t { K_UPDATE = 0, K_BFS = 1 };
 // segment of the new array, below 1 B per slot).
 constexpr uint64_t UPDATE_BYTES_PER_RECORD = 128;
 constexpr uint64_t STAGE_BYTES_PER_RECORD = 9;
-constexpr uint64_t MSBFS_BYTES_PER_ROW = 256;
+constexpr uint64_t MSBFS_BYTES_PER_ROW = 384;
 constexpr uint64_t REBUILD_BYTES_PER_SLOT = 28;
 constexpr uint64_t MIN_FUSED = 1024;
 

@@ -193,7 +193,7 @@ def test_server_budget_reserves_rebuild_peak():
     N = 4096
     g = G.GPMA(N)
     C = g.get_size()
-    bfs = 1 * N * 4 + N * 256  # one BFS per group: its levels and the traversal scratch
+    bfs = 1 * N * 4 + N * 384  # one BFS per group: its levels and the traversal scratch
     with G.GpmaServer(g, max_bfs_group=1, hbm_budget_bytes=28 * C + bfs + 146 * 50000) as srv:
         st = srv.stats()
         assert st["growth_reserve_bytes"] == 28 * C
