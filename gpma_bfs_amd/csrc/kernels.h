@@ -193,7 +193,7 @@ void launch_bfs_level(const gkey *keys, const gval *values, uint64_t C, uint32_t
 // Device state of one multi-source BFS (<= 64 sources, bit i = starts[i]).
 struct MsState {
     uint64_t *seen, *fr[2], *nx;
-    uint4 *ev;                 // [N][8] level events: fresh source mask (x, y), level (z)
+    uint4 *ev;                 // [8][N] level events, slot-major: fresh source mask (x, y), level (z)
     uint8_t *evn;              // [N] events recorded (flag 0x80: more levels in the overflow row)
     uint16_t *lv16;            // [N][64] overflow levels, vertex-major
     uint32_t *lv32;            // [N][64] u32 levels once the traversal passed level 65,534 (else null)

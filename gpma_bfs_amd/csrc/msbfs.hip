@@ -22,7 +22,8 @@
 // Levels are recorded as events: each time a vertex w gets newly seen
 // sources, the lane that owns w appends one 16-byte event (the fresh source
 // mask and the level) to w's own list of MS_EVENTS slots, one vector store
-// for the whole wave. The output pass rebuilds every source's level from a
+// for the whole wave. Slots are slot-major (ev[k][v]), so the output pass
+// reads each slot of 64 consecutive vertices as one coalesced load. The output pass rebuilds every source's level from a
 // vertex's events (the sources in one event share its level) and transposes
 // them into the caller's source-major u32 arrays, with per-source reachable
 // counts. A vertex reached at more than MS_EVENTS distinct levels records
@@ -100,7 +101,7 @@ __global__ void k_ms_sources(const uint32_t *__restrict__ starts, uint32_t nsrc,
         uint64_t m = 0;
         for (uint32_t j = lane; j < nsrc; j++)
             if (ss[j] == s) m |= 1ull << j;
-        ev[(uint64_t)s * MS_EVENTS] = make_uint4((uint32_t)m, (uint32_t)(m >> 32), 1u, 0u);
+        ev[s] = make_uint4((uint32_t)m, (uint32_t)(m >> 32), 1u, 0u);  // slot 0 of vertex s
         evn[s] = 1;
     }
     const uint64_t len = first ? rlen(ro, s) : 0;
@@ -392,7 +393,7 @@ __global__ __launch_bounds__(T) void k_ms_finalize(const uint32_t *__restrict__ 
                 // the level of the fresh sources: one event, or the overflow row
                 const uint32_t k = en & ~EV_OVF;
                 if (k < MS_EVENTS) {
-                    ev[(uint64_t)w * MS_EVENTS + k] = make_uint4((uint32_t)fresh, (uint32_t)(fresh >> 32), newlev, 0u);
+                    ev[(uint64_t)k * N + w] = make_uint4((uint32_t)fresh, (uint32_t)(fresh >> 32), newlev, 0u);
                     evn[w] = (uint8_t)(en + 1);
                 } else {
                     ovf = true;
@@ -480,18 +481,28 @@ __global__ __launch_bounds__(T) void k_ms_output(const uint4 *__restrict__ ev, c
         const uint64_t mask = in ? seen[v] : 0;
         const uint8_t en = mask ? evn[v] : 0;
         const uint32_t ne = en & ~EV_OVF;
-        uint4 e[MS_EVENTS];
+        // the events as bit planes: source s is in event idx(s) = p0 | p1 << 1 | p2 << 2
+        // (a source sits in at most one event), cov = every source some event holds
+        uint64_t p0 = 0, p1 = 0, p2 = 0, cov = 0;
+        uint32_t L[MS_EVENTS];
 #pragma unroll
-        for (uint32_t k = 0; k < MS_EVENTS; k++)
-            e[k] = k < ne ? ev[v * MS_EVENTS + k] : make_uint4(0, 0, 0, 0);
+        for (uint32_t k = 0; k < MS_EVENTS; k++) {
+            const uint4 e = k < ne ? ev[(uint64_t)k * N + v] : make_uint4(0, 0, 0, 0);
+            const uint64_t m = (uint64_t)e.y << 32 | e.x;
+            L[k] = e.z;
+            cov |= m;
+            if (k & 1) p0 |= m;
+            if (k & 2) p1 |= m;
+            if (k & 4) p2 |= m;
+        }
+        static_assert(MS_EVENTS == 8, "three bit planes");
 #pragma unroll
         for (uint32_t s = 0; s < 64; s++) {
-            uint32_t level = 0;
-#pragma unroll
-            for (uint32_t k = 0; k < MS_EVENTS; k++)
-                if ((((s < 32 ? e[k].x : e[k].y) >> (s & 31)) & 1u)) level = e[k].z;
+            const bool b0 = (p0 >> s) & 1ull, b1 = (p1 >> s) & 1ull, b2 = (p2 >> s) & 1ull;
+            const uint32_t t0 = b0 ? L[1] : L[0], t1 = b0 ? L[3] : L[2], t2 = b0 ? L[5] : L[4], t3 = b0 ? L[7] : L[6];
+            uint32_t level = b2 ? (b1 ? t3 : t2) : (b1 ? t1 : t0);
             const bool hit = (mask >> s) & 1ull;
-            if (hit && level == 0 && (en & EV_OVF))
+            if (hit && !((cov >> s) & 1ull))  // past the event slots: the overflow row
                 level = WIDE ? reinterpret_cast<const uint32_t *>(lv)[v * 64 + s]
                              : reinterpret_cast<const uint16_t *>(lv)[v * 64 + s];
             if (s < nsrc && in) out[(uint64_t)s * N + v] = hit ? level : 0u;
