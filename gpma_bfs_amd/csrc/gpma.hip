@@ -587,7 +587,7 @@ uint32_t next_epoch(gpma_handle *g) {
 // launch_compact_chained for the arguments.
 void compact_step(gpma_handle *g, const gkey *ik, const gval *iv, const uint32_t *iw, const uint8_t *idone,
                   uint64_t n_ub, const uint32_t *d_n, gkey *ok, gval *ov, uint32_t *ow, uint8_t *odone, uint32_t mask,
-                  uint32_t *d_out) {
+                  uint32_t *d_out, const gkey *fix = nullptr, gkey *seps = nullptr) {
     uint32_t *ctr = g->ctr.as<uint32_t>();
     if (g->cc_ticket >= N_TICKETS) {
         fill_u32(ctr + C_TICKET, 0, N_TICKETS, g->st);
@@ -596,7 +596,7 @@ void compact_step(gpma_handle *g, const gkey *ik, const gval *iv, const uint32_t
     const uint32_t epoch = next_epoch(g);
     launch_compact_chained(ik, iv, iw, idone, n_ub, d_n, ok, ov, ow, odone, mask,
                            g->ccstat.as<unsigned long long>(), epoch, ctr + C_TICKET + g->cc_ticket, d_out,
-                           ctr + C_SORTERR, g->st);
+                           ctr + C_SORTERR, g->st, fix, seps);
     g->cc_ticket++;
 }
 
@@ -647,17 +647,8 @@ uint32_t enqueue_update(gpma_handle *g, const gkey *src_k, const gval *src_v, ui
     }
     const gkey *ks = cur ? g->sk1.as<gkey>() : g->sk0.as<gkey>();
     const gval *vs = cur ? g->sv1.as<gval>() : g->sv0.as<gval>();
-    // locate + in-place delete / modify
-    phase_begin(g, P_LOCATE);
-    launch_locate(ks, vs, n_ub, d_n, g->keys.as<gkey>(), g->values.as<gval>(), g->seps.as<gkey>(), g->C / g->S, g->S,
-                  g->row_offset.as<uint32_t>(), !guards_ok, g->win.as<uint32_t>(), st);
-    phase_end(g, P_LOCATE, 1);
     Buf *PK[2] = {&g->pk0, &g->pk1}, *PV[2] = {&g->pv0, &g->pv1}, *PW[2] = {&g->pw0, &g->pw1},
         *PD[2] = {&g->pd0, &g->pd1};
-    phase_begin(g, P_COMPACT);
-    compact_step(g, ks, vs, g->win.as<uint32_t>(), nullptr, n_ub, d_n, g->pk0.as<gkey>(), g->pv0.as<gval>(),
-                 g->pw0.as<uint32_t>(), g->pd0.as<uint8_t>(), 0xFFFFFFFFu, ctr + C_LVL);
-    phase_end(g, P_COMPACT, 1);
     // Drop placed inserts of list `pc` (count in *d_in), move the rest up to
     // level+1 into the other list, count in *d_out.
     auto compact_pending = [&](int pc, uint32_t level, const uint32_t *d_in, uint32_t *d_out) {
@@ -668,14 +659,39 @@ uint32_t enqueue_update(gpma_handle *g, const gkey *src_k, const gval *src_v, ui
                      PD[pc ^ 1]->as<uint8_t>(), mask, d_out);
         phase_end(g, P_COMPACT, 1);
     };
-    // level 0, then levels 1..spec; every kernel stands down if the batch
-    // turns out to be a bulk load
-    phase_begin(g, P_LEAF);
-    launch_leaf(g->S, g->pk0.as<gkey>(), g->pv0.as<gval>(), g->pw0.as<uint32_t>(), g->pd0.as<uint8_t>(), n_ub,
-                ctr + C_LVL, g->keys.as<gkey>(), g->values.as<gval>(), g->seps.as<gkey>(),
-                g->row_offset.as<uint32_t>(), g->lower[0], g->upper[0], g->bulk_threshold, st);
-    phase_end(g, P_LEAF, 1);
-    compact_pending(0, 0, ctr + C_LVL, ctr + C_LVL + 1);
+    if (!guards_ok && (g->S == 16 || g->S == 32)) {
+        // locate and level 0 in one pass, then the pending inserts of the
+        // leaves that could not take them straight into the level-1 list
+        // (the compaction also publishes the merged leaves' separators)
+        phase_begin(g, P_LOCATE);
+        launch_apply(g->S, ks, vs, n_ub, d_n, g->keys.as<gkey>(), g->values.as<gval>(), g->seps.as<gkey>(),
+                     g->C / g->S, g->row_offset.as<uint32_t>(), g->lower[0], g->upper[0], g->win.as<uint32_t>(),
+                     g->pk0.as<gkey>(), ctr + C_LVL, st);
+        phase_end(g, P_LOCATE, 1);
+        phase_begin(g, P_COMPACT);
+        compact_step(g, ks, vs, g->win.as<uint32_t>(), nullptr, n_ub, d_n, g->pk1.as<gkey>(), g->pv1.as<gval>(),
+                     g->pw1.as<uint32_t>(), g->pd1.as<uint8_t>(), ~(uint32_t)(((uint64_t)g->S << 1) - 1),
+                     ctr + C_LVL + 1, g->pk0.as<gkey>(), g->seps.as<gkey>());
+        phase_end(g, P_COMPACT, 1);
+    } else {
+        // locate + in-place delete / modify
+        phase_begin(g, P_LOCATE);
+        launch_locate(ks, vs, n_ub, d_n, g->keys.as<gkey>(), g->values.as<gval>(), g->seps.as<gkey>(), g->C / g->S,
+                      g->S, g->row_offset.as<uint32_t>(), !guards_ok, g->win.as<uint32_t>(), st);
+        phase_end(g, P_LOCATE, 1);
+        phase_begin(g, P_COMPACT);
+        compact_step(g, ks, vs, g->win.as<uint32_t>(), nullptr, n_ub, d_n, g->pk0.as<gkey>(), g->pv0.as<gval>(),
+                     g->pw0.as<uint32_t>(), g->pd0.as<uint8_t>(), 0xFFFFFFFFu, ctr + C_LVL);
+        phase_end(g, P_COMPACT, 1);
+        // level 0; it and every level above stand down if the batch turns
+        // out to be a bulk load
+        phase_begin(g, P_LEAF);
+        launch_leaf(g->S, g->pk0.as<gkey>(), g->pv0.as<gval>(), g->pw0.as<uint32_t>(), g->pd0.as<uint8_t>(), n_ub,
+                    ctr + C_LVL, g->keys.as<gkey>(), g->values.as<gval>(), g->seps.as<gkey>(),
+                    g->row_offset.as<uint32_t>(), g->lower[0], g->upper[0], g->bulk_threshold, st);
+        phase_end(g, P_LEAF, 1);
+        compact_pending(0, 0, ctr + C_LVL, ctr + C_LVL + 1);
+    }
     uint32_t spec = 0;
     while (spec < dev_levels && spec + 1 <= g->h && ((uint64_t)g->S << (spec + 1)) <= WIN_MAX &&
            spec + 2 < C_LVL_SLOTS) {
@@ -708,8 +724,15 @@ int finish_update(gpma_handle *g, uint64_t n, uint32_t spec, bool used_msd) {
     g->applied += n - inval;
     const uint64_t np0 = g->hctr[C_LVL];
     if (np0 >= g->bulk_threshold) {
+        // The level kernels stood down; their compactions carried every
+        // pending insert up to the list after the last device level. Any
+        // leaf merge of the fused level-0 pass is kept: the rebuild lays
+        // out the live set (array items plus this list) from scratch,
+        // whichever of them a leaf already holds.
         g->last_levels = 0;
-        TRY(rebuild(g, g->pk0.as<gkey>(), g->pv0.as<gval>(), np0, false));
+        const int lc = (int)((spec + 1) & 1);
+        Buf *LK[2] = {&g->pk0, &g->pk1}, *LV[2] = {&g->pv0, &g->pv1};
+        TRY(rebuild(g, LK[lc]->as<gkey>(), LV[lc]->as<gval>(), g->hctr[C_LVL + spec + 1], false));
         TRY(stream_sync(g));
         HIP_TRY(hipGetLastError());
         return inval ? fail(GPMA_ERR_KEY_RANGE, std::to_string(inval) + " keys with src/dst >= row_num skipped")

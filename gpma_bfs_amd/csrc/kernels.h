@@ -67,6 +67,9 @@ void msd_sort(const gkey *ik, const gval *iv, gkey *ok, gval *ov, const MsdPlan 
 
 // ---- update.hip
 constexpr uint32_t CMP_BLOCK = 256;  // items per compaction block
+// win of a record that heads a leaf merged by launch_apply: WIN_SEP | leaf
+// index (leaf indices stay below 2^27, slot indices below 2^31).
+constexpr uint32_t WIN_SEP = 0x80000000u;
 constexpr uint32_t WIN_MAX = 4096;   // widest window rebalanced by one workgroup in LDS
 constexpr uint32_t CHUNK = 4096;     // slots per workgroup in device-wide window passes
 
@@ -106,16 +109,26 @@ void launch_compact(const gkey *ik, const gval *iv, const uint32_t *iw, const ui
 // clears the output done flags and stores the kept count in *d_out. status:
 // compact_status_words(n_ub) words, zeroed once at allocation; epoch: non-zero
 // and distinct per call; *ticket zero before the call; *err set if the
-// look-back spin limit is exceeded.
+// look-back spin limit is exceeded. With fix (after launch_apply): keeps
+// win < WIN_SEP and writes seps[leaf] = fix[i] for win = WIN_SEP | leaf.
 uint64_t compact_status_words(uint64_t n_ub);
 void launch_compact_chained(const gkey *ik, const gval *iv, const uint32_t *iw, const uint8_t *idone, uint64_t n_ub,
                             const uint32_t *d_n, gkey *ok, gval *ov, uint32_t *ow, uint8_t *odone, uint32_t mask,
                             unsigned long long *status, uint32_t epoch, uint32_t *ticket, uint32_t *d_out,
-                            uint32_t *err, hipStream_t st);
+                            uint32_t *err, hipStream_t st, const gkey *fix = nullptr, gkey *seps = nullptr);
 void launch_count_pending(const uint8_t *done, uint64_t n_ub, const uint32_t *d_n, uint32_t *blkcnt,
                           hipStream_t st);
 // Level-0 rebalance: one thread per leaf run; does nothing when *d_np >=
 // bulk_threshold (that batch takes the bulk path).
+// Locate and level 0 in one pass, for S = 16 or 32 with row hints: every
+// sorted record finds its leaf; the first record of each leaf's run applies
+// the run's deletes and modifications and merges its pending inserts into
+// the leaf when level 0's bounds allow (win[head] = WIN_SEP | leaf, the new
+// first key in fix[head]), else leaves them pending (win = leaf base);
+// other records get SIZE_NONE. Adds the level-0 pending count to *np0.
+void launch_apply(uint32_t S, const gkey *ks, const gval *vs, uint64_t n_ub, const uint32_t *d_n, gkey *keys,
+                  gval *values, const gkey *seps, uint64_t nseg, uint32_t *row_offset, uint32_t lower, uint32_t upper,
+                  uint32_t *win, gkey *fix, uint32_t *np0, hipStream_t st);
 void launch_leaf(uint32_t S, const gkey *pk, const gval *pv, const uint32_t *pw, uint8_t *pd, uint64_t n_ub,
                  const uint32_t *d_np, gkey *keys, gval *values, gkey *seps, uint32_t *row_offset, uint32_t lower,
                  uint32_t upper, uint32_t bulk_threshold, hipStream_t st);

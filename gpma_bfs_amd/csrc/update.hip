@@ -287,6 +287,9 @@ __global__ __launch_bounds__(CMP_BLOCK) void k_compact(const gkey *__restrict__ 
 // tiles (read 64 at a time by one wave), publish the inclusive prefix and
 // scatter. Status words carry the caller's epoch in their high half, so no
 // clearing pass is needed between calls; tiles past *d_n exit at once.
+// With fix (the output of k_apply), an entry is kept when iw < WIN_SEP, and
+// an entry iw = WIN_SEP | leaf writes fix[i] to seps[leaf]: the merged
+// leaf's new first key, held back until every search of the batch ran.
 constexpr uint32_t CC_THREADS = 256;
 constexpr uint32_t CC_ROWS = 8;                        // 64-entry rows per wave
 constexpr uint32_t CC_WAVE_ITEMS = WAVE * CC_ROWS;     // 512
@@ -298,16 +301,24 @@ __global__ __launch_bounds__(CC_THREADS) void k_compact_chained(
     const uint8_t *__restrict__ idone, uint64_t n_ub, const uint32_t *__restrict__ d_n, gkey *__restrict__ ok,
     gval *__restrict__ ov, uint32_t *__restrict__ ow, uint8_t *__restrict__ odone, uint32_t mask,
     unsigned long long *__restrict__ status, uint32_t epoch, uint32_t *__restrict__ ticket,
-    uint32_t *__restrict__ d_out, uint32_t *__restrict__ err) {
+    uint32_t *__restrict__ d_out, uint32_t *__restrict__ err, const gkey *__restrict__ fix,
+    gkey *__restrict__ seps) {
     __shared__ uint32_t s_tile, s_excl;
     __shared__ uint32_t wtot[CC_THREADS / WAVE];
     const uint64_t n = d_n ? std::min<uint64_t>(*d_n, n_ub) : n_ub;
-    // a bounded grid takes tiles by ticket until the list is exhausted
-    for (;;) {
-    __syncthreads();  // s_tile, wtot and s_excl of the previous tile are consumed
+    // Workgroups beyond what the list needs leave before taking a ticket
+    // (tickets are one atomic on one word: a few hundred of them queue up).
+    // The rest take one ticket each, in the order they start, and the
+    // workgroup with ticket q owns tiles q, q + G, q + 2G, ... (G workgroups
+    // take part), each in turn: every tile's look-back waits only on lower
+    // tiles, owned by workgroups that started earlier or by itself.
+    const uint64_t need = std::max<uint64_t>(1, (n + CC_TILE - 1) / CC_TILE);
+    if (blockIdx.x >= need) return;
+    const uint32_t G = (uint32_t)std::min<uint64_t>(gridDim.x, need);
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
-    const uint32_t tile = s_tile;
+    for (uint32_t tile = s_tile;; tile += G) {
+    __syncthreads();  // wtot and s_excl of the previous tile are consumed
     const uint64_t base = (uint64_t)tile * CC_TILE;
     if (base >= n) {
         if (tile == 0 && threadIdx.x == 0) *d_out = 0;
@@ -321,7 +332,15 @@ __global__ __launch_bounds__(CC_THREADS) void k_compact_chained(
     for (uint32_t j = 0; j < CC_ROWS; j++) {
         const uint64_t i = wbase + (uint64_t)j * WAVE + lane;
         bool keep = false;
-        if (i < n) keep = idone ? (idone[i] == 0) : (iw[i] != SIZE_NONE);
+        if (i < n) {
+            if (idone) {
+                keep = idone[i] == 0;
+            } else {
+                const uint32_t w = iw[i];
+                keep = fix ? w < WIN_SEP : w != SIZE_NONE;
+                if (fix && w != SIZE_NONE && w >= WIN_SEP) seps[w - WIN_SEP] = fix[i];
+            }
+        }
         km[j] = ballot(keep);
         wcount += (uint32_t)__popcll(km[j]);
     }
@@ -452,6 +471,250 @@ __global__ __launch_bounds__(256) void k_leaf(const gkey *__restrict__ pk, const
         for (uint32_t j = 0; j < S; j++) values[(uint64_t)w + j] = ov[j];
     }
     seps[w / S] = ok[0];
+}
+
+// Locate and level 0 in one pass (S >= 16, row hints): k_locate's search,
+// then k_leaf's merge, by the record that heads each leaf's run.
+//
+// Every thread finds the leaf p of its record as k_locate does. With 16+
+// slots per segment the separators increase strictly, so the records of
+// leaf p are exactly the sorted records in [seps[p], seps[p + 1]); the
+// first of them (the previous record lies below seps[p]) is the run head
+// and owns the leaf: it loads the leaf once, applies the run's deletes and
+// modifications to it (handle_del_mod, gpma.cuh:188-199; the last of equal
+// keys applies), counts the run's pending inserts and, when level 0's
+// bounds allow (gpma.cuh:636-654 with f = 1: the merged items packed at the
+// front of the leaf), merges them into the leaf. A run may continue past
+// the workgroup's tile: the head reads those records from global memory,
+// and their own threads, which are not heads, leave them alone. Nothing in
+// this kernel writes seps, so every search sees the separators as they
+// were before the batch; a merge that changes its leaf's first key passes
+// it on in fix[head] with win[head] = WIN_SEP | p, for the compaction that
+// follows. A merge moves guards only inside their leaf, so the concurrent
+// row hints (slot / S) read the same segment before and after it.
+//
+// The head keeps the leaf's keys in registers, its values as packed bytes
+// and its live slots as a bit mask; the merge stores every key straight to
+// its slot (pending insert u, in key order, goes to u + the live keys below
+// it; the live keys fill the remaining slots in order) and gathers the
+// values in a per-thread LDS row, written back with 16-byte stores (single
+// byte stores to the leaf cost more than the whole merge).
+template <uint32_t S>
+__device__ __forceinline__ uint32_t byte_of(const uint32_t (&vw)[S / 4], uint32_t s) {
+    return (vw[s / 4] >> (8 * (s % 4))) & 0xFFu;
+}
+
+template <uint32_t S>
+__global__ __launch_bounds__(256) void k_apply(const gkey *__restrict__ ks, const gval *__restrict__ vs, uint64_t n,
+                                               const uint32_t *__restrict__ dn, gkey *__restrict__ keys,
+                                               gval *__restrict__ values, const gkey *__restrict__ seps,
+                                               uint64_t nseg, uint32_t *__restrict__ row_offset, uint32_t lower,
+                                               uint32_t upper, uint32_t *__restrict__ win, gkey *__restrict__ fix,
+                                               uint32_t *__restrict__ np0) {
+    static_assert(S == 16 || S == 32, "one leaf's live slots fit a 32-bit mask");
+    constexpr uint32_t T = 256;
+    constexpr uint32_t SMASK = S == 32 ? 0xFFFFFFFFu : (1u << S) - 1;
+    __shared__ gkey tk[T];
+    __shared__ gval tv[T];
+    __shared__ uint32_t wsum[T / WAVE];
+    __shared__ uint32_t rowv[T * S / 4];  // each head's merged values, packed
+    if (dn) n = *dn;
+    const uint64_t base = (uint64_t)blockIdx.x * T;
+    if (base >= n) return;
+    const uint32_t tid = threadIdx.x;
+    const uint64_t i = base + tid;
+    const uint64_t tend = std::min<uint64_t>(n, base + T);
+    gkey k = KEY_NONE, kprev = 0;
+    gval v = 0;
+    if (i < n) {
+        k = ks[i];
+        v = vs[i];
+        if (i > 0) kprev = ks[i - 1];
+    }
+    tk[tid] = k;
+    tv[tid] = v;
+    bool head = false;
+    uint64_t p = 0;
+    gkey nsep = KEY_NONE;  // first key of leaf p + 1
+    if (k != KEY_NONE) {
+        const uint32_t src = (uint32_t)(k >> 32);
+        uint64_t lo = row_offset[src] / S, hi = row_offset[src + 1] / S;
+        if (hi >= nseg) hi = nseg - 1;
+        if (lo > hi) lo = hi;
+        while (hi - lo > 4) {
+            const uint64_t mid = (lo + hi + 1) >> 1;
+            if (seps[mid] <= k) lo = mid;
+            else hi = mid - 1;
+        }
+        // the last separator <= k in [lo, hi], with its own and the next
+        // one, from one round of loads
+        gkey sv[6];
+#pragma unroll
+        for (uint32_t t = 0; t < 6; t++) sv[t] = (lo + t <= hi + 1 && lo + t < nseg) ? seps[lo + t] : KEY_NONE;
+        p = lo;
+        gkey sp = sv[0];
+        nsep = sv[1];
+#pragma unroll
+        for (uint32_t t = 1; t < 5; t++) {
+            if (lo + t <= hi && sv[t] <= k) {
+                p = lo + t;
+                sp = sv[t];
+                nsep = sv[t + 1];
+            }
+        }
+        // the bitwise search never compares seps[0]: every key below seps[1] is in leaf 0
+        head = i == 0 || (p > 0 && kprev < sp);
+    } else if (i < n) {
+        win[i] = SIZE_NONE;
+    }
+    __syncthreads();  // tk / tv staged
+    uint32_t r = 0;
+    if (head) {
+        const uint64_t w = p * S;
+        gkey lk[S];
+        uint32_t vw[S / 4];
+        load_leaf_keys<S>(keys, w, lk);
+        {
+            const uint4 *q = reinterpret_cast<const uint4 *>(values + w);
+#pragma unroll
+            for (uint32_t j = 0; j < S / 16; j++) {
+                const uint4 t = q[j];
+                vw[4 * j] = t.x;
+                vw[4 * j + 1] = t.y;
+                vw[4 * j + 2] = t.z;
+                vw[4 * j + 3] = t.w;
+            }
+        }
+        // record j + 1 of the run (KEY_NONE past the batch)
+        auto next = [&](uint64_t j, gkey &kn, gval &vn) {
+            kn = KEY_NONE;
+            vn = 0;
+            if (j + 1 < tend) {
+                kn = tk[j + 1 - base];
+                vn = tv[j + 1 - base];
+            } else if (j + 1 < n) {
+                kn = ks[j + 1];
+                vn = vs[j + 1];
+            }
+        };
+        auto hits = [&](gkey x) {
+            uint32_t h = 0;
+#pragma unroll
+            for (uint32_t s = 0; s < S; s++) h |= (lk[s] == x ? 1u : 0u) << s;
+            return h;
+        };
+        // pass 1: deletes and modifications in place, count the pending inserts
+        bool dirty = false;
+        uint64_t jend = i;
+        {
+            gkey kj = k, kn;
+            gval vj = v, vn;
+            for (;;) {
+                next(jend, kn, vn);
+                if (kn != kj) {
+                    const uint32_t h = hits(kj);
+                    if (h) {
+#pragma unroll
+                        for (uint32_t s = 0; s < S; s++) {
+                            if ((h >> s) & 1u) {
+                                const uint32_t sh = 8 * (s % 4);
+                                vw[s / 4] = (vw[s / 4] & ~(0xFFu << sh)) | ((uint32_t)vj << sh);
+                            }
+                        }
+                        dirty = true;
+                    } else if (vj != 0) {
+                        r++;
+                    }
+                }
+                jend++;
+                if (kn >= nsep || kn == KEY_NONE) break;
+                kj = kn;
+                vj = vn;
+            }
+        }
+        uint32_t livem = 0;
+#pragma unroll
+        for (uint32_t s = 0; s < S; s++) livem |= (lk[s] != KEY_NONE && byte_of<S>(vw, s) != 0 ? 1u : 0u) << s;
+        const uint32_t m = (uint32_t)__popc(livem) + r;
+        if (r > 0 && r <= upper && m >= lower && m <= upper) {
+            // pass 2: pending inserts to their slots
+            gkey kj = k, kn;
+            gval vj = v, vn;
+            uint32_t u = 0, occ = 0;
+            gkey first = KEY_NONE;
+            uint32_t *ov = rowv + tid * (S / 4);
+            gval *ovb = reinterpret_cast<gval *>(ov);
+#pragma unroll
+            for (uint32_t q = 0; q < S / 4; q++) ov[q] = 0;
+            for (uint64_t j = i; j < jend; j++) {
+                next(j, kn, vn);
+                if (kn != kj && vj != 0 && hits(kj) == 0) {
+                    uint32_t below = 0;
+#pragma unroll
+                    for (uint32_t s = 0; s < S; s++) below |= (lk[s] < kj ? 1u : 0u) << s;
+                    const uint32_t pos = u + (uint32_t)__popc(livem & below);
+                    keys[w + pos] = kj;
+                    ovb[pos] = vj;
+                    if (is_guard(kj)) row_offset[(kj >> 32) + 1] = (uint32_t)(w + pos);
+                    if (pos == 0) first = kj;
+                    occ |= 1u << pos;
+                    u++;
+                }
+                if (j != i) win[j] = SIZE_NONE;
+                kj = kn;
+                vj = vn;
+            }
+            // live keys into the free slots, in order; then the empty tail
+            uint32_t fr = ~occ & SMASK;
+#pragma unroll
+            for (uint32_t s = 0; s < S; s++) {
+                if ((livem >> s) & 1u) {
+                    const uint32_t pos = (uint32_t)__builtin_ctz(fr);
+                    fr &= fr - 1;
+                    keys[w + pos] = lk[s];
+                    ovb[pos] = (gval)byte_of<S>(vw, s);
+                    if (is_guard(lk[s])) row_offset[(lk[s] >> 32) + 1] = (uint32_t)(w + pos);
+                    if (pos == 0) first = lk[s];
+                }
+            }
+            for (uint32_t pos = m; pos < S; pos++) keys[w + pos] = KEY_NONE;
+            // the row is private to this thread: no barrier needed before reading it back
+            uint4 *qv = reinterpret_cast<uint4 *>(values + w);
+#pragma unroll
+            for (uint32_t j = 0; j < S / 16; j++)
+                qv[j] = make_uint4(ov[4 * j], ov[4 * j + 1], ov[4 * j + 2], ov[4 * j + 3]);
+            fix[i] = first;
+            win[i] = WIN_SEP | (uint32_t)p;
+        } else {
+            if (dirty) {
+                uint4 *q = reinterpret_cast<uint4 *>(values + w);
+#pragma unroll
+                for (uint32_t j = 0; j < S / 16; j++)
+                    q[j] = make_uint4(vw[4 * j], vw[4 * j + 1], vw[4 * j + 2], vw[4 * j + 3]);
+            }
+            // the run's pending inserts wait for level 1
+            gkey kj = k, kn;
+            gval vj = v, vn;
+            for (uint64_t j = i; j < jend; j++) {
+                next(j, kn, vn);
+                const bool pend = kn != kj && vj != 0 && hits(kj) == 0;
+                win[j] = pend ? (uint32_t)w : SIZE_NONE;
+                kj = kn;
+                vj = vn;
+            }
+        }
+    }
+    // level-0 pending inserts of the tile's runs
+#pragma unroll
+    for (uint32_t off = 32; off > 0; off >>= 1) r += __shfl_xor(r, off, WAVE);
+    if (lane_id() == 0) wsum[tid / WAVE] = r;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t t = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < T / WAVE; q++) t += wsum[q];
+        if (t) atomicAdd(np0, t);
+    }
 }
 
 // number of entries of a sorted array a[0..n) strictly below x
@@ -1129,10 +1392,10 @@ uint64_t compact_status_words(uint64_t n_ub) { return (n_ub + CC_TILE - 1) / CC_
 void launch_compact_chained(const gkey *ik, const gval *iv, const uint32_t *iw, const uint8_t *idone, uint64_t n_ub,
                             const uint32_t *d_n, gkey *ok, gval *ov, uint32_t *ow, uint8_t *odone, uint32_t mask,
                             unsigned long long *status, uint32_t epoch, uint32_t *ticket, uint32_t *d_out,
-                            uint32_t *err, hipStream_t st) {
+                            uint32_t *err, hipStream_t st, const gkey *fix, gkey *seps) {
     const uint64_t nt = std::max<uint64_t>(1, std::min<uint64_t>((n_ub + CC_TILE - 1) / CC_TILE, CC_GRID_MAX));
     hipLaunchKernelGGL(k_compact_chained, dim3((uint32_t)nt), dim3(CC_THREADS), 0, st, ik, iv, iw, idone, n_ub, d_n,
-                       ok, ov, ow, odone, mask, status, epoch, ticket, d_out, err);
+                       ok, ov, ow, odone, mask, status, epoch, ticket, d_out, err, fix, seps);
 }
 
 void launch_count_pending(const uint8_t *done, uint64_t n_ub, const uint32_t *d_n, uint32_t *blkcnt,
@@ -1141,6 +1404,19 @@ void launch_count_pending(const uint8_t *done, uint64_t n_ub, const uint32_t *d_
     uint64_t nb = (n_ub + CMP_BLOCK - 1) / CMP_BLOCK;
     hipLaunchKernelGGL(k_count_pending, dim3((uint32_t)std::min<uint64_t>(nb, 2048)), dim3(CMP_BLOCK), 0, st, done,
                        d_n, blkcnt);
+}
+
+void launch_apply(uint32_t S, const gkey *ks, const gval *vs, uint64_t n_ub, const uint32_t *d_n, gkey *keys,
+                  gval *values, const gkey *seps, uint64_t nseg, uint32_t *row_offset, uint32_t lower, uint32_t upper,
+                  uint32_t *win, gkey *fix, uint32_t *np0, hipStream_t st) {
+    if (!n_ub) return;
+    dim3 g((uint32_t)((n_ub + 255) / 256)), b(256);
+    if (S == 16)
+        hipLaunchKernelGGL(k_apply<16>, g, b, 0, st, ks, vs, n_ub, d_n, keys, values, seps, nseg, row_offset, lower,
+                           upper, win, fix, np0);
+    else
+        hipLaunchKernelGGL(k_apply<32>, g, b, 0, st, ks, vs, n_ub, d_n, keys, values, seps, nseg, row_offset, lower,
+                           upper, win, fix, np0);
 }
 
 void launch_leaf(uint32_t S, const gkey *pk, const gval *pv, const uint32_t *pw, uint8_t *pd, uint64_t n_ub,

@@ -252,6 +252,8 @@ void *gpma_stream(gpma_t g);
 /* Per-phase timing with HIP events on the handle's stream (off by default).
  * Phases: 0 stage, 1 sort, 2 locate, 3 compact, 4 leaf rebalance,
  * 5 upper-level rebalance, 6 rebuild (resize / bulk load), 7 BFS expand.
+ * With 16 or 32 slots per segment (after the guard load) the locate and
+ * the leaf rebalance are one kernel, counted under 2; 4 then stays empty.
  * gpma_get_profile fills ms[i] (cumulative milliseconds) and calls[i]
  * (kernel launches timed) for i < n; gpma_profile(g, 0) clears nothing,
  * gpma_profile(g, 2) resets the counters and enables. */

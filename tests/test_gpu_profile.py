@@ -24,8 +24,13 @@ def test_phase_timing_reports_and_keeps_results():
         G.update_gpma(a, k, v)
         G.update_gpma(b, k, v)
     rep = a.profile_report()
-    for phase in ("sort", "locate", "compact", "leaf"):
+    # with 16 or 32 slots per segment the locate and the leaf rebalance run
+    # as one kernel, timed as the locate phase
+    fused = a.stats()["segment_length"] in (16, 32)
+    for phase in ("sort", "locate", "compact") + (() if fused else ("leaf",)):
         assert rep[phase]["ms"] > 0 and rep[phase]["launches"] >= 5, (phase, rep[phase])
+    if fused:
+        assert rep["leaf"]["launches"] == 0, rep["leaf"]
     # only the sort phase from here on
     a.profile_phases(["sort"])
     before = a.profile_report()
