@@ -810,41 +810,58 @@ __device__ void window_one(uint32_t S, uint32_t W, const gkey *__restrict__ pk, 
 
 // window_one for W <= WAVE_WIN_MAX by one wave, without workgroup barriers:
 // the run length and the live-item compaction by ballots, lk/lv the wave's
-// own LDS rows. Same results as window_one.
+// own LDS rows. Same results as window_one. The caller has loaded, in one
+// round of loads, the window ids of entries i - 1 + lane (wv) and the first
+// 64 pending entries of the run (lpk / lpv, the wave's LDS rows, valid when
+// the run is at most 64 long); the window's slots come in one more round.
 constexpr uint32_t WAVE_WIN_MAX = 256;
 __device__ void window_wave(uint32_t S, uint32_t W, const gkey *__restrict__ pk, const gval *__restrict__ pv,
                             const uint32_t *__restrict__ pw, uint8_t *__restrict__ pd, uint64_t np, uint64_t i,
-                            uint32_t w, gkey *__restrict__ keys, gval *__restrict__ values, gkey *__restrict__ seps,
-                            uint32_t *__restrict__ row_offset, uint32_t lower, uint32_t upper, gkey *lk, gval *lv) {
+                            uint32_t w, uint32_t wv, gkey *__restrict__ keys, gval *__restrict__ values,
+                            gkey *__restrict__ seps, uint32_t *__restrict__ row_offset, uint32_t lower,
+                            uint32_t upper, gkey *lk, gval *lv, const gkey *lpk, const gval *lpv) {
     const uint32_t lane = lane_id();
     // run length: the first t <= upper whose entry i + t is not in window w
+    // (lane 1 + t holds entry i + t's window for t < 63)
     uint32_t r = upper + 1;
-    for (uint32_t t0 = 0; t0 <= upper; t0 += WAVE) {
-        const uint32_t t = t0 + lane;
-        const bool out = t <= upper && !((i + t < np) && pw[i + t] == w);
-        const uint64_t b = ballot(out);
-        if (b) {
-            r = t0 + (uint32_t)__ffsll((unsigned long long)b) - 1;
-            break;
+    const uint64_t out0 = ballot(lane >= 1 && wv != w);
+    if (out0) {
+        r = (uint32_t)__ffsll((unsigned long long)out0) - 2;
+    } else {
+        for (uint32_t t0 = WAVE - 1; t0 <= upper; t0 += WAVE) {
+            const uint32_t t = t0 + lane;
+            const bool out = t <= upper && !((i + t < np) && pw[i + t] == w);
+            const uint64_t b = ballot(out);
+            if (b) {
+                r = t0 + (uint32_t)__ffsll((unsigned long long)b) - 1;
+                break;
+            }
         }
     }
     if (r > upper) return;
-    uint32_t c = 0;
-    for (uint32_t b0 = 0; b0 < W; b0 += WAVE) {
-        const uint32_t p = b0 + lane;
-        gkey k = KEY_NONE;
-        gval v = 0;
-        bool live = false;
+    const bool lds_run = r <= WAVE;
+    // the window's slots, every load issued before the first ballot
+    gkey kq[WAVE_WIN_MAX / WAVE];
+    gval vq[WAVE_WIN_MAX / WAVE];
+#pragma unroll
+    for (uint32_t q = 0; q < WAVE_WIN_MAX / WAVE; q++) {
+        const uint32_t p = q * WAVE + lane;
+        kq[q] = KEY_NONE;
+        vq[q] = 0;
         if (p < W) {
-            k = keys[(uint64_t)w + p];
-            v = values[(uint64_t)w + p];
-            live = is_live(k, v);
+            kq[q] = keys[(uint64_t)w + p];
+            vq[q] = values[(uint64_t)w + p];
         }
+    }
+    uint32_t c = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < WAVE_WIN_MAX / WAVE; q++) {
+        const bool live = is_live(kq[q], vq[q]);
         const uint64_t m = ballot(live);
         if (live) {
             const uint32_t pos = c + (uint32_t)__popcll(m & lanemask_lt());
-            lk[pos] = k;
-            lv[pos] = v;
+            lk[pos] = kq[q];
+            lv[pos] = vq[q];
         }
         c += (uint32_t)__popcll(m);
     }
@@ -854,15 +871,26 @@ __device__ void window_wave(uint32_t S, uint32_t W, const gkey *__restrict__ pk,
         const uint64_t f = W / S;
         for (uint32_t t = lane; t < c; t += WAVE) {
             const gkey k = lk[t];
-            const uint64_t mi = t + lower_bound_g(pk + i, r, k);
-            const uint32_t slot = redistribute_slot(f, S, m, mi);
+            uint64_t below;
+            if (lds_run) {
+                uint32_t lo = 0, hi = r;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (lpk[mid] < k) lo = mid + 1;
+                    else hi = mid;
+                }
+                below = lo;
+            } else {
+                below = lower_bound_g(pk + i, r, k);
+            }
+            const uint32_t slot = redistribute_slot(f, S, m, t + below);
             keys[(uint64_t)w + slot] = k;
             values[(uint64_t)w + slot] = lv[t];
             if (is_guard(k)) row_offset[(k >> 32) + 1] = w + slot;
             if (slot % S == 0) seps[(w + slot) / S] = k;
         }
         for (uint32_t u = lane; u < r; u += WAVE) {
-            const gkey k = pk[i + u];
+            const gkey k = lds_run ? lpk[u] : pk[i + u];
             uint32_t lo = 0, hi = c;
             while (lo < hi) {
                 uint32_t mid = (lo + hi) >> 1;
@@ -872,7 +900,7 @@ __device__ void window_wave(uint32_t S, uint32_t W, const gkey *__restrict__ pk,
             const uint64_t mi = u + lo;
             const uint32_t slot = redistribute_slot(f, S, m, mi);
             keys[(uint64_t)w + slot] = k;
-            values[(uint64_t)w + slot] = pv[i + u];
+            values[(uint64_t)w + slot] = lds_run ? lpv[u] : pv[i + u];
             if (is_guard(k)) row_offset[(k >> 32) + 1] = w + slot;
             if (slot % S == 0) seps[(w + slot) / S] = k;
             pd[i + u] = 1;
@@ -888,7 +916,6 @@ __device__ void window_wave(uint32_t S, uint32_t W, const gkey *__restrict__ pk,
             }
         }
     }
-    wave_sync();  // the wave's next window reuses lk / lv
 }
 
 // Workgroups stride over the pending list; each run head is one window.
@@ -905,14 +932,30 @@ __global__ __launch_bounds__(256) void k_window_wave(uint32_t S, uint32_t W, con
     constexpr uint32_t NW = 256 / WAVE;
     __shared__ gkey lk[NW][WAVE_WIN_MAX];
     __shared__ gval lv[NW][WAVE_WIN_MAX];
+    __shared__ gkey lpk[NW][WAVE];
+    __shared__ gval lpv[NW][WAVE];
     if (d_gate && *d_gate >= gate) return;
     const uint64_t np = *d_np;
-    const uint32_t wid = threadIdx.x / WAVE;
+    const uint32_t wid = threadIdx.x / WAVE, lane = lane_id();
     const uint64_t nw = (uint64_t)gridDim.x * NW;
     for (uint64_t i = (uint64_t)blockIdx.x * NW + wid; i < np; i += nw) {
-        const uint32_t w = pw[i];
-        if (i > 0 && pw[i - 1] == w) continue;
-        window_wave(S, W, pk, pv, pw, pd, np, i, w, keys, values, seps, row_offset, lower, upper, lk[wid], lv[wid]);
+        // one round of loads: the window ids of entries i - 1 .. i + 62 and
+        // the pending entries i .. i + 63
+        uint32_t wv = SIZE_NONE;
+        if (lane > 0 ? i - 1 + lane < np : i > 0) wv = pw[i - 1 + lane];
+        gkey k = KEY_NONE;
+        gval v = 0;
+        if (i + lane < np) {
+            k = pk[i + lane];
+            v = pv[i + lane];
+        }
+        const uint32_t w = __shfl(wv, 1, WAVE);
+        if (i > 0 && __shfl(wv, 0, WAVE) == w) continue;  // not a run head
+        wave_sync();  // the wave's previous window is done with its LDS rows
+        lpk[wid][lane] = k;
+        lpv[wid][lane] = v;
+        window_wave(S, W, pk, pv, pw, pd, np, i, w, wv, keys, values, seps, row_offset, lower, upper, lk[wid],
+                    lv[wid], lpk[wid], lpv[wid]);
     }
 }
 __global__ __launch_bounds__(256) void k_window(uint32_t S, uint32_t W, const gkey *__restrict__ pk,
