@@ -32,7 +32,7 @@ namespace {
 
 constexpr uint32_t MT = 256;                 // threads per workgroup
 constexpr uint32_t MW = MT / WAVE;           // waves per workgroup
-constexpr uint32_t MIPT = 16;                // items per lane
+constexpr uint32_t MIPT = 4;                 // items per lane (the local pass's LSD passes over a large bucket)
 constexpr uint32_t MWI = WAVE * MIPT;        // items per wave
 constexpr uint32_t MTILE = MT * MIPT;        // items per tile
 constexpr uint32_t MAX_D = 12;               // bucket bits
