@@ -692,11 +692,38 @@ uint32_t enqueue_update(gpma_handle *g, const gkey *src_k, const gval *src_v, ui
         phase_end(g, P_LEAF, 1);
         compact_pending(0, 0, ctr + C_LVL, ctr + C_LVL + 1);
     }
+    // Levels 1..spec on the device. Windows of one wave (W <= LEVEL_WIN_MAX)
+    // work straight from the level-1 list: its done flags mark the placed
+    // entries, and each level counts the inserts entering the next one.
     uint32_t spec = 0;
+    while (spec < dev_levels && spec + 1 <= g->h && ((uint64_t)g->S << (spec + 1)) <= LEVEL_WIN_MAX &&
+           spec + 2 < C_LVL_SLOTS) {
+        const uint32_t L = spec + 1;
+        phase_begin(g, P_UPPER);
+        launch_window_level(g->S, g->S << L, g->pk1.as<gkey>(), g->pv1.as<gval>(), g->pw1.as<uint32_t>(),
+                            g->pd1.as<uint8_t>(), n_ub, ctr + C_LVL + 1, ctr + C_LVL + L, ctr + C_LVL + L + 1,
+                            g->keys.as<gkey>(), g->values.as<gval>(), g->seps.as<gkey>(),
+                            g->row_offset.as<uint32_t>(), g->lower[L], g->upper[L], ctr + C_LVL, g->bulk_threshold,
+                            st);
+        phase_end(g, P_UPPER, 1);
+        spec++;
+    }
+    // Wider windows (one workgroup, W <= WIN_MAX) take compacted lists: the
+    // entries not placed yet, with their windows at the level, into list 0,
+    // then a compaction after each level alternating lists 0 and 1 (level L's
+    // list is list (L - waves - 1) & 1).
+    const uint32_t waves = spec;
+    int pc = 0;
     while (spec < dev_levels && spec + 1 <= g->h && ((uint64_t)g->S << (spec + 1)) <= WIN_MAX &&
            spec + 2 < C_LVL_SLOTS) {
         const uint32_t L = spec + 1;
-        const int pc = L & 1;
+        if (L == waves + 1) {
+            phase_begin(g, P_COMPACT);
+            compact_step(g, g->pk1.as<gkey>(), g->pv1.as<gval>(), g->pw1.as<uint32_t>(), g->pd1.as<uint8_t>(), n_ub,
+                         ctr + C_LVL + 1, g->pk0.as<gkey>(), g->pv0.as<gval>(), g->pw0.as<uint32_t>(),
+                         g->pd0.as<uint8_t>(), ~(uint32_t)(((uint64_t)g->S << L) - 1), ctr + C_LVL + L);
+            phase_end(g, P_COMPACT, 1);
+        }
         phase_begin(g, P_UPPER);
         launch_window(g->S, g->S << L, PK[pc]->as<gkey>(), PV[pc]->as<gval>(), PW[pc]->as<uint32_t>(),
                       PD[pc]->as<uint8_t>(), n_ub, ctr + C_LVL + L, g->keys.as<gkey>(), g->values.as<gval>(),
@@ -704,6 +731,7 @@ uint32_t enqueue_update(gpma_handle *g, const gkey *src_k, const gval *src_v, ui
                       g->bulk_threshold, st);
         phase_end(g, P_UPPER, 1);
         compact_pending(pc, L, ctr + C_LVL + L, ctr + C_LVL + L + 1);
+        pc ^= 1;
         spec++;
     }
     return spec;
@@ -724,15 +752,12 @@ int finish_update(gpma_handle *g, uint64_t n, uint32_t spec, bool used_msd) {
     g->applied += n - inval;
     const uint64_t np0 = g->hctr[C_LVL];
     if (np0 >= g->bulk_threshold) {
-        // The level kernels stood down; their compactions carried every
-        // pending insert up to the list after the last device level. Any
-        // leaf merge of the fused level-0 pass is kept: the rebuild lays
-        // out the live set (array items plus this list) from scratch,
-        // whichever of them a leaf already holds.
+        // The level kernels stood down: the level-1 list holds every pending
+        // insert the level-0 pass did not place. Any leaf merge of the fused
+        // level-0 pass is kept: the rebuild lays out the live set (array
+        // items plus this list) from scratch, whichever of them a leaf holds.
         g->last_levels = 0;
-        const int lc = (int)((spec + 1) & 1);
-        Buf *LK[2] = {&g->pk0, &g->pk1}, *LV[2] = {&g->pv0, &g->pv1};
-        TRY(rebuild(g, LK[lc]->as<gkey>(), LV[lc]->as<gval>(), g->hctr[C_LVL + spec + 1], false));
+        TRY(rebuild(g, g->pk1.as<gkey>(), g->pv1.as<gval>(), g->hctr[C_LVL + 1], false));
         TRY(stream_sync(g));
         HIP_TRY(hipGetLastError());
         return inval ? fail(GPMA_ERR_KEY_RANGE, std::to_string(inval) + " keys with src/dst >= row_num skipped")
@@ -745,9 +770,22 @@ int finish_update(gpma_handle *g, uint64_t n, uint32_t spec, bool used_msd) {
         *PD[2] = {&g->pd0, &g->pd1};
     // host-driven levels above the prefix (rare: wide windows, growth)
     uint32_t level = spec + 1;
-    int pc = level & 1;
     uint64_t np = g->hctr[C_LVL + level];
     const uint32_t npslot[2] = {C_NP0, C_NP1};
+    // the device prefix's one-wave levels (enqueue_update): after wider ones
+    // the next level's entries are a compacted list already
+    uint32_t waves = 0;
+    while (waves < spec && ((uint64_t)g->S << (waves + 1)) <= LEVEL_WIN_MAX) waves++;
+    int pc = spec > waves ? (int)((spec - waves) & 1) : 0;
+    if (np > 0 && spec == waves) {
+        // the level-1 list's entries not placed yet, with their windows at
+        // `level`, into list 0
+        fill_u32(ctr + C_NP1, (uint32_t)g->hctr[C_LVL + 1], 1, st);
+        compact_step(g, g->pk1.as<gkey>(), g->pv1.as<gval>(), g->pw1.as<uint32_t>(), g->pd1.as<uint8_t>(), n,
+                     ctr + C_NP1, g->pk0.as<gkey>(), g->pv0.as<gval>(), g->pw0.as<uint32_t>(), g->pd0.as<uint8_t>(),
+                     ~(uint32_t)(((uint64_t)g->S << level) - 1), ctr + C_NP0);
+        g->ctr_clean = false;
+    }
     while (np > 0 && level <= g->h) {
         const uint64_t W = (uint64_t)g->S << level;
         // the list's count into its device slot (reading the counters cleared them)

@@ -132,6 +132,17 @@ void launch_apply(uint32_t S, const gkey *ks, const gval *vs, uint64_t n_ub, con
 void launch_leaf(uint32_t S, const gkey *pk, const gval *pv, const uint32_t *pw, uint8_t *pd, uint64_t n_ub,
                  const uint32_t *d_np, gkey *keys, gval *values, gkey *seps, uint32_t *row_offset, uint32_t lower,
                  uint32_t upper, uint32_t bulk_threshold, hipStream_t st);
+// Device-driven levels: windows of at most this many slots, one wave each.
+constexpr uint32_t LEVEL_WIN_MAX = 256;
+// One device-driven level (W <= LEVEL_WIN_MAX) over the level-1 list itself
+// (*d_n1 entries, level-1 windows in pw, done flags in pd): the first entry
+// of each window gathers the window's entries not placed yet and merges
+// them (done flags set) or adds their count to *d_next. Nothing runs when
+// *d_in == 0 or d_gate is set and *d_gate >= gate.
+void launch_window_level(uint32_t S, uint32_t W, const gkey *pk, const gval *pv, const uint32_t *pw, uint8_t *pd,
+                         uint64_t n_ub, const uint32_t *d_n1, const uint32_t *d_in, uint32_t *d_next, gkey *keys,
+                         gval *values, gkey *seps, uint32_t *row_offset, uint32_t lower, uint32_t upper,
+                         const uint32_t *d_gate, uint32_t gate, hipStream_t st);
 // Levels whose window fits in LDS: workgroups stride over the first *d_np
 // pending entries, one window per run head. Does nothing when d_gate is set
 // and *d_gate >= gate (that batch takes the bulk path).

@@ -815,6 +815,7 @@ __device__ void window_one(uint32_t S, uint32_t W, const gkey *__restrict__ pk, 
 // 64 pending entries of the run (lpk / lpv, the wave's LDS rows, valid when
 // the run is at most 64 long); the window's slots come in one more round.
 constexpr uint32_t WAVE_WIN_MAX = 256;
+static_assert(WAVE_WIN_MAX == LEVEL_WIN_MAX, "device-driven levels use the wave windows");
 __device__ void window_wave(uint32_t S, uint32_t W, const gkey *__restrict__ pk, const gval *__restrict__ pv,
                             const uint32_t *__restrict__ pw, uint8_t *__restrict__ pd, uint64_t np, uint64_t i,
                             uint32_t w, uint32_t wv, gkey *__restrict__ keys, gval *__restrict__ values,
@@ -958,6 +959,151 @@ __global__ __launch_bounds__(256) void k_window_wave(uint32_t S, uint32_t W, con
                     lv[wid], lpk[wid], lpv[wid]);
     }
 }
+// One window of W <= WAVE_WIN_MAX slots at base w by one wave, given its r
+// pending entries (sorted, r <= upper) in the wave's LDS rows lpk / lpv.
+// Same redistribution as window_wave; returns whether the level's bounds
+// let the window take them.
+__device__ bool window_gathered(uint32_t S, uint32_t W, uint32_t w, uint32_t r, const gkey *lpk, const gval *lpv,
+                                gkey *__restrict__ keys, gval *__restrict__ values, gkey *__restrict__ seps,
+                                uint32_t *__restrict__ row_offset, uint32_t lower, uint32_t upper, gkey *lk,
+                                gval *lv) {
+    const uint32_t lane = lane_id();
+    gkey kq[WAVE_WIN_MAX / WAVE];
+    gval vq[WAVE_WIN_MAX / WAVE];
+#pragma unroll
+    for (uint32_t q = 0; q < WAVE_WIN_MAX / WAVE; q++) {
+        const uint32_t p = q * WAVE + lane;
+        kq[q] = KEY_NONE;
+        vq[q] = 0;
+        if (p < W) {
+            kq[q] = keys[(uint64_t)w + p];
+            vq[q] = values[(uint64_t)w + p];
+        }
+    }
+    uint32_t c = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < WAVE_WIN_MAX / WAVE; q++) {
+        const bool live = is_live(kq[q], vq[q]);
+        const uint64_t m = ballot(live);
+        if (live) {
+            const uint32_t pos = c + (uint32_t)__popcll(m & lanemask_lt());
+            lk[pos] = kq[q];
+            lv[pos] = vq[q];
+        }
+        c += (uint32_t)__popcll(m);
+    }
+    wave_sync();
+    const uint32_t m = c + r;
+    if (m < lower || m > upper) return false;
+    const uint64_t f = W / S;
+    for (uint32_t t = lane; t < c; t += WAVE) {
+        const gkey k = lk[t];
+        uint32_t lo = 0, hi = r;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (lpk[mid] < k) lo = mid + 1;
+            else hi = mid;
+        }
+        const uint32_t slot = redistribute_slot(f, S, m, t + lo);
+        keys[(uint64_t)w + slot] = k;
+        values[(uint64_t)w + slot] = lv[t];
+        if (is_guard(k)) row_offset[(k >> 32) + 1] = w + slot;
+        if (slot % S == 0) seps[(w + slot) / S] = k;
+    }
+    for (uint32_t u = lane; u < r; u += WAVE) {
+        const gkey k = lpk[u];
+        uint32_t lo = 0, hi = c;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (lk[mid] < k) lo = mid + 1;
+            else hi = mid;
+        }
+        const uint32_t slot = redistribute_slot(f, S, m, u + lo);
+        keys[(uint64_t)w + slot] = k;
+        values[(uint64_t)w + slot] = lpv[u];
+        if (is_guard(k)) row_offset[(k >> 32) + 1] = w + slot;
+        if (slot % S == 0) seps[(w + slot) / S] = k;
+    }
+    for (uint32_t p = lane; p < W; p += WAVE) {
+        const uint64_t s = p / S, l = p % S;
+        const uint64_t a = seg_first_item(f, m, s);
+        const uint64_t cnt = seg_first_item(f, m, s + 1) - a;
+        if (l >= cnt) {
+            keys[(uint64_t)w + p] = KEY_NONE;
+            values[(uint64_t)w + p] = 0;
+            if (l == 0) seps[(w + p) / S] = KEY_NONE;
+        }
+    }
+    return true;
+}
+
+// A device-driven level (W = S << level <= WAVE_WIN_MAX) straight from the
+// level-1 list: no compaction between levels. The list's entries keep their
+// level-1 window (pw) and done flag (pd) throughout; the level-`level`
+// window of an entry is pw & ~(W - 1), and the entries of one window are
+// consecutive (the list is sorted). One wave per entry: the first entry of
+// each window gathers the window's entries not placed yet (those whose
+// window one level down failed, gpma.cuh:1012-1029), merges them when the
+// level's bounds allow and sets their done flags, else adds their count to
+// *d_next (the inserts entering the next level). Nothing runs when no
+// insert enters this level (*d_in == 0) or the batch takes the bulk path.
+__global__ __launch_bounds__(256) void k_window_level(uint32_t S, uint32_t W, const gkey *__restrict__ pk,
+                                                      const gval *__restrict__ pv, const uint32_t *__restrict__ pw,
+                                                      uint8_t *__restrict__ pd, const uint32_t *__restrict__ d_n1,
+                                                      const uint32_t *__restrict__ d_in, uint32_t *__restrict__ d_next,
+                                                      gkey *__restrict__ keys, gval *__restrict__ values,
+                                                      gkey *__restrict__ seps, uint32_t *__restrict__ row_offset,
+                                                      uint32_t lower, uint32_t upper,
+                                                      const uint32_t *__restrict__ d_gate, uint32_t gate) {
+    constexpr uint32_t NW = 256 / WAVE;
+    __shared__ gkey lk[NW][WAVE_WIN_MAX];
+    __shared__ gval lv[NW][WAVE_WIN_MAX];
+    __shared__ gkey lpk[NW][WAVE_WIN_MAX];
+    __shared__ gval lpv[NW][WAVE_WIN_MAX];
+    __shared__ uint32_t lix[NW][WAVE_WIN_MAX];
+    if (d_gate && *d_gate >= gate) return;
+    if (*d_in == 0) return;
+    const uint64_t n1 = *d_n1;
+    const uint32_t mask = ~(W - 1);
+    const uint32_t wid = threadIdx.x / WAVE, lane = lane_id();
+    const uint64_t nw = (uint64_t)gridDim.x * NW;
+    for (uint64_t i = (uint64_t)blockIdx.x * NW + wid; i < n1; i += nw) {
+        uint32_t wv = SIZE_NONE;  // lane l: the window of entry i - 1 + l
+        if (lane > 0 ? i - 1 + lane < n1 : i > 0) wv = pw[i - 1 + lane] & mask;
+        const uint32_t w = __shfl(wv, 1, WAVE);
+        if (i > 0 && __shfl(wv, 0, WAVE) == w) continue;  // not the window's first entry
+        wave_sync();  // the wave's previous window is done with its LDS rows
+        // the window's entries not placed yet, in order, 64 entries per round
+        uint32_t r = 0;
+        for (uint64_t j0 = i;; j0 += WAVE) {
+            const uint64_t j = j0 + lane;
+            const bool in = j < n1 && (pw[j] & mask) == w;
+            const bool todo = in && pd[j] == 0;
+            const uint64_t b = ballot(todo);
+            if (todo) {
+                const uint32_t pos = r + (uint32_t)__popcll(b & lanemask_lt());
+                if (pos < WAVE_WIN_MAX) {
+                    lpk[wid][pos] = pk[j];
+                    lpv[wid][pos] = pv[j];
+                    lix[wid][pos] = (uint32_t)j;
+                }
+            }
+            r += (uint32_t)__popcll(b);
+            if (ballot(!in)) break;
+        }
+        if (r == 0) continue;  // every entry of the window was placed below
+        wave_sync();
+        // more than `upper` pending (all of them when the LDS rows overflow) cannot fit
+        const bool took = r <= upper && window_gathered(S, W, w, r, lpk[wid], lpv[wid], keys, values, seps,
+                                                         row_offset, lower, upper, lk[wid], lv[wid]);
+        if (took) {
+            for (uint32_t u = lane; u < r; u += WAVE) pd[lix[wid][u]] = 1;
+        } else if (lane == 0) {
+            atomicAdd(d_next, r);
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_window(uint32_t S, uint32_t W, const gkey *__restrict__ pk,
                                                 const gval *__restrict__ pv, const uint32_t *__restrict__ pw,
                                                 uint8_t *__restrict__ pd, const uint32_t *__restrict__ d_np,
@@ -1478,6 +1624,16 @@ void launch_leaf(uint32_t S, const gkey *pk, const gval *pv, const uint32_t *pw,
     default: GPMA_LEAF(32); break;
     }
 #undef GPMA_LEAF
+}
+
+void launch_window_level(uint32_t S, uint32_t W, const gkey *pk, const gval *pv, const uint32_t *pw, uint8_t *pd,
+                         uint64_t n_ub, const uint32_t *d_n1, const uint32_t *d_in, uint32_t *d_next, gkey *keys,
+                         gval *values, gkey *seps, uint32_t *row_offset, uint32_t lower, uint32_t upper,
+                         const uint32_t *d_gate, uint32_t gate, hipStream_t st) {
+    if (!n_ub) return;
+    const dim3 grid((uint32_t)std::min<uint64_t>(n_ub, WINDOW_GRID));
+    hipLaunchKernelGGL(k_window_level, grid, dim3(256), 0, st, S, W, pk, pv, pw, pd, d_n1, d_in, d_next, keys, values,
+                       seps, row_offset, lower, upper, d_gate, gate);
 }
 
 void launch_window(uint32_t S, uint32_t W, const gkey *pk, const gval *pv, const uint32_t *pw, uint8_t *pd,
