@@ -123,11 +123,12 @@ void launch_count_pending(const uint8_t *done, uint64_t n_ub, const uint32_t *d_
 // Locate and level 0 in one pass, for S = 16 or 32 with row hints: every
 // sorted record finds its leaf; the first record of each leaf's run applies
 // the run's deletes and modifications and merges its pending inserts into
-// the leaf when level 0's bounds allow (win[head] = WIN_SEP | leaf, the new
-// first key in fix[head]), else leaves them pending (win = leaf base);
+// the leaf when level 0's bounds allow (a run at a tile edge leaves its new
+// first key in fix[head], win[head] = WIN_SEP | leaf, for the compaction),
+// else leaves them pending (win = leaf base);
 // other records get SIZE_NONE. Adds the level-0 pending count to *np0.
 void launch_apply(uint32_t S, const gkey *ks, const gval *vs, uint64_t n_ub, const uint32_t *d_n, gkey *keys,
-                  gval *values, const gkey *seps, uint64_t nseg, uint32_t *row_offset, uint32_t lower, uint32_t upper,
+                  gval *values, gkey *seps, uint64_t nseg, uint32_t *row_offset, uint32_t lower, uint32_t upper,
                   uint32_t *win, gkey *fix, uint32_t *np0, hipStream_t st);
 void launch_leaf(uint32_t S, const gkey *pk, const gval *pv, const uint32_t *pw, uint8_t *pd, uint64_t n_ub,
                  const uint32_t *d_np, gkey *keys, gval *values, gkey *seps, uint32_t *row_offset, uint32_t lower,

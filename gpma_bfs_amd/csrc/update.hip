@@ -486,12 +486,22 @@ __global__ __launch_bounds__(256) void k_leaf(const gkey *__restrict__ pk, const
 // bounds allow (gpma.cuh:636-654 with f = 1: the merged items packed at the
 // front of the leaf), merges them into the leaf. A run may continue past
 // the workgroup's tile: the head reads those records from global memory,
-// and their own threads, which are not heads, leave them alone. Nothing in
-// this kernel writes seps, so every search sees the separators as they
-// were before the batch; a merge that changes its leaf's first key passes
-// it on in fix[head] with win[head] = WIN_SEP | p, for the compaction that
-// follows. A merge moves guards only inside their leaf, so the concurrent
-// row hints (slot / S) read the same segment before and after it.
+// and their own threads, which are not heads, leave them alone.
+//
+// Separators: a merge only raises its leaf's first key (deletes drop keys,
+// inserts are >= the old separator; leaf 0's is never compared), so a
+// concurrent search for a key of any other leaf compares the same way with
+// the old or the new value. What could change is the head test of a record
+// in the same leaf (its predecessor may now lie below the raised separator)
+// and the run end of the previous leaf's head (a record of this leaf may
+// now lie below it). A merged run whose records all sit in the workgroup's
+// tile, after another head of the tile (so the previous leaf's head is here
+// too), has those reads done before any merge and writes its separator at
+// once; any other run (the tile's first, or one continuing into the next
+// tile) passes the new first key on in fix[head] with win[head] = WIN_SEP |
+// p, for the compaction that follows. A merge moves guards only inside
+// their leaf, so the concurrent row hints (slot / S) read the same segment
+// before and after it.
 //
 // The head keeps the leaf's keys in registers, its values as packed bytes
 // and its live slots as a bit mask; the merge stores every key straight to
@@ -507,7 +517,7 @@ __device__ __forceinline__ uint32_t byte_of(const uint32_t (&vw)[S / 4], uint32_
 template <uint32_t S>
 __global__ __launch_bounds__(256) void k_apply(const gkey *__restrict__ ks, const gval *__restrict__ vs, uint64_t n,
                                                const uint32_t *__restrict__ dn, gkey *__restrict__ keys,
-                                               gval *__restrict__ values, const gkey *__restrict__ seps,
+                                               gval *__restrict__ values, gkey *__restrict__ seps,
                                                uint64_t nseg, uint32_t *__restrict__ row_offset, uint32_t lower,
                                                uint32_t upper, uint32_t *__restrict__ win, gkey *__restrict__ fix,
                                                uint32_t *__restrict__ np0) {
@@ -517,6 +527,7 @@ __global__ __launch_bounds__(256) void k_apply(const gkey *__restrict__ ks, cons
     __shared__ gkey tk[T];
     __shared__ gval tv[T];
     __shared__ uint32_t wsum[T / WAVE];
+    __shared__ uint64_t hmask[T / WAVE];  // run heads of the tile
     __shared__ uint32_t rowv[T * S / 4];  // each head's merged values, packed
     if (dn) n = *dn;
     const uint64_t base = (uint64_t)blockIdx.x * T;
@@ -567,7 +578,9 @@ __global__ __launch_bounds__(256) void k_apply(const gkey *__restrict__ ks, cons
     } else if (i < n) {
         win[i] = SIZE_NONE;
     }
-    __syncthreads();  // tk / tv staged
+    const uint64_t hb = ballot(head);
+    if (lane_id() == 0) hmask[tid / WAVE] = hb;
+    __syncthreads();  // tk / tv / hmask staged
     uint32_t r = 0;
     if (head) {
         const uint64_t w = p * S;
@@ -683,8 +696,18 @@ __global__ __launch_bounds__(256) void k_apply(const gkey *__restrict__ ks, cons
 #pragma unroll
             for (uint32_t j = 0; j < S / 16; j++)
                 qv[j] = make_uint4(ov[4 * j], ov[4 * j + 1], ov[4 * j + 2], ov[4 * j + 3]);
-            fix[i] = first;
-            win[i] = WIN_SEP | (uint32_t)p;
+            // an earlier head in this tile: the previous leaf's head is here too
+            bool earlier = (hmask[tid / WAVE] & lanemask_lt()) != 0;
+            for (uint32_t q = 0; q < tid / WAVE; q++) earlier |= hmask[q] != 0;
+            if (jend <= tend && (i == 0 || earlier)) {
+                // the leaf's records and the previous leaf's head are all in
+                // this tile: no other workgroup reads this separator
+                seps[p] = first;
+                win[i] = SIZE_NONE;
+            } else {
+                fix[i] = first;
+                win[i] = WIN_SEP | (uint32_t)p;
+            }
         } else {
             if (dirty) {
                 uint4 *q = reinterpret_cast<uint4 *>(values + w);
@@ -1596,7 +1619,7 @@ void launch_count_pending(const uint8_t *done, uint64_t n_ub, const uint32_t *d_
 }
 
 void launch_apply(uint32_t S, const gkey *ks, const gval *vs, uint64_t n_ub, const uint32_t *d_n, gkey *keys,
-                  gval *values, const gkey *seps, uint64_t nseg, uint32_t *row_offset, uint32_t lower, uint32_t upper,
+                  gval *values, gkey *seps, uint64_t nseg, uint32_t *row_offset, uint32_t lower, uint32_t upper,
                   uint32_t *win, gkey *fix, uint32_t *np0, hipStream_t st) {
     if (!n_ub) return;
     dim3 g((uint32_t)((n_ub + 255) / 256)), b(256);
