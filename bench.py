@@ -953,8 +953,12 @@ def main():
 
     ub = update_bytes(step, step)
     step_s = t_total / K
-    # attribute algorithmic bytes to the kernels that move them
-    attr = {"sort": ub["sort"], "leaf": ub["leaf"], "locate": ub["locate"]}
+    # attribute algorithmic bytes to the kernels that move them; with 16 or 32
+    # slots per segment the locate and the leaf rebalance are one kernel
+    # (k_apply), timed as the locate phase
+    fused = bool(brk) and brk_launches.get("leaf", 0) == 0 and brk_launches.get("locate", 0) > 0
+    attr = {"sort": ub["sort"], "leaf": ub["leaf"],
+            "locate": ub["locate"] + (ub["leaf"] if fused else 0)}
     dom_ms = prof[dom]["ms"] / max(sampled, 1)
     dom_s = dom_ms / 1e3
     dom_launches = prof[dom]["launches"] / max(sampled, 1)
@@ -964,7 +968,8 @@ def main():
         "kernel": {"sort": "batch sort group: k_msd_rank + k_msd_scan + k_msd_scatter + k_msd_local, "
                            "one launch each per batch (k_stage_hist + k_sort_pass for batches that take the LSD sort)",
                    "leaf": "k_leaf<16> (leaf-level rebalance)",
-                   "locate": "k_locate<16> (locate + in-place delete / modify)"}[dom],
+                   "locate": ("k_apply<16> (locate, in-place delete / modify and the leaf-level rebalance, one pass)"
+                              if fused else "k_locate<16> (locate + in-place delete / modify)")}[dom],
         "achieved": attr[dom] / dom_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": attr[dom] / dom_s / 1e9 / HBM_PEAK_GBS,
         "traffic": traffic,
