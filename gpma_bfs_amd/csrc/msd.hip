@@ -423,7 +423,7 @@ MsdPlan msd_plan(uint64_t n, uint32_t N, uint32_t bucket_bits, bool by_row, uint
     pl.bs = bs;
     const uint32_t B = 2 * bs;
     uint32_t D = 1;
-    while (D < MAX_D && (n >> (D + 7)) != 0) D++;  // about 128 items per bucket
+    while (D < MAX_D && (n >> (D + 8)) != 0) D++;  // about 256 items per bucket
     if (bucket_bits) D = std::min(bucket_bits, MAX_D);
     if (D > B) D = B;
     pl.D = D;
