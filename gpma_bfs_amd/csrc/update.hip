@@ -983,16 +983,15 @@ __global__ __launch_bounds__(256) void k_window_wave(uint32_t S, uint32_t W, con
     }
 }
 // One window of W <= WAVE_WIN_MAX slots at base w by one wave, given its r
-// pending entries (sorted, r <= upper) in the wave's LDS rows lpk / lpv.
-// Same redistribution as window_wave; returns whether the level's bounds
-// let the window take them.
-__device__ bool window_gathered(uint32_t S, uint32_t W, uint32_t w, uint32_t r, const gkey *lpk, const gval *lpv,
-                                gkey *__restrict__ keys, gval *__restrict__ values, gkey *__restrict__ seps,
-                                uint32_t *__restrict__ row_offset, uint32_t lower, uint32_t upper, gkey *lk,
-                                gval *lv) {
+// pending entries (sorted, r <= upper) in the wave's LDS rows lpk / lpv and
+// its slots (window_slots). Same redistribution as window_wave; returns
+// whether the level's bounds let the window take them.
+// The window's slots, lane-strided, into registers (issued before the
+// caller gathers the pending entries, so both rounds of loads overlap).
+__device__ __forceinline__ void window_slots(uint32_t W, uint32_t w, const gkey *__restrict__ keys,
+                                             const gval *__restrict__ values, gkey (&kq)[WAVE_WIN_MAX / WAVE],
+                                             gval (&vq)[WAVE_WIN_MAX / WAVE]) {
     const uint32_t lane = lane_id();
-    gkey kq[WAVE_WIN_MAX / WAVE];
-    gval vq[WAVE_WIN_MAX / WAVE];
 #pragma unroll
     for (uint32_t q = 0; q < WAVE_WIN_MAX / WAVE; q++) {
         const uint32_t p = q * WAVE + lane;
@@ -1003,6 +1002,14 @@ __device__ bool window_gathered(uint32_t S, uint32_t W, uint32_t w, uint32_t r, 
             vq[q] = values[(uint64_t)w + p];
         }
     }
+}
+
+__device__ bool window_gathered(uint32_t S, uint32_t W, uint32_t w, uint32_t r, const gkey *lpk, const gval *lpv,
+                                const gkey (&kq)[WAVE_WIN_MAX / WAVE], const gval (&vq)[WAVE_WIN_MAX / WAVE],
+                                gkey *__restrict__ keys, gval *__restrict__ values, gkey *__restrict__ seps,
+                                uint32_t *__restrict__ row_offset, uint32_t lower, uint32_t upper, gkey *lk,
+                                gval *lv) {
+    const uint32_t lane = lane_id();
     uint32_t c = 0;
 #pragma unroll
     for (uint32_t q = 0; q < WAVE_WIN_MAX / WAVE; q++) {
@@ -1096,18 +1103,29 @@ __global__ __launch_bounds__(256) void k_window_level(uint32_t S, uint32_t W, co
         const uint32_t w = __shfl(wv, 1, WAVE);
         if (i > 0 && __shfl(wv, 0, WAVE) == w) continue;  // not the window's first entry
         wave_sync();  // the wave's previous window is done with its LDS rows
+        gkey kq[WAVE_WIN_MAX / WAVE];
+        gval vq[WAVE_WIN_MAX / WAVE];
+        window_slots(W, w, keys, values, kq, vq);
         // the window's entries not placed yet, in order, 64 entries per round
+        // (each entry's fields loaded with its window and flag)
         uint32_t r = 0;
         for (uint64_t j0 = i;; j0 += WAVE) {
             const uint64_t j = j0 + lane;
-            const bool in = j < n1 && (pw[j] & mask) == w;
-            const bool todo = in && pd[j] == 0;
+            bool in = false, todo = false;
+            gkey kj = 0;
+            gval vj = 0;
+            if (j < n1) {
+                in = (pw[j] & mask) == w;
+                todo = in && pd[j] == 0;
+                kj = pk[j];
+                vj = pv[j];
+            }
             const uint64_t b = ballot(todo);
             if (todo) {
                 const uint32_t pos = r + (uint32_t)__popcll(b & lanemask_lt());
                 if (pos < WAVE_WIN_MAX) {
-                    lpk[wid][pos] = pk[j];
-                    lpv[wid][pos] = pv[j];
+                    lpk[wid][pos] = kj;
+                    lpv[wid][pos] = vj;
                     lix[wid][pos] = (uint32_t)j;
                 }
             }
@@ -1117,7 +1135,7 @@ __global__ __launch_bounds__(256) void k_window_level(uint32_t S, uint32_t W, co
         if (r == 0) continue;  // every entry of the window was placed below
         wave_sync();
         // more than `upper` pending (all of them when the LDS rows overflow) cannot fit
-        const bool took = r <= upper && window_gathered(S, W, w, r, lpk[wid], lpv[wid], keys, values, seps,
+        const bool took = r <= upper && window_gathered(S, W, w, r, lpk[wid], lpv[wid], kq, vq, keys, values, seps,
                                                          row_offset, lower, upper, lk[wid], lv[wid]);
         if (took) {
             for (uint32_t u = lane; u < r; u += WAVE) pd[lix[wid][u]] = 1;
