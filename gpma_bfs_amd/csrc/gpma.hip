@@ -695,20 +695,24 @@ uint32_t enqueue_update(gpma_handle *g, const gkey *src_k, const gval *src_v, ui
         compact_pending(0, 0, ctr + C_LVL, ctr + C_LVL + 1);
     }
     // Levels 1..spec on the device. Windows of one wave (W <= LEVEL_WIN_MAX)
-    // work straight from the level-1 list: its done flags mark the placed
-    // entries, and each level counts the inserts entering the next one.
+    // work straight from the level-1 list, all such levels in one launch: its
+    // done flags mark the placed entries, and each level counts the inserts
+    // entering the next one.
     uint32_t spec = 0;
+    LevelBounds lb{};
     while (spec < dev_levels && spec + 1 <= g->h && ((uint64_t)g->S << (spec + 1)) <= LEVEL_WIN_MAX &&
            spec + 2 < C_LVL_SLOTS) {
-        const uint32_t L = spec + 1;
-        phase_begin(g, P_UPPER);
-        launch_window_level(g->S, g->S << L, g->pk1.as<gkey>(), g->pv1.as<gval>(), g->pw1.as<uint32_t>(),
-                            g->pd1.as<uint8_t>(), n_ub, ctr + C_LVL + 1, ctr + C_LVL + L, ctr + C_LVL + L + 1,
-                            g->keys.as<gkey>(), g->values.as<gval>(), g->seps.as<gkey>(),
-                            g->row_offset.as<uint32_t>(), g->lower[L], g->upper[L], ctr + C_LVL, g->bulk_threshold,
-                            st);
-        phase_end(g, P_UPPER, 1);
+        lb.lower[spec] = g->lower[spec + 1];
+        lb.upper[spec] = g->upper[spec + 1];
         spec++;
+    }
+    if (spec) {
+        phase_begin(g, P_UPPER);
+        launch_window_region(g->S, spec, g->pk1.as<gkey>(), g->pv1.as<gval>(), g->pw1.as<uint32_t>(),
+                             g->pd1.as<uint8_t>(), n_ub, ctr + C_LVL, g->keys.as<gkey>(), g->values.as<gval>(),
+                             g->seps.as<gkey>(), g->row_offset.as<uint32_t>(), lb, ctr + C_LVL, g->bulk_threshold,
+                             st);
+        phase_end(g, P_UPPER, 1);
     }
     // Wider windows (one workgroup, W <= WIN_MAX) take compacted lists: the
     // entries not placed yet, with their windows at the level, into list 0,

@@ -135,15 +135,19 @@ void launch_leaf(uint32_t S, const gkey *pk, const gval *pv, const uint32_t *pw,
                  uint32_t upper, uint32_t bulk_threshold, hipStream_t st);
 // Device-driven levels: windows of at most this many slots, one wave each.
 constexpr uint32_t LEVEL_WIN_MAX = 256;
-// One device-driven level (W <= LEVEL_WIN_MAX) over the level-1 list itself
-// (*d_n1 entries, level-1 windows in pw, done flags in pd): the first entry
-// of each window gathers the window's entries not placed yet and merges
-// them (done flags set) or adds their count to *d_next. Nothing runs when
-// *d_in == 0 or d_gate is set and *d_gate >= gate.
-void launch_window_level(uint32_t S, uint32_t W, const gkey *pk, const gval *pv, const uint32_t *pw, uint8_t *pd,
-                         uint64_t n_ub, const uint32_t *d_n1, const uint32_t *d_in, uint32_t *d_next, gkey *keys,
-                         gval *values, gkey *seps, uint32_t *row_offset, uint32_t lower, uint32_t upper,
-                         const uint32_t *d_gate, uint32_t gate, hipStream_t st);
+// Density bounds of levels 1..8 (index L - 1), in items per window.
+struct LevelBounds {
+    uint32_t lower[8], upper[8];
+};
+// Device-driven levels 1..nl (S << nl <= LEVEL_WIN_MAX) in one launch over
+// the level-1 list itself (d_lvl[1] entries, level-1 windows in pw, done
+// flags in pd): one wave per region of S << nl slots runs the levels in
+// order; placed entries get their done flag, and the inserts level L could
+// not place are added to d_lvl[L + 1]. Nothing runs when d_lvl[1] == 0 or
+// d_gate is set and *d_gate >= gate.
+void launch_window_region(uint32_t S, uint32_t nl, const gkey *pk, const gval *pv, const uint32_t *pw, uint8_t *pd,
+                          uint64_t n_ub, uint32_t *d_lvl, gkey *keys, gval *values, gkey *seps, uint32_t *row_offset,
+                          const LevelBounds &lb, const uint32_t *d_gate, uint32_t gate, hipStream_t st);
 // Levels whose window fits in LDS: workgroups stride over the first *d_np
 // pending entries, one window per run head. Does nothing when d_gate is set
 // and *d_gate >= gate (that batch takes the bulk path).

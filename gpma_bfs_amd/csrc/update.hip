@@ -982,165 +982,246 @@ __global__ __launch_bounds__(256) void k_window_wave(uint32_t S, uint32_t W, con
                     lv[wid], lpk[wid], lpv[wid]);
     }
 }
-// One window of W <= WAVE_WIN_MAX slots at base w by one wave, given its r
-// pending entries (sorted, r <= upper) in the wave's LDS rows lpk / lpv and
-// its slots (window_slots). Same redistribution as window_wave; returns
-// whether the level's bounds let the window take them.
-// The window's slots, lane-strided, into registers (issued before the
-// caller gathers the pending entries, so both rounds of loads overlap).
-__device__ __forceinline__ void window_slots(uint32_t W, uint32_t w, const gkey *__restrict__ keys,
-                                             const gval *__restrict__ values, gkey (&kq)[WAVE_WIN_MAX / WAVE],
-                                             gval (&vq)[WAVE_WIN_MAX / WAVE]) {
-    const uint32_t lane = lane_id();
-#pragma unroll
-    for (uint32_t q = 0; q < WAVE_WIN_MAX / WAVE; q++) {
-        const uint32_t p = q * WAVE + lane;
-        kq[q] = KEY_NONE;
-        vq[q] = 0;
-        if (p < W) {
-            kq[q] = keys[(uint64_t)w + p];
-            vq[q] = values[(uint64_t)w + p];
-        }
-    }
-}
-
-__device__ bool window_gathered(uint32_t S, uint32_t W, uint32_t w, uint32_t r, const gkey *lpk, const gval *lpv,
-                                const gkey (&kq)[WAVE_WIN_MAX / WAVE], const gval (&vq)[WAVE_WIN_MAX / WAVE],
-                                gkey *__restrict__ keys, gval *__restrict__ values, gkey *__restrict__ seps,
-                                uint32_t *__restrict__ row_offset, uint32_t lower, uint32_t upper, gkey *lk,
-                                gval *lv) {
-    const uint32_t lane = lane_id();
-    uint32_t c = 0;
-#pragma unroll
-    for (uint32_t q = 0; q < WAVE_WIN_MAX / WAVE; q++) {
-        const bool live = is_live(kq[q], vq[q]);
-        const uint64_t m = ballot(live);
-        if (live) {
-            const uint32_t pos = c + (uint32_t)__popcll(m & lanemask_lt());
-            lk[pos] = kq[q];
-            lv[pos] = vq[q];
-        }
-        c += (uint32_t)__popcll(m);
-    }
-    wave_sync();
-    const uint32_t m = c + r;
-    if (m < lower || m > upper) return false;
-    const uint64_t f = W / S;
-    for (uint32_t t = lane; t < c; t += WAVE) {
-        const gkey k = lk[t];
-        uint32_t lo = 0, hi = r;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (lpk[mid] < k) lo = mid + 1;
-            else hi = mid;
-        }
-        const uint32_t slot = redistribute_slot(f, S, m, t + lo);
-        keys[(uint64_t)w + slot] = k;
-        values[(uint64_t)w + slot] = lv[t];
-        if (is_guard(k)) row_offset[(k >> 32) + 1] = w + slot;
-        if (slot % S == 0) seps[(w + slot) / S] = k;
-    }
-    for (uint32_t u = lane; u < r; u += WAVE) {
-        const gkey k = lpk[u];
-        uint32_t lo = 0, hi = c;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (lk[mid] < k) lo = mid + 1;
-            else hi = mid;
-        }
-        const uint32_t slot = redistribute_slot(f, S, m, u + lo);
-        keys[(uint64_t)w + slot] = k;
-        values[(uint64_t)w + slot] = lpv[u];
-        if (is_guard(k)) row_offset[(k >> 32) + 1] = w + slot;
-        if (slot % S == 0) seps[(w + slot) / S] = k;
-    }
-    for (uint32_t p = lane; p < W; p += WAVE) {
-        const uint64_t s = p / S, l = p % S;
-        const uint64_t a = seg_first_item(f, m, s);
-        const uint64_t cnt = seg_first_item(f, m, s + 1) - a;
-        if (l >= cnt) {
-            keys[(uint64_t)w + p] = KEY_NONE;
-            values[(uint64_t)w + p] = 0;
-            if (l == 0) seps[(w + p) / S] = KEY_NONE;
-        }
-    }
-    return true;
-}
-
-// A device-driven level (W = S << level <= WAVE_WIN_MAX) straight from the
-// level-1 list: no compaction between levels. The list's entries keep their
-// level-1 window (pw) and done flag (pd) throughout; the level-`level`
-// window of an entry is pw & ~(W - 1), and the entries of one window are
-// consecutive (the list is sorted). One wave per entry: the first entry of
-// each window gathers the window's entries not placed yet (those whose
-// window one level down failed, gpma.cuh:1012-1029), merges them when the
-// level's bounds allow and sets their done flags, else adds their count to
-// *d_next (the inserts entering the next level). Nothing runs when no
-// insert enters this level (*d_in == 0) or the batch takes the bulk path.
-__global__ __launch_bounds__(256) void k_window_level(uint32_t S, uint32_t W, const gkey *__restrict__ pk,
-                                                      const gval *__restrict__ pv, const uint32_t *__restrict__ pw,
-                                                      uint8_t *__restrict__ pd, const uint32_t *__restrict__ d_n1,
-                                                      const uint32_t *__restrict__ d_in, uint32_t *__restrict__ d_next,
-                                                      gkey *__restrict__ keys, gval *__restrict__ values,
-                                                      gkey *__restrict__ seps, uint32_t *__restrict__ row_offset,
-                                                      uint32_t lower, uint32_t upper,
-                                                      const uint32_t *__restrict__ d_gate, uint32_t gate) {
+// Levels 1..nl of one batch in one launch, for W = S << L <= WAVE_WIN_MAX
+// at every L <= nl. Each window of those levels lies inside one aligned
+// region of R = S << nl slots, and the windows of different regions never
+// meet, so one wave per region runs the levels in the reference's order
+// (gpma.cuh:1012-1029: level L rebalances the windows that hold inserts level
+// L - 1 could not place) on an LDS copy of the region's slots, and writes the
+// windows it rebalanced back once at the end. Results equal level-by-level
+// launches exactly.
+//
+// The level-1 list (*d_lvl[1] entries, sorted) gives each entry its level-1
+// window (pw) and a done flag (pd, all clear on entry). The first entry of a
+// region makes its wave the region's owner; the entries of one level-1
+// window are consecutive, and they are placed or not together, so the wave
+// keeps one count per level-1 window (lane l: window l of the region, at
+// most R / 2S <= 64) and gathers a window's pending entries only for a merge
+// (the region's first 64 entries come from an LDS copy, later ones from the
+// list). Inserts that level L cannot place are added to d_lvl[L + 1] (the
+// inserts entering level L + 1). Nothing runs when no insert entered level 1
+// or the batch takes the bulk path (*d_gate >= gate).
+__global__ __launch_bounds__(256) void k_window_region(uint32_t S, uint32_t nl, const gkey *__restrict__ pk,
+                                                       const gval *__restrict__ pv,
+                                                       const uint32_t *__restrict__ pw, uint8_t *__restrict__ pd,
+                                                       uint32_t *__restrict__ d_lvl, gkey *__restrict__ keys,
+                                                       gval *__restrict__ values, gkey *__restrict__ seps,
+                                                       uint32_t *__restrict__ row_offset, LevelBounds lb,
+                                                       const uint32_t *__restrict__ d_gate, uint32_t gate) {
     constexpr uint32_t NW = 256 / WAVE;
-    __shared__ gkey lk[NW][WAVE_WIN_MAX];
+    __shared__ gkey rk[NW][WAVE_WIN_MAX];  // the region's slots
+    __shared__ __align__(16) gval rv[NW][WAVE_WIN_MAX];  // read and written as 4-byte words
+    __shared__ gkey lk[NW][WAVE_WIN_MAX];  // live items of the window being merged
     __shared__ gval lv[NW][WAVE_WIN_MAX];
-    __shared__ gkey lpk[NW][WAVE_WIN_MAX];
+    __shared__ gkey lpk[NW][WAVE_WIN_MAX];  // its pending inserts
     __shared__ gval lpv[NW][WAVE_WIN_MAX];
-    __shared__ uint32_t lix[NW][WAVE_WIN_MAX];
+    __shared__ gkey ek[NW][WAVE];  // the region's first 64 entries
+    __shared__ gval ev[NW][WAVE];
+    __shared__ uint32_t wlo[NW][WAVE];  // first entry of each level-1 window, relative
     if (d_gate && *d_gate >= gate) return;
-    if (*d_in == 0) return;
-    const uint64_t n1 = *d_n1;
-    const uint32_t mask = ~(W - 1);
+    const uint64_t n1 = d_lvl[1];
+    if (n1 == 0) return;
+    const uint32_t R = S << nl, W1 = S << 1, nw1 = R / W1;
+    const uint32_t sh1 = (uint32_t)__builtin_ctz(W1);
     const uint32_t wid = threadIdx.x / WAVE, lane = lane_id();
     const uint64_t nw = (uint64_t)gridDim.x * NW;
     for (uint64_t i = (uint64_t)blockIdx.x * NW + wid; i < n1; i += nw) {
-        uint32_t wv = SIZE_NONE;  // lane l: the window of entry i - 1 + l
-        if (lane > 0 ? i - 1 + lane < n1 : i > 0) wv = pw[i - 1 + lane] & mask;
-        const uint32_t w = __shfl(wv, 1, WAVE);
-        if (i > 0 && __shfl(wv, 0, WAVE) == w) continue;  // not the window's first entry
-        wave_sync();  // the wave's previous window is done with its LDS rows
-        gkey kq[WAVE_WIN_MAX / WAVE];
-        gval vq[WAVE_WIN_MAX / WAVE];
-        window_slots(W, w, keys, values, kq, vq);
-        // the window's entries not placed yet, in order, 64 entries per round
-        // (each entry's fields loaded with its window and flag)
-        uint32_t r = 0;
-        for (uint64_t j0 = i;; j0 += WAVE) {
-            const uint64_t j = j0 + lane;
-            bool in = false, todo = false;
-            gkey kj = 0;
-            gval vj = 0;
-            if (j < n1) {
-                in = (pw[j] & mask) == w;
-                todo = in && pd[j] == 0;
-                kj = pk[j];
-                vj = pv[j];
+        const uint32_t a = i + lane < n1 ? pw[i + lane] : SIZE_NONE;
+        const uint32_t prev = i > 0 ? pw[i - 1] : SIZE_NONE;
+        const uint32_t wr = __shfl(a, 0, WAVE) & ~(R - 1);
+        if (i > 0 && (prev & ~(R - 1)) == wr) continue;  // not the region's first entry
+        wave_sync();  // the wave's previous region is done with its LDS rows
+        // one round of loads: the region's slots (4 per lane) and the first
+        // 64 entries' keys and values
+        const uint32_t p4 = 4 * lane;
+        uint4 ka = make_uint4(0, 0, 0, 0), kb = ka;
+        uint32_t va = 0;
+        if (p4 < R) {
+            const uint4 *src = reinterpret_cast<const uint4 *>(keys + wr + p4);
+            ka = src[0];
+            kb = src[1];
+            va = *reinterpret_cast<const uint32_t *>(values + wr + p4);
+        }
+        gkey ekj = 0;
+        gval evj = 0;
+        if (i + lane < n1) {
+            ekj = pk[i + lane];
+            evj = pv[i + lane];
+        }
+        if (lane < nw1) wlo[wid][lane] = SIZE_NONE;
+        // the region's entries: first entry of each level-1 window, and the count e
+        uint32_t e = 0;
+        {
+            bool in = (a & ~(R - 1)) == wr && a != SIZE_NONE;
+            uint32_t carry = SIZE_NONE, x = a;
+            for (uint64_t j0 = i;; j0 += WAVE) {
+                const uint32_t up = __shfl_up(x, 1, WAVE);
+                const uint32_t before = lane == 0 ? carry : up;
+                wave_sync();
+                if (in && x != before) wlo[wid][(x & (R - 1)) >> sh1] = (uint32_t)(j0 - i) + lane;
+                const uint64_t out = ballot(!in);
+                if (out) {
+                    e = (uint32_t)(j0 - i) + (uint32_t)__builtin_ctzll(out);
+                    break;
+                }
+                carry = __shfl(x, WAVE - 1, WAVE);
+                const uint64_t j = j0 + WAVE + lane;
+                x = j < n1 ? pw[j] : SIZE_NONE;
+                in = (x & ~(R - 1)) == wr && x != SIZE_NONE;
             }
-            const uint64_t b = ballot(todo);
-            if (todo) {
-                const uint32_t pos = r + (uint32_t)__popcll(b & lanemask_lt());
-                if (pos < WAVE_WIN_MAX) {
-                    lpk[wid][pos] = kj;
-                    lpv[wid][pos] = vj;
-                    lix[wid][pos] = (uint32_t)j;
+        }
+        if (lane < e) {
+            ek[wid][lane] = ekj;
+            ev[wid][lane] = evj;
+        }
+        if (p4 < R) {
+            rk[wid][p4 + 0] = (gkey)ka.x | ((gkey)ka.y << 32);
+            rk[wid][p4 + 1] = (gkey)ka.z | ((gkey)ka.w << 32);
+            rk[wid][p4 + 2] = (gkey)kb.x | ((gkey)kb.y << 32);
+            rk[wid][p4 + 3] = (gkey)kb.z | ((gkey)kb.w << 32);
+            *reinterpret_cast<uint32_t *>(&rv[wid][p4]) = va;
+        }
+        wave_sync();
+        // lane l < nw1: level-1 window l's entries [lo, lo + cnt)
+        const uint32_t lo = lane < nw1 ? wlo[wid][lane] : SIZE_NONE;
+        const uint64_t present = ballot(lo != SIZE_NONE);
+        const uint64_t above = present & ~((2ull << lane) - 1);  // lane 63: 2 << 63 wraps to 0, so none
+        const uint32_t nxt = __shfl(lo, above ? (uint32_t)__builtin_ctzll(above) : 0, WAVE);
+        const uint32_t cnt0 = lo != SIZE_NONE ? (above ? nxt : e) - lo : 0;
+        uint32_t cnt = cnt0;    // pending (not placed yet)
+        bool dirty = false;     // window rebalanced: write it back
+        for (uint32_t L = 1; L <= nl; L++) {
+            const uint32_t g = 1u << (L - 1), W = S << L;
+            uint32_t P = cnt;  // pending inserts of this lane's level-L window
+            for (uint32_t o = 1; o < g; o <<= 1) P += __shfl_xor(P, o, WAVE);
+            uint64_t heads = ballot((lane & (g - 1)) == 0 && P > 0);
+            uint32_t fails = 0;
+            while (heads) {
+                const uint32_t h = (uint32_t)__builtin_ctzll(heads);
+                heads &= heads - 1;
+                const uint32_t r = __shfl(P, h, WAVE);
+                if (r > lb.upper[L - 1] || r > W) {  // cannot fit whatever the window holds
+                    fails += r;
+                    continue;
+                }
+                const bool mine = lane >= h && lane < h + g;
+                // its pending inserts, in order, from its level-1 windows not placed yet
+                uint64_t sub = ballot(mine && cnt > 0);
+                uint32_t off = 0;
+                while (sub) {
+                    const uint32_t s = (uint32_t)__builtin_ctzll(sub);
+                    sub &= sub - 1;
+                    const uint32_t cs = __shfl(cnt, s, WAVE), ls = __shfl(lo, s, WAVE);
+                    for (uint32_t t = lane; t < cs; t += WAVE) {
+                        const uint32_t rel = ls + t;
+                        gkey k;
+                        gval v;
+                        if (rel < WAVE) {
+                            k = ek[wid][rel];
+                            v = ev[wid][rel];
+                        } else {
+                            k = pk[i + rel];
+                            v = pv[i + rel];
+                        }
+                        lpk[wid][off + t] = k;
+                        lpv[wid][off + t] = v;
+                    }
+                    off += cs;
+                }
+                // the window's live items, in order
+                const uint32_t xo = h << sh1;
+                uint32_t c = 0;
+                for (uint32_t q = 0; q * WAVE < W; q++) {
+                    const uint32_t p = q * WAVE + lane;
+                    bool live = false;
+                    gkey k = 0;
+                    gval v = 0;
+                    if (p < W) {
+                        k = rk[wid][xo + p];
+                        v = rv[wid][xo + p];
+                        live = is_live(k, v);
+                    }
+                    const uint64_t mb = ballot(live);
+                    if (live) {
+                        const uint32_t pos = c + (uint32_t)__popcll(mb & lanemask_lt());
+                        lk[wid][pos] = k;
+                        lv[wid][pos] = v;
+                    }
+                    c += (uint32_t)__popcll(mb);
+                }
+                const uint32_t m = c + r;
+                if (m < lb.lower[L - 1] || m > lb.upper[L - 1]) {
+                    fails += r;
+                    continue;
+                }
+                wave_sync();
+                for (uint32_t p = lane; p < W; p += WAVE) {
+                    rk[wid][xo + p] = KEY_NONE;
+                    rv[wid][xo + p] = 0;
+                }
+                wave_sync();
+                const uint64_t f = W / S;
+                for (uint32_t t = lane; t < c; t += WAVE) {
+                    const gkey k = lk[wid][t];
+                    uint32_t b0 = 0, b1 = r;
+                    while (b0 < b1) {
+                        const uint32_t mid = (b0 + b1) >> 1;
+                        if (lpk[wid][mid] < k) b0 = mid + 1;
+                        else b1 = mid;
+                    }
+                    const uint32_t slot = redistribute_slot(f, S, m, t + b0);
+                    rk[wid][xo + slot] = k;
+                    rv[wid][xo + slot] = lv[wid][t];
+                }
+                for (uint32_t u = lane; u < r; u += WAVE) {
+                    const gkey k = lpk[wid][u];
+                    uint32_t b0 = 0, b1 = c;
+                    while (b0 < b1) {
+                        const uint32_t mid = (b0 + b1) >> 1;
+                        if (lk[wid][mid] < k) b0 = mid + 1;
+                        else b1 = mid;
+                    }
+                    const uint32_t slot = redistribute_slot(f, S, m, u + b0);
+                    rk[wid][xo + slot] = k;
+                    rv[wid][xo + slot] = lpv[wid][u];
+                }
+                wave_sync();
+                if (mine) {
+                    cnt = 0;
+                    dirty = true;
                 }
             }
-            r += (uint32_t)__popcll(b);
-            if (ballot(!in)) break;
+            if (fails && lane == 0) atomicAdd(d_lvl + L + 1, fails);
         }
-        if (r == 0) continue;  // every entry of the window was placed below
-        wave_sync();
-        // more than `upper` pending (all of them when the LDS rows overflow) cannot fit
-        const bool took = r <= upper && window_gathered(S, W, w, r, lpk[wid], lpv[wid], kq, vq, keys, values, seps,
-                                                         row_offset, lower, upper, lk[wid], lv[wid]);
-        if (took) {
-            for (uint32_t u = lane; u < r; u += WAVE) pd[lix[wid][u]] = 1;
-        } else if (lane == 0) {
-            atomicAdd(d_next, r);
+        // the rebalanced windows back (4 slots per lane: a level-1 window
+        // holds whole groups of 4), their separators and moved guards' rows
+        const uint64_t dmask = ballot(dirty);
+        if (dmask) {
+            if (p4 < R && ((dmask >> (p4 >> sh1)) & 1)) {
+                gkey k4[4];
+#pragma unroll
+                for (uint32_t q = 0; q < 4; q++) k4[q] = rk[wid][p4 + q];
+                uint4 *dst = reinterpret_cast<uint4 *>(keys + wr + p4);
+                dst[0] = make_uint4((uint32_t)k4[0], (uint32_t)(k4[0] >> 32), (uint32_t)k4[1],
+                                    (uint32_t)(k4[1] >> 32));
+                dst[1] = make_uint4((uint32_t)k4[2], (uint32_t)(k4[2] >> 32), (uint32_t)k4[3],
+                                    (uint32_t)(k4[3] >> 32));
+                *reinterpret_cast<uint32_t *>(values + wr + p4) = *reinterpret_cast<const uint32_t *>(&rv[wid][p4]);
+#pragma unroll
+                for (uint32_t q = 0; q < 4; q++) {
+                    const uint32_t p = wr + p4 + q;
+                    if (p % S == 0) seps[p / S] = k4[q];
+                    if (k4[q] != KEY_NONE && is_guard(k4[q])) row_offset[(k4[q] >> 32) + 1] = p;
+                }
+            }
+            // done flags of the placed entries
+            uint64_t placed = ballot(dirty && cnt0 > 0);
+            while (placed) {
+                const uint32_t s = (uint32_t)__builtin_ctzll(placed);
+                placed &= placed - 1;
+                const uint32_t cs = __shfl(cnt0, s, WAVE), ls = __shfl(lo, s, WAVE);
+                for (uint32_t t = lane; t < cs; t += WAVE) pd[i + ls + t] = 1;
+            }
         }
     }
 }
@@ -1667,14 +1748,13 @@ void launch_leaf(uint32_t S, const gkey *pk, const gval *pv, const uint32_t *pw,
 #undef GPMA_LEAF
 }
 
-void launch_window_level(uint32_t S, uint32_t W, const gkey *pk, const gval *pv, const uint32_t *pw, uint8_t *pd,
-                         uint64_t n_ub, const uint32_t *d_n1, const uint32_t *d_in, uint32_t *d_next, gkey *keys,
-                         gval *values, gkey *seps, uint32_t *row_offset, uint32_t lower, uint32_t upper,
-                         const uint32_t *d_gate, uint32_t gate, hipStream_t st) {
-    if (!n_ub) return;
+void launch_window_region(uint32_t S, uint32_t nl, const gkey *pk, const gval *pv, const uint32_t *pw, uint8_t *pd,
+                          uint64_t n_ub, uint32_t *d_lvl, gkey *keys, gval *values, gkey *seps, uint32_t *row_offset,
+                          const LevelBounds &lb, const uint32_t *d_gate, uint32_t gate, hipStream_t st) {
+    if (!n_ub || nl == 0) return;
     const dim3 grid((uint32_t)std::min<uint64_t>(n_ub, WINDOW_GRID));
-    hipLaunchKernelGGL(k_window_level, grid, dim3(256), 0, st, S, W, pk, pv, pw, pd, d_n1, d_in, d_next, keys, values,
-                       seps, row_offset, lower, upper, d_gate, gate);
+    hipLaunchKernelGGL(k_window_region, grid, dim3(256), 0, st, S, nl, pk, pv, pw, pd, d_lvl, keys, values, seps,
+                       row_offset, lb, d_gate, gate);
 }
 
 void launch_window(uint32_t S, uint32_t W, const gkey *pk, const gval *pv, const uint32_t *pw, uint8_t *pd,
