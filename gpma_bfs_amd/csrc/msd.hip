@@ -43,6 +43,7 @@ constexpr uint32_t RUN = WAVE;               // items per register-sorted run
 constexpr uint32_t SCAN_B = WAVE;            // buckets per scan block
 constexpr uint32_t SCAN_T = 1024;            // threads per scan block
 constexpr uint32_t SCAN_G = SCAN_T / SCAN_B; // tile groups per scan block
+constexpr uint32_t SCAN_R = 32;              // tiles per scan thread held in registers (else read twice)
 
 struct PackInfo {
     uint32_t N;          // row_num
@@ -82,7 +83,11 @@ __global__ __launch_bounds__(MT) void k_msd_rank(const gkey *__restrict__ ik, co
     if (dn) n = *dn;
     if ((uint64_t)blockIdx.x * TILE >= n) return;  // tiles past the batch (grid sized for a capacity)
     const uint32_t NB = 1u << D;
-    for (uint32_t i = threadIdx.x; i < MW * MAX_NB / 2; i += MT) reinterpret_cast<uint32_t *>(&cnt[0][0])[i] = 0;
+    {  // zero the first NB counters of every wave's row (NB >= 2)
+        const uint32_t hs = D - 1;  // log2 of the u32 words in use per row
+        for (uint32_t i = threadIdx.x; i < (MW << hs); i += MT)
+            reinterpret_cast<uint32_t *>(&cnt[i >> hs][0])[i & ((1u << hs) - 1)] = 0;
+    }
     const uint32_t wid = threadIdx.x / WAVE, lane = lane_id();
     const uint64_t wbase = (uint64_t)blockIdx.x * TILE + (uint64_t)wid * WI;
     gkey k[IPT];
@@ -166,8 +171,20 @@ __global__ __launch_bounds__(SCAN_T) void k_msd_scan(const uint16_t *__restrict_
     const uint32_t bkt = blk * SCAN_B + bl;
     const uint32_t tg = (ntiles + SCAN_G - 1) / SCAN_G;
     const uint32_t t0 = min(ntiles, g * tg), t1 = min(ntiles, t0 + tg);
+    // up to SCAN_R tiles per thread: every count load in flight at once and
+    // kept in registers for the second pass
+    const bool inreg = tg <= SCAN_R;
+    uint32_t cr[SCAN_R];
     uint32_t sum = 0;
-    if (bkt < NB) {
+    if (inreg) {
+#pragma unroll
+        for (uint32_t r = 0; r < SCAN_R; r++) {
+            const uint32_t t = t0 + r;
+            cr[r] = bkt < NB && t < t1 ? counts[(uint64_t)t * NB + bkt] : 0u;
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < SCAN_R; r++) sum += cr[r];
+    } else if (bkt < NB) {
 #pragma unroll 8
         for (uint32_t t = t0; t < t1; t++) sum += counts[(uint64_t)t * NB + bkt];
     }
@@ -192,7 +209,16 @@ __global__ __launch_bounds__(SCAN_T) void k_msd_scan(const uint16_t *__restrict_
         base_s[bl] = excl + v - col;
     }
     __syncthreads();
-    if (bkt < NB) {
+    if (bkt < NB && inreg) {
+        uint32_t run = base_s[bl] + gpre;
+        if (g == 0) bstart[bkt] = base_s[bl];
+#pragma unroll
+        for (uint32_t r = 0; r < SCAN_R; r++) {
+            const uint32_t t = t0 + r;
+            if (t < t1) offs[(uint64_t)t * NB + bkt] = run;
+            run += cr[r];
+        }
+    } else if (bkt < NB) {
         uint32_t run = base_s[bl] + gpre;
         if (g == 0) bstart[bkt] = base_s[bl];
 #pragma unroll 8
