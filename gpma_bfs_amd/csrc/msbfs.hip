@@ -456,22 +456,48 @@ __global__ __launch_bounds__(T) void k_ms_finalize(const uint32_t *__restrict__ 
     }
 }
 
+// The 8 bits of x at bit positions 0, 4, 8, ..., 28.
+__device__ __forceinline__ uint32_t spread4(uint32_t x) {
+    x = (x | (x << 12)) & 0x000F000Fu;
+    x = (x | (x << 6)) & 0x03030303u;
+    return (x | (x << 3)) & 0x11111111u;
+}
+
+// Row of source S in a lane's level table: bit k of the index is bit S of
+// plane k (the planes of the event index), bit 3 is bit S of nc (no event
+// holds S).
+__device__ __forceinline__ uint32_t ms_event_row(uint32_t S, uint64_t p0, uint64_t p1, uint64_t p2, uint64_t nc) {
+    const uint32_t h = S & 32, b = S & 31;
+    const uint32_t w0 = (uint32_t)(p0 >> h), w1 = (uint32_t)(p1 >> h), w2 = (uint32_t)(p2 >> h),
+                   w3 = (uint32_t)(nc >> h);
+    return ((w0 >> b) & 1u) | (((w1 >> b) & 1u) << 1) | (((w2 >> b) & 1u) << 2) | (((w3 >> b) & 1u) << 3);
+}
+
 // Source-major u32 levels from the events (and the overflow rows), with
-// per-source reachable counts. One wave per 64-vertex group, no LDS and no
-// barrier: each lane loads its vertex's events (only if some source reached
-// it): source s's level is the level of the event whose mask holds bit s, or
-// the overflow row's entry if none does. Then for every source the wave
-// writes the group's 64 levels as one coalesced store, and lane s counts
-// source s with a ballot. partial[wave][s]: per-wave counts. Workgroup 0 also
-// zeroes reach[0 .. nsrc) for k_ms_reach.
-// WIDE: the overflow rows hold u32 levels, else u16.
+// per-source reachable counts. One wave per 64-vertex group, no barrier:
+// each lane loads its vertex's events (only if some source reached it) and
+// keeps their levels in its own column of an LDS table (rows 0..7; rows
+// 8..15 hold 0). Source s's level at the lane's vertex is then one table
+// read at row idx(s) = e(s) | 8 * (no event holds s), e(s) = the index of the
+// event whose mask holds bit s (a source sits in at most one event), taken
+// from three bit planes of the event indices: no branch per source. For
+// every source the wave writes the group's 64 levels as one coalesced store,
+// and lane s counts source s with a ballot. A vertex reached at more than
+// MS_EVENTS levels then overwrites its sources past the events from its
+// overflow row. partial[wave][s]: per-wave counts. Workgroup 0 also zeroes
+// reach[0 .. nsrc) for k_ms_reach. WIDE: the overflow rows hold u32 levels,
+// else u16.
 template <bool WIDE>
 __global__ __launch_bounds__(T) void k_ms_output(const uint4 *__restrict__ ev, const uint8_t *__restrict__ evn,
                                                  const void *__restrict__ lv, const uint64_t *__restrict__ seen,
                                                  uint32_t N, uint32_t nsrc, uint32_t *__restrict__ out,
                                                  uint32_t *__restrict__ partial, uint32_t *__restrict__ reach) {
+    __shared__ uint32_t tab[2 * MS_EVENTS][T];
+    __shared__ uint32_t wcnt[T / WAVE][64];  // per wave: one group's count per source
     const uint32_t lane = lane_id();
     if (blockIdx.x == 0 && threadIdx.x < nsrc) reach[threadIdx.x] = 0;  // k_ms_reach adds into it
+#pragma unroll
+    for (uint32_t k = MS_EVENTS; k < 2 * MS_EVENTS; k++) tab[k][threadIdx.x] = 0;
     const uint32_t gw = (blockIdx.x * T + threadIdx.x) / WAVE;
     const uint32_t nw = gridDim.x * (T / WAVE);
     uint32_t acc = 0;
@@ -481,33 +507,58 @@ __global__ __launch_bounds__(T) void k_ms_output(const uint4 *__restrict__ ev, c
         const uint64_t mask = in ? seen[v] : 0;
         const uint8_t en = mask ? evn[v] : 0;
         const uint32_t ne = en & ~EV_OVF;
-        // the events as bit planes: source s is in event idx(s) = p0 | p1 << 1 | p2 << 2
-        // (a source sits in at most one event), cov = every source some event holds
         uint64_t p0 = 0, p1 = 0, p2 = 0, cov = 0;
-        uint32_t L[MS_EVENTS];
 #pragma unroll
         for (uint32_t k = 0; k < MS_EVENTS; k++) {
             const uint4 e = k < ne ? ev[(uint64_t)k * N + v] : make_uint4(0, 0, 0, 0);
             const uint64_t m = (uint64_t)e.y << 32 | e.x;
-            L[k] = e.z;
+            tab[k][threadIdx.x] = e.z;
             cov |= m;
             if (k & 1) p0 |= m;
             if (k & 2) p1 |= m;
             if (k & 4) p2 |= m;
         }
         static_assert(MS_EVENTS == 8, "three bit planes");
+        const uint64_t nc = ~cov;
+        if (nsrc == 64 && v0 + WAVE <= N && N <= (1u << 30)) {
+            // a full group and every source: no per-lane or per-source
+            // condition. nib[j]: the table rows of sources 8j..8j+7, 4 bits each
+            uint32_t nib[8];
 #pragma unroll
-        for (uint32_t s = 0; s < 64; s++) {
-            const bool b0 = (p0 >> s) & 1ull, b1 = (p1 >> s) & 1ull, b2 = (p2 >> s) & 1ull;
-            const uint32_t t0 = b0 ? L[1] : L[0], t1 = b0 ? L[3] : L[2], t2 = b0 ? L[5] : L[4], t3 = b0 ? L[7] : L[6];
-            uint32_t level = b2 ? (b1 ? t3 : t2) : (b1 ? t1 : t0);
-            const bool hit = (mask >> s) & 1ull;
-            if (hit && !((cov >> s) & 1ull))  // past the event slots: the overflow row
-                level = WIDE ? reinterpret_cast<const uint32_t *>(lv)[v * 64 + s]
-                             : reinterpret_cast<const uint16_t *>(lv)[v * 64 + s];
-            if (s < nsrc && in) out[(uint64_t)s * N + v] = hit ? level : 0u;
-            const uint32_t c = (uint32_t)__popcll(ballot(hit));
-            if (lane == s) acc += c;
+            for (uint32_t j = 0; j < 8; j++) {
+                const uint32_t h = (j >> 2) * 32, b = (j & 3) * 8;
+                nib[j] = spread4(((uint32_t)(p0 >> h) >> b) & 0xFFu) | spread4(((uint32_t)(p1 >> h) >> b) & 0xFFu) << 1 |
+                         spread4(((uint32_t)(p2 >> h) >> b) & 0xFFu) << 2 | spread4(((uint32_t)(nc >> h) >> b) & 0xFFu) << 3;
+            }
+            const uint32_t voff = (uint32_t)v * 4u;  // byte offset in a source row
+            uint32_t *wc = wcnt[threadIdx.x / WAVE];
+#pragma unroll
+            for (uint32_t s = 0; s < 64; s++) {
+                const uint32_t idx = (nib[s >> 3] >> ((s & 7) * 4)) & 15u;
+                char *row = reinterpret_cast<char *>(out + (uint64_t)s * N);
+                *reinterpret_cast<uint32_t *>(row + voff) = tab[idx][threadIdx.x];
+                wc[s] = (uint32_t)__popcll(ballot((uint32_t)(mask >> s) & 1u));  // same word from every lane
+            }
+            wave_sync();
+            acc += wc[lane];
+            wave_sync();
+        } else {
+            const uint32_t vi = in ? (uint32_t)v : 0u;
+            for (uint32_t s = 0; s < nsrc; s++) {
+                const uint32_t idx = ms_event_row(s, p0, p1, p2, nc);
+                const uint32_t level = tab[idx][threadIdx.x];
+                if (in) out[(uint64_t)s * N + vi] = level;
+                const uint32_t c = (uint32_t)__popcll(ballot((uint32_t)(mask >> s) & 1u));
+                acc += lane == s ? c : 0u;
+            }
+        }
+        if (en & EV_OVF) {  // sources past the event slots: the overflow row
+            for (uint64_t bits = mask & ~cov; bits; bits &= bits - 1) {
+                const uint32_t s = (uint32_t)__ffsll((unsigned long long)bits) - 1;
+                if (s < nsrc)
+                    out[(uint64_t)s * N + v] = WIDE ? reinterpret_cast<const uint32_t *>(lv)[v * 64 + s]
+                                                    : reinterpret_cast<const uint16_t *>(lv)[v * 64 + s];
+            }
         }
     }
     partial[(uint64_t)gw * WAVE + lane] = acc;
