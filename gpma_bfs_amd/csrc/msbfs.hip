@@ -484,7 +484,7 @@ __device__ __forceinline__ uint32_t ms_event_row(uint32_t S, uint64_t p0, uint64
 // every source the wave writes the group's 64 levels as one coalesced store,
 // and lane s counts source s with a ballot. A vertex reached at more than
 // MS_EVENTS levels then overwrites its sources past the events from its
-// overflow row. partial[wave][s]: per-wave counts. Workgroup 0 also zeroes
+// overflow row. partial[workgroup][s]: per-workgroup counts. Workgroup 0 also zeroes
 // reach[0 .. nsrc) for k_ms_reach. WIDE: the overflow rows hold u32 levels,
 // else u16.
 template <bool WIDE>
@@ -561,13 +561,21 @@ __global__ __launch_bounds__(T) void k_ms_output(const uint4 *__restrict__ ev, c
             }
         }
     }
-    partial[(uint64_t)gw * WAVE + lane] = acc;
+    __syncthreads();  // every wave is past its loop: wcnt is free
+    wcnt[threadIdx.x / WAVE][lane] = acc;
+    __syncthreads();
+    if (threadIdx.x < WAVE) {
+        uint32_t t = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < T / WAVE; w++) t += wcnt[w][threadIdx.x];
+        partial[(uint64_t)blockIdx.x * WAVE + threadIdx.x] = t;
+    }
 }
 
 // reach[s] += the sum of column s over a share of the partials' rows: each
 // workgroup reads whole 64-word rows (coalesced), reduces in LDS and adds
 // one word per source (reach was zeroed by the output pass).
-constexpr uint32_t RT = 1024, RB = 16;  // threads and workgroups of the reduction
+constexpr uint32_t RT = 1024, RB = 64;  // threads and workgroups of the reduction
 __global__ __launch_bounds__(RT) void k_ms_reach(const uint32_t *__restrict__ partial, uint32_t nrows, uint32_t nsrc,
                                                  uint32_t *__restrict__ reach) {
     __shared__ uint32_t red[RT / 64][64];
@@ -645,10 +653,10 @@ void ms_output(const MsState &m, uint32_t N, uint32_t nsrc, uint32_t *out, uint3
     else
         hipLaunchKernelGGL(k_ms_output<false>, dim3(nb), dim3(T), 0, st, m.ev, m.evn, (const void *)m.lv16, m.seen, N,
                            nsrc, out, m.partial, reach);
-    hipLaunchKernelGGL(k_ms_reach, dim3(RB), dim3(RT), 0, st, m.partial, nb * (T / WAVE), nsrc, reach);
+    hipLaunchKernelGGL(k_ms_reach, dim3(RB), dim3(RT), 0, st, m.partial, nb, nsrc, reach);
 }
 
-uint32_t ms_output_blocks(uint32_t N) { return ms_output_grid(N) * (T / WAVE); }
+uint32_t ms_output_blocks(uint32_t N) { return ms_output_grid(N); }
 
 void ms_widen(const MsState &m, uint32_t N, uint32_t *lv32, hipStream_t st) {
     hipLaunchKernelGGL(k_ms_widen, dim3(4096), dim3(256), 0, st, m.lv16, m.seen, N, lv32);
