@@ -29,8 +29,10 @@ then 64 single-source BFS), and the reference's GPMA CPU path (restated in
 oracle/) on every host core, and a serving leg (R-MAT 2^24 nodes / 2^28
 draws through the request server: 10^4 requests open-loop at 5,000/s, then
 10^4 submitted at once for the saturation rate; p50/p99 latency per request
-class). --large, --serve and --batched-only run the other configurations
-instead.
+class) and a bounded large leg (R-MAT 2^25 nodes / 2^29 draws: bulk load, 20
+slides of 2^21 records with a 64-source BFS after every 10th, then 10 captured
+batched steps). --large, --serve and --batched-only run the other
+configurations instead (--large: the whole 200-slide protocol).
 
 Prints progress on stderr and ONE JSON line on stdout.
 """
@@ -721,6 +723,10 @@ def main():
     ap.add_argument("--large", action="store_true",
                     help="R-MAT 2^25 nodes / 2^29 draws instead (bulk load, 2^21-update slides, 64-source BFS)")
     ap.add_argument("--large-steps", type=int, default=200)
+    ap.add_argument("--large-leg-steps", type=int, default=20,
+                    help="slides of the R-MAT 2^25 / 2^29 leg in the default run (0 = skip)")
+    ap.add_argument("--large-leg-batched-steps", type=int, default=10,
+                    help="batched steps of that leg")
     ap.add_argument("--large-batched-steps", type=int, default=20,
                     help="captured batched steps after the plain slides of --large (0 = skip)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
@@ -1062,6 +1068,20 @@ def main():
         serve_leg = run_serve(G, args.serve_leg_requests, args.serve_rate, args.serve_order)
         log(f"serving leg parity: {serve_leg['parity']}")
 
+    large_leg = None
+    if args.large_leg_steps > 0 and not args.small:
+        g.close()
+        la = argparse.Namespace(**vars(args))
+        la.large_steps, la.large_batched_steps = args.large_leg_steps, args.large_leg_batched_steps
+        log(f"large leg: R-MAT 2^25 / 2^29, {la.large_steps} slides of 2^21 records, "
+            f"{la.large_batched_steps} batched steps")
+        large_leg = run_large(G, la)
+        lb = large_leg.get("batched") or {}
+        log(f"large leg: {large_leg['edge_updates_per_s'] / 1e9:.3f} G edge-updates/s plain, "
+            f"{(lb.get('edge_updates_per_s') or 0) / 1e9:.3f} batched; 64-source BFS "
+            f"{large_leg['msbfs']['ms_median']:.1f} ms ({large_leg['msbfs']['gteps']:.1f} GTEPS, "
+            f"{large_leg['msbfs']['frac'] * 100:.1f}% of HBM peak); parity {large_leg['parity']}")
+
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -1107,6 +1127,7 @@ def main():
         "tuned_plain": tuned_plain,
         "batched": batched,
         "serve": serve_leg,
+        "large": large_leg,
         "latency_ms": {"p50": float(np.percentile(per_ms, 50)), "p99": float(np.percentile(per_ms, 99))},
         "protocol": protocol,
         "parity": parity,
