@@ -539,6 +539,35 @@ def test_multi_source_bfs_deeper_than_u16_levels():
     g.close()
 
 
+def test_multi_source_bfs_many_levels_per_vertex():
+    """Vertices reached at more than the 8 event slots' distinct levels
+    (the u16 overflow rows, depth below 65,535): 64 sources spread along a
+    3,000-vertex chain with side branches, so a chain vertex is reached at up
+    to 64 levels. N = 3,000 leaves a partial last 64-vertex group, so the
+    output pass's full-group path and its per-lane path both run; 63 sources
+    take the per-lane path everywhere. Every row equals the oracle."""
+    n_chain, n_side = 2900, 100
+    N = n_chain + n_side
+    c = np.arange(n_chain - 1, dtype=np.uint32)
+    k = np.arange(n_side, dtype=np.uint32)
+    src = np.concatenate([c, 29 * k, n_chain + k])
+    dst = np.concatenate([c + 1, n_chain + k, np.minimum(29 * k + 2, n_chain - 1)])
+    keys = np.unique(G.make_keys(src, dst))
+    g = G.GPMA(N)
+    G.update_gpma(g, keys, np.ones(len(keys), np.uint8))
+    s, d = _edges(keys)
+    starts = (np.arange(64, dtype=np.uint32) * 45).astype(np.uint32)
+    starts[63] = n_chain + 7  # a side vertex
+    olv, oreach = O.bfs_edges_multi(N, s, d, starts)
+    per_vertex = np.array([len(np.unique(col[col > 0])) for col in olv.T])
+    assert per_vertex.max() > 8 and olv.max() < 65535
+    lv, reach = G.gpma_bfs_multi(g, starts)
+    assert np.array_equal(lv, olv) and np.array_equal(reach, oreach)
+    lv63, reach63 = G.gpma_bfs_multi(g, starts[:63])
+    assert np.array_equal(lv63, olv[:63]) and np.array_equal(reach63, oreach[:63])
+    g.close()
+
+
 def test_rebuild_allocation_failure_keeps_the_handle_usable():
     """A bulk rebuild whose new arrays cannot be allocated: (a) the new arrays
     do not fit beside the old ones, so the old go first and the rebuild
