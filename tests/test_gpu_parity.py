@@ -214,6 +214,40 @@ def test_million_record_batches():
     assert reach == oreach and np.array_equal(lv, olv)
 
 
+def test_sort_buckets_past_the_lds_size():
+    """Batches of about 3.2M records: their 4,096 sort buckets average ~785
+    items, and many pass the local pass's 1,024-item in-LDS size (those take
+    its LSD passes over the bucket's range). Two slides through update_gpma
+    and one as 8 fused batches through gpma_update_batches; the live set
+    equals the closed form after each, and BFS equals the oracle."""
+    N = 1 << 20
+    src, dst = G.gen_rmat(20, 1 << 24, seed=19)
+    keys = G.make_keys(src, dst)
+    half = len(keys) // 2
+    step = half // 5
+    assert 2 * step > 640 * 4096 and 2 * step <= 4 << 20
+    g = G.GPMA(N)
+    G.update_gpma(g, keys[:half], np.ones(half, np.uint8))
+    batches = list(protocol_batches(keys, 3, num_slide=5))
+    for i, b, vals in batches[:2]:
+        G.update_gpma(g, b, vals)
+        assert np.array_equal(g.export_live(), closed_form_live(keys, N, i + 1, num_slide=5)), f"slide {i}"
+    _, b, vals = batches[2]
+    ins, dele = b[:step], b[step:]
+    parts, pv = [], []
+    for j in range(8):  # uneven inserts, then uneven deletes, in 8 batches
+        a0, a1 = (j * step) // 8, ((j + 1) * step) // 8
+        parts.append(np.concatenate([ins[a0:a1], dele[a0:a1]]))
+        pv.append(np.concatenate([np.ones(a1 - a0, np.uint8), np.zeros(a1 - a0, np.uint8)]))
+    G.update_gpma_batches(g, parts, pv)
+    assert np.array_equal(g.export_live(), closed_form_live(keys, N, 3, num_slide=5)), "fused slide"
+    assert_invariants(g)
+    lv, reach = G.gpma_bfs(g, 0)
+    olv, oreach = O.bfs_edges(N, src[3 * step: half + 3 * step], dst[3 * step: half + 3 * step], 0)
+    assert reach == oreach and np.array_equal(lv, olv)
+    g.close()
+
+
 def test_wide_window_levels_match_oracle_layout():
     """The 4.2M-record window on 2^20 vertices (under the bulk threshold)
     climbs through rebalance levels whose windows (8,192 slots and more) are
