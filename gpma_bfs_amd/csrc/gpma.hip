@@ -1533,6 +1533,7 @@ int gpma_create(uint32_t row_num, const gpma_config *cfg, gpma_t *out) {
             rc = fail(GPMA_ERR_HIP, "hipStreamCreate failed");
             break;
         }
+        window_region_prepare();  // its launch shape, before any graph capture
         if ((rc = g->ctr.ensure(C_COUNT * 4, &g->mem)) != GPMA_OK) break;
         if (hipHostMalloc((void **)&g->hctr, C_COUNT * 4, hipHostMallocDefault) != hipSuccess ||
             hipHostMalloc((void **)&g->sig, SIG_WORDS * 4, hipHostMallocCoherent) != hipSuccess) {

@@ -145,6 +145,9 @@ struct LevelBounds {
 // order; placed entries get their done flag, and the inserts level L could
 // not place are added to d_lvl[L + 1]. Nothing runs when d_lvl[1] == 0 or
 // d_gate is set and *d_gate >= gate.
+// Looks up (once per device) how many workgroups of the region kernel the
+// current device holds at once; the region launches use it.
+void window_region_prepare();
 void launch_window_region(uint32_t S, uint32_t nl, const gkey *pk, const gval *pv, const uint32_t *pw, uint8_t *pd,
                           uint64_t n_ub, uint32_t *d_lvl, gkey *keys, gval *values, gkey *seps, uint32_t *row_offset,
                           const LevelBounds &lb, const uint32_t *d_gate, uint32_t gate, hipStream_t st);

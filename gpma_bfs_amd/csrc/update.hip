@@ -12,6 +12,8 @@
 //           a failing window is left as it was and its inserts move up.
 // The per-window merge replaces the reference's per-window radix sort: the
 // live items are already sorted and so are the pending inserts.
+#include <atomic>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -1001,6 +1003,7 @@ __global__ __launch_bounds__(256) void k_window_wave(uint32_t S, uint32_t W, con
 // list). Inserts that level L cannot place are added to d_lvl[L + 1] (the
 // inserts entering level L + 1). Nothing runs when no insert entered level 1
 // or the batch takes the bulk path (*d_gate >= gate).
+template <uint32_t RM>
 __global__ __launch_bounds__(256) void k_window_region(uint32_t S, uint32_t nl, const gkey *__restrict__ pk,
                                                        const gval *__restrict__ pv,
                                                        const uint32_t *__restrict__ pw, uint8_t *__restrict__ pd,
@@ -1009,12 +1012,13 @@ __global__ __launch_bounds__(256) void k_window_region(uint32_t S, uint32_t nl, 
                                                        uint32_t *__restrict__ row_offset, LevelBounds lb,
                                                        const uint32_t *__restrict__ d_gate, uint32_t gate) {
     constexpr uint32_t NW = 256 / WAVE;
-    __shared__ gkey rk[NW][WAVE_WIN_MAX];  // the region's slots
-    __shared__ __align__(16) gval rv[NW][WAVE_WIN_MAX];  // read and written as 4-byte words
-    __shared__ gkey lk[NW][WAVE_WIN_MAX];  // live items of the window being merged
-    __shared__ gval lv[NW][WAVE_WIN_MAX];
-    __shared__ gkey lpk[NW][WAVE_WIN_MAX];  // its pending inserts
-    __shared__ gval lpv[NW][WAVE_WIN_MAX];
+    static_assert(RM == 128 || RM == WAVE_WIN_MAX, "4 slots per lane cover a region");
+    __shared__ gkey rk[NW][RM];  // the region's slots (R = S << nl <= RM)
+    __shared__ __align__(16) gval rv[NW][RM];  // read and written as 4-byte words
+    __shared__ gkey lk[NW][RM];  // live items of the window being merged
+    __shared__ gval lv[NW][RM];
+    __shared__ gkey lpk[NW][RM];  // its pending inserts
+    __shared__ gval lpv[NW][RM];
     __shared__ gkey ek[NW][WAVE];  // the region's first 64 entries
     __shared__ gval ev[NW][WAVE];
     __shared__ uint32_t wlo[NW][WAVE];  // first entry of each level-1 window, relative
@@ -1748,13 +1752,46 @@ void launch_leaf(uint32_t S, const gkey *pk, const gval *pv, const uint32_t *pw,
 #undef GPMA_LEAF
 }
 
+// Workgroups of k_window_region<RM> the current device holds at once (at
+// least WINDOW_GRID), cached per device.
+template <uint32_t RM>
+static uint64_t region_grid() {
+    static std::atomic<uint64_t> cache[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return WINDOW_GRID;
+    uint64_t g = cache[dev].load(std::memory_order_relaxed);
+    if (!g) {
+        int per_cu = 0, cus = 0;
+        g = WINDOW_GRID;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_window_region<RM>, 256, 0) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && per_cu > 0 &&
+            cus > 0)
+            g = std::max<uint64_t>(WINDOW_GRID, (uint64_t)per_cu * (uint64_t)cus);
+        cache[dev].store(g, std::memory_order_relaxed);
+    }
+    return g;
+}
+
+void window_region_prepare() {
+    (void)region_grid<128>();
+    (void)region_grid<WAVE_WIN_MAX>();
+}
+
 void launch_window_region(uint32_t S, uint32_t nl, const gkey *pk, const gval *pv, const uint32_t *pw, uint8_t *pd,
                           uint64_t n_ub, uint32_t *d_lvl, gkey *keys, gval *values, gkey *seps, uint32_t *row_offset,
                           const LevelBounds &lb, const uint32_t *d_gate, uint32_t gate, hipStream_t st) {
     if (!n_ub || nl == 0) return;
-    const dim3 grid((uint32_t)std::min<uint64_t>(n_ub, WINDOW_GRID));
-    hipLaunchKernelGGL(k_window_region, grid, dim3(256), 0, st, S, nl, pk, pv, pw, pd, d_lvl, keys, values, seps,
-                       row_offset, lb, d_gate, gate);
+    // LDS rows sized for the region (128 slots at three device levels of 16-slot
+    // segments), and as many workgroups as the GPU holds at once: each wave walks
+    // its share of the regions one after another
+    const bool narrow = ((uint64_t)S << nl) <= 128;
+    const dim3 grid((uint32_t)std::min<uint64_t>(n_ub, narrow ? region_grid<128>() : region_grid<WAVE_WIN_MAX>()));
+    if (narrow)
+        hipLaunchKernelGGL(k_window_region<128>, grid, dim3(256), 0, st, S, nl, pk, pv, pw, pd, d_lvl, keys, values,
+                           seps, row_offset, lb, d_gate, gate);
+    else
+        hipLaunchKernelGGL(k_window_region<WAVE_WIN_MAX>, grid, dim3(256), 0, st, S, nl, pk, pv, pw, pd, d_lvl, keys,
+                           values, seps, row_offset, lb, d_gate, gate);
 }
 
 void launch_window(uint32_t S, uint32_t W, const gkey *pk, const gval *pv, const uint32_t *pw, uint8_t *pd,
