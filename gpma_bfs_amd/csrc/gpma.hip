@@ -242,7 +242,7 @@ struct gpma_handle {
     std::vector<StepGraph> graphs;
     bool capturing = false;
     uint32_t step_blevels = 8;  // BFS levels a new capture holds (grows to the depth seen)
-    uint32_t step_ulevels = 5;  // rebalance levels a new capture holds (grows when a replay needed more)
+    uint32_t step_ulevels = 4;  // rebalance levels a new capture holds (grows when a replay needed more)
     uint64_t step_clock = 0;
     Buf step_n;                    // device u32: records of the step being replayed
     Buf step_stage;                // device: BFS counts and reachable counts gathered for the read-back
