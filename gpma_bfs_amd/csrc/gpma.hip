@@ -1157,10 +1157,10 @@ int capture_step(gpma_handle *g, uint32_t cls, const MsdPlan &pl, uint32_t uleve
     g->ms_wide = false;  // graphs hold at most 64 BFS levels: u16 staging only
     HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     g->capturing = true;
-    fill_u32(ctr, 0, C_COUNT, st);
-    // each replay reuses the same tags, so the status words start clear
-    fill_u64(g->ccstat.as<uint64_t>(), 0, g->ccstat.bytes / 8, st);
-    fill_u64(g->msd_status.as<uint64_t>(), 0, g->msd_status.bytes / 8, st);
+    // the counters, and the look-back status words (each replay reuses the
+    // same tags, so they start clear), in one launch
+    zero3(ctr, C_COUNT, g->ccstat.as<uint64_t>(), g->ccstat.bytes / 8, g->msd_status.as<uint64_t>(),
+          g->msd_status.bytes / 8, st);
     g->cc_ticket = 0;
     const uint32_t spec = enqueue_update(g, g->in_k.as<gkey>(), g->in_v.as<gval>(), 1ull << cls,
                                          g->step_n.as<uint32_t>(), false, &pl, ulevels);

@@ -69,6 +69,13 @@ __global__ void k_fill_u64(uint64_t *p, uint64_t v, uint64_t n) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = v;
 }
+// Zeroes a[0..na) (u32) and b[0..nb), c[0..nc) (u64) in one launch.
+__global__ void k_zero3(uint32_t *a, uint64_t na, uint64_t *b, uint64_t nb, uint64_t *c, uint64_t nc) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, st = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = t; i < na; i += st) a[i] = 0;
+    for (uint64_t i = t; i < nb; i += st) b[i] = 0;
+    for (uint64_t i = t; i < nc; i += st) c[i] = 0;
+}
 
 uint32_t grid_for(uint64_t n, uint32_t threads, uint32_t cap = 4096) {
     uint64_t b = (n + threads - 1) / threads;
@@ -1668,6 +1675,12 @@ void fill_u8(uint8_t *p, uint8_t v, uint64_t n, hipStream_t st) {
 void fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t st) {
     if (!n) return;
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256, 2048)), dim3(256), 0, st, p, v, n);
+}
+
+void zero3(uint32_t *a, uint64_t na, uint64_t *b, uint64_t nb, uint64_t *c, uint64_t nc, hipStream_t st) {
+    const uint64_t n = std::max(na, std::max(nb, nc));
+    if (!n) return;
+    hipLaunchKernelGGL(k_zero3, dim3(grid_for(n, 256, 2048)), dim3(256), 0, st, a, na, b, nb, c, nc);
 }
 
 void launch_locate(const gkey *ks, const gval *vs, uint64_t n, const uint32_t *dn, gkey *keys, gval *values,
