@@ -1262,10 +1262,13 @@ int run_step(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint64
     TRY(step_graph(g, cls, nsrc, levels && levels_on_device ? levels : nullptr, &sg, &out, &blevels));
     sg->used = ++g->step_clock;
     // stage the step's inputs where the graph reads them
-    const hipMemcpyKind kin = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-    HIP_TRY(hipMemcpyAsync(g->in_k.p, keys, n * 8, kin, st));
-    HIP_TRY(hipMemcpyAsync(g->in_v.p, values, n, kin, st));
-    fill_u32(g->step_n.as<uint32_t>(), (uint32_t)n, 1, st);
+    if (on_device) {
+        stage_step(g->in_k.as<uint64_t>(), keys, g->in_v.as<uint8_t>(), values, n, g->step_n.as<uint32_t>(), st);
+    } else {
+        HIP_TRY(hipMemcpyAsync(g->in_k.p, keys, n * 8, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(g->in_v.p, values, n, hipMemcpyHostToDevice, st));
+        fill_u32(g->step_n.as<uint32_t>(), (uint32_t)n, 1, st);
+    }
     if (nsrc) HIP_TRY(hipMemcpyAsync(g->ms_starts.p, starts, nsrc * 4, hipMemcpyHostToDevice, st));
     const uint32_t spec = sg->spec;
     // the graph ends with its own publish: counters (left cleared on the

@@ -18,6 +18,9 @@ struct SortShifts {
 void fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st);
 void fill_u8(uint8_t *p, uint8_t v, uint64_t n, hipStream_t st);
 void fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t st);
+// dk[0..n) = sk[0..n), dv[0..n) = sv[0..n) and *dn = n by one launch (device buffers)
+void stage_step(uint64_t *dk, const uint64_t *sk, uint8_t *dv, const uint8_t *sv, uint64_t n, uint32_t *dn,
+                hipStream_t st);
 // a[0..na) (u32), b[0..nb) and c[0..nc) (u64) set to zero by one launch
 void zero3(uint32_t *a, uint64_t na, uint64_t *b, uint64_t nb, uint64_t *c, uint64_t nc, hipStream_t st);
 

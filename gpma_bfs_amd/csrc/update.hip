@@ -69,6 +69,18 @@ __global__ void k_fill_u64(uint64_t *p, uint64_t v, uint64_t n) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = v;
 }
+// A step's device inputs into the handle's buffers, and its record count,
+// in one launch.
+__global__ void k_stage_step(uint64_t *__restrict__ dk, const uint64_t *__restrict__ sk, uint8_t *__restrict__ dv,
+                             const uint8_t *__restrict__ sv, uint64_t n, uint32_t *__restrict__ dn) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, st = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = t; i < n; i += st) {
+        dk[i] = sk[i];
+        dv[i] = sv[i];
+    }
+    if (t == 0) *dn = (uint32_t)n;
+}
+
 // Zeroes a[0..na) (u32) and b[0..nb), c[0..nc) (u64) in one launch.
 __global__ void k_zero3(uint32_t *a, uint64_t na, uint64_t *b, uint64_t nb, uint64_t *c, uint64_t nc) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, st = (uint64_t)gridDim.x * blockDim.x;
@@ -1675,6 +1687,11 @@ void fill_u8(uint8_t *p, uint8_t v, uint64_t n, hipStream_t st) {
 void fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t st) {
     if (!n) return;
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256, 2048)), dim3(256), 0, st, p, v, n);
+}
+
+void stage_step(uint64_t *dk, const uint64_t *sk, uint8_t *dv, const uint8_t *sv, uint64_t n, uint32_t *dn,
+                hipStream_t st) {
+    hipLaunchKernelGGL(k_stage_step, dim3(grid_for(n, 256, 2048)), dim3(256), 0, st, dk, sk, dv, sv, n, dn);
 }
 
 void zero3(uint32_t *a, uint64_t na, uint64_t *b, uint64_t nb, uint64_t *c, uint64_t nc, hipStream_t st) {
