@@ -1165,12 +1165,18 @@ int capture_step(gpma_handle *g, uint32_t cls, const MsdPlan &pl, uint32_t uleve
     const uint32_t spec = enqueue_update(g, g->in_k.as<gkey>(), g->in_v.as<gval>(), 1ull << cls,
                                          g->step_n.as<uint32_t>(), false, &pl, ulevels);
     *spec_out = spec;
-    launch_step_gate(ctr + C_LVL, ctr + C_LVL + spec + 1, ctr + C_SORTERR, g->bulk_threshold, ctr + C_GATE, st);
+    if (!nsrc)
+        launch_step_gate(ctr + C_LVL, ctr + C_LVL + spec + 1, ctr + C_SORTERR, g->bulk_threshold, ctr + C_GATE, st);
     if (nsrc) {
         const MsState m = ms_state(g);
         const unsigned long long thr = ms_pull_threshold(g);
-        ms_init(m, g->N, g->ms_starts.as<uint32_t>(), nsrc, g->row_offset.as<uint32_t>(), st, ctr + C_GATE);
-        ms_clear(m, 2, blevels + 2, st);
+        MsStepGate sg;
+        sg.np0 = ctr + C_LVL;
+        sg.np_top = ctr + C_LVL + spec + 1;
+        sg.err = ctr + C_SORTERR;
+        sg.bulk = g->bulk_threshold;
+        ms_init_step(m, g->N, g->ms_starts.as<uint32_t>(), nsrc, g->row_offset.as<uint32_t>(), sg, ctr + C_GATE, 2,
+                     blevels + 2, st);
         for (uint32_t L = 1; L <= blevels; L++)
             ms_level(g->keys.as<gkey>(), g->values.as<gval>(), g->C, g->N, g->row_offset.as<uint32_t>(), m, L, thr,
                      st);

@@ -245,6 +245,17 @@ struct MsState {
 // an empty frontier.
 void ms_init(const MsState &m, uint32_t N, const uint32_t *starts, uint32_t nsrc, const uint32_t *ro, hipStream_t st,
              const uint32_t *gate = nullptr);
+// A captured step's form of ms_init: the source setup also computes the
+// step's host-work flag (*gate = 1 when *np0 >= bulk, *np_top != 0 or
+// *err != 0; then no source is queued) and clears the counts of levels
+// clear_a..clear_b-1 (what launch_step_gate and ms_clear did as launches of
+// their own).
+struct MsStepGate {
+    const uint32_t *np0 = nullptr, *np_top = nullptr, *err = nullptr;
+    uint32_t bulk = 0;
+};
+void ms_init_step(const MsState &m, uint32_t N, const uint32_t *starts, uint32_t nsrc, const uint32_t *ro,
+                  const MsStepGate &sg, uint32_t *gate, uint32_t clear_a, uint32_t clear_b, hipStream_t st);
 void ms_clear(const MsState &m, uint32_t a, uint32_t b, hipStream_t st);
 void ms_level(const gkey *keys, const gval *values, uint64_t C, uint32_t N, const uint32_t *ro, const MsState &m,
               uint32_t level, unsigned long long pull_threshold, hipStream_t st);

@@ -82,9 +82,23 @@ __global__ void k_ms_sources(const uint32_t *__restrict__ starts, uint32_t nsrc,
                              uint32_t *__restrict__ q,
                              uint32_t *__restrict__ qbig, unsigned long long *__restrict__ cnt,
                              unsigned long long *__restrict__ cnt_big, uint32_t *__restrict__ tcnt,
-                             const uint32_t *__restrict__ ro, const uint32_t *__restrict__ gate) {
+                             const uint32_t *__restrict__ ro, uint32_t *__restrict__ gate, MsStepGate sg,
+                             uint32_t ca, uint32_t cb) {
     __shared__ uint32_t ss[WAVE];
-    if (gate && *gate) nsrc = 0;
+    if (sg.np0) {
+        // a captured step: its host-work flag (the update left work for the
+        // host), which also stands this traversal down
+        const uint32_t g = (*sg.np0 >= sg.bulk || *sg.np_top != 0 || *sg.err != 0) ? 1u : 0u;
+        if (threadIdx.x == 0) *gate = g;
+        if (g) nsrc = 0;
+    } else if (gate && *gate) {
+        nsrc = 0;
+    }
+    for (uint32_t i = ca + threadIdx.x; i < cb; i += WAVE) {  // the counts of levels ca..cb-1
+        cnt[i] = 0;
+        cnt_big[i] = 0;
+        tcnt[i] = 0;
+    }
     const uint32_t lane = threadIdx.x;  // one wave
     const bool valid = lane < nsrc;
     const uint32_t s = valid ? starts[lane] : 0;
@@ -619,7 +633,14 @@ void ms_init(const MsState &m, uint32_t N, const uint32_t *starts, uint32_t nsrc
              const uint32_t *gate) {
     hipLaunchKernelGGL(k_ms_init, dim3(2048), dim3(256), 0, st, m.seen, m.fr[0], m.fr[1], m.nx, m.evn, N);
     hipLaunchKernelGGL(k_ms_sources, dim3(1), dim3(64), 0, st, starts, nsrc, m.seen, m.fr[1], m.ev, m.evn, m.q[1],
-                       m.qbig[1], m.cnt, m.cnt_big, m.tcnt, ro, gate);
+                       m.qbig[1], m.cnt, m.cnt_big, m.tcnt, ro, const_cast<uint32_t *>(gate), MsStepGate{}, 0u, 0u);
+}
+
+void ms_init_step(const MsState &m, uint32_t N, const uint32_t *starts, uint32_t nsrc, const uint32_t *ro,
+                  const MsStepGate &sg, uint32_t *gate, uint32_t clear_a, uint32_t clear_b, hipStream_t st) {
+    hipLaunchKernelGGL(k_ms_init, dim3(2048), dim3(256), 0, st, m.seen, m.fr[0], m.fr[1], m.nx, m.evn, N);
+    hipLaunchKernelGGL(k_ms_sources, dim3(1), dim3(64), 0, st, starts, nsrc, m.seen, m.fr[1], m.ev, m.evn, m.q[1],
+                       m.qbig[1], m.cnt, m.cnt_big, m.tcnt, ro, gate, sg, clear_a, clear_b);
 }
 
 void ms_clear(const MsState &m, uint32_t a, uint32_t b, hipStream_t st) {
