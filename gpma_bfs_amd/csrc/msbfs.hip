@@ -84,7 +84,6 @@ __global__ void k_ms_sources(const uint32_t *__restrict__ starts, uint32_t nsrc,
                              unsigned long long *__restrict__ cnt_big, uint32_t *__restrict__ tcnt,
                              const uint32_t *__restrict__ ro, uint32_t *__restrict__ gate, MsStepGate sg,
                              uint32_t ca, uint32_t cb) {
-    __shared__ uint32_t ss[WAVE];
     if (sg.np0) {
         // a captured step: its host-work flag (the update left work for the
         // host), which also stands this traversal down
@@ -102,20 +101,21 @@ __global__ void k_ms_sources(const uint32_t *__restrict__ starts, uint32_t nsrc,
     const uint32_t lane = threadIdx.x;  // one wave
     const bool valid = lane < nsrc;
     const uint32_t s = valid ? starts[lane] : 0;
-    ss[lane] = s;
-    __syncthreads();
-    bool first = valid;
-    for (uint32_t j = 0; j < lane; j++)
-        if (ss[j] == s) first = false;
-    if (valid) {
-        atomicOr((unsigned long long *)(seen + s), 1ull << lane);
-        atomicOr((unsigned long long *)(fr + s), 1ull << lane);
+    // m: the lanes (sources) with this lane's start, from one ballot per bit
+    uint64_t m = ballot(valid);
+#pragma unroll
+    for (uint32_t b = 0; b < 32; b++) {
+        const bool bit = (s >> b) & 1u;
+        const uint64_t bb = ballot(valid && bit);
+        m &= bit ? bb : ~bb;
     }
-    if (first) {  // level 1: one event per distinct start, all of its sources
-        uint64_t m = 0;
-        for (uint32_t j = lane; j < nsrc; j++)
-            if (ss[j] == s) m |= 1ull << j;
-        ev[s] = make_uint4((uint32_t)m, (uint32_t)(m >> 32), 1u, 0u);  // slot 0 of vertex s
+    // the first lane of each distinct start writes its masks and its level-1
+    // event (slot 0 of vertex s): all of its sources
+    const bool first = valid && (m & lanemask_lt()) == 0;
+    if (first) {
+        seen[s] = m;
+        fr[s] = m;
+        ev[s] = make_uint4((uint32_t)m, (uint32_t)(m >> 32), 1u, 0u);
         evn[s] = 1;
     }
     const uint64_t len = first ? rlen(ro, s) : 0;
