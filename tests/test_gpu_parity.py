@@ -67,6 +67,99 @@ def test_small_protocol_layout_exact(seed, N, draw_scale):
         assert reach == oreach and np.array_equal(lv, olv), f"bfs from {start}"
 
 
+def _model_check(g, model, N, tag):
+    """The live keys and their stored value bytes against a dict model
+    (guards and the two KEY_MAX sentinels added)."""
+    guards = (np.arange(N, dtype=np.uint64) << np.uint64(32)) | np.uint64(0xFFFFFFFF)
+    expect = np.sort(np.concatenate([np.array(list(model), np.uint64), guards,
+                                     np.array([G.KEY_MAX, G.KEY_MAX], np.uint64)]))
+    assert np.array_equal(g.export_live(), expect), f"{tag}: live set"
+    k, v, _ = g.copy_layout()
+    sel = (k != np.uint64(G.KEY_NONE)) & (v != 0) & ((k & np.uint64(0xFFFFFFFF)) != np.uint64(0xFFFFFFFF)) & \
+        (k != np.uint64(G.KEY_MAX))
+    stored = dict(zip(k[sel].tolist(), v[sel].tolist()))
+    assert stored == model, f"{tag}: stored value bytes"
+
+
+def test_random_mixed_batches_layout_exact():
+    """Random batches mixing every kind of record: inserts of new keys,
+    modifications of live keys to other non-zero values, deletes of live
+    keys, deletes of absent keys and revivals of tombstones (gpma.cuh:188-199
+    and the leaf merge). One record per key per batch: the layout equals
+    the oracle slot for slot after every batch, and the live set and stored
+    value bytes equal a model. Then batches with repeated keys (the last
+    record of a key applies, a documented difference from the reference's
+    parallel locate) and fused batches (gpma_update_batches), against the
+    model."""
+    rng = np.random.default_rng(41)
+    N = 300
+    g = G.GPMA(N)
+    o = O.OracleGPMA(N)
+    model = {}
+
+    def rand_keys(n):
+        a = rng.integers(0, N, n).astype(np.uint64)
+        b = rng.integers(0, N, n).astype(np.uint64)
+        return ((a << np.uint64(32)) | b)[a != b]
+
+    def batch(dups):
+        live = np.array(sorted(model), np.uint64)
+        ks, vs = [], []
+        new = rand_keys(int(rng.integers(1, 500)))
+        ks.append(new)
+        vs.append(rng.choice(np.array([1, 2, 7], np.uint8), len(new)))
+        if len(live):
+            d = rng.choice(live, size=min(len(live), int(rng.integers(0, 350))), replace=False)
+            ks.append(d)
+            vs.append(np.zeros(len(d), np.uint8))
+            m = rng.choice(live, size=min(len(live), int(rng.integers(0, 60))), replace=False)
+            ks.append(m)
+            vs.append(rng.choice(np.array([3, 9], np.uint8), len(m)))
+        a = rand_keys(int(rng.integers(0, 60)))
+        ks.append(a)
+        vs.append(np.zeros(len(a), np.uint8))
+        k, v = np.concatenate(ks), np.concatenate(vs)
+        if not dups:
+            _, first = np.unique(k, return_index=True)
+            first.sort()
+            k, v = k[first], v[first]
+        else:  # repeat some of the batch's keys with other values
+            r = rng.choice(len(k), size=min(len(k), 40), replace=False)
+            k = np.concatenate([k, k[r]])
+            v = np.concatenate([v, rng.choice(np.array([0, 1, 5], np.uint8), len(r))])
+        p = rng.permutation(len(k))
+        return k[p], v[p]
+
+    def apply(k, v):
+        for kk, vv in zip(k.tolist(), v.tolist()):
+            if vv:
+                model[kk] = vv
+            else:
+                model.pop(kk, None)
+
+    for step in range(40):
+        k, v = batch(dups=False)
+        G.update_gpma(g, k, v)
+        o.update(k, v)
+        apply(k, v)
+        assert_layout_equal(g, o, f"batch {step}")
+        _model_check(g, model, N, f"batch {step}")
+    for step in range(10):
+        k, v = batch(dups=True)
+        assert len(np.unique(k)) < len(k)
+        G.update_gpma(g, k, v)
+        apply(k, v)
+        _model_check(g, model, N, f"repeated keys {step}")
+    for step in range(5):
+        parts = [batch(dups=True) for _ in range(int(rng.integers(2, 9)))]
+        G.update_gpma_batches(g, [p[0] for p in parts], [p[1] for p in parts])
+        for k, v in parts:
+            apply(k, v)
+        _model_check(g, model, N, f"fused {step}")
+    assert_invariants(g)
+    g.close()
+
+
 def test_c1_protocol():
     """R-MAT 2^16 / 2^20 (seed 1), full 100-slide protocol: live set after every
     batch equals the closed form; layout equals the oracle every 10 slides;
