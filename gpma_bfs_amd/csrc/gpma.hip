@@ -148,6 +148,11 @@ constexpr uint32_t C_LVL_SLOTS = C_COUNT - C_LVL;
 constexpr uint64_t MSD_MAX_N = 4ull << 20;
 constexpr uint32_t MSD_BIG_BUCKET = 16384;
 constexpr uint32_t MSD_COOL_BATCHES = 64;
+// Updates between refreshes of the row splitters of the bucket sort. Between
+// refreshes the guards move only inside the windows the updates rebalance,
+// so a stale table shifts a bucket boundary by less than a window; captured
+// steps refresh the table at their start.
+constexpr uint32_t SPL_EVERY = 8;
 // Levels 1..spec_levels (default SPEC_LEVELS) run without a host round trip;
 // the host reads the counters once after them and drives any higher level
 // itself. Each device level costs a launch on every batch (one that no insert
@@ -192,6 +197,8 @@ struct gpma_handle {
     Buf wide;                // per-window scratch of the wide rebalance levels
     Buf ccstat;              // look-back status words of the chained compaction
     Buf msd_kp, msd_rk, msd_mat, msd_bstart, msd_status;  // bucket sort scratch
+    Buf msd_spl;             // row splitters of the bucket sort (SPL_TABLE u32)
+    uint32_t spl_age = SPL_EVERY;  // updates since the splitters were computed (SPL_EVERY: refresh first)
     uint32_t msd_cool = 0;   // batches left on the LSD sort after a skewed batch
     uint32_t cc_epoch = 0;   // tag of the latest compaction call (never 0)
     uint32_t cc_ticket = 0;  // next ticket slot of this update
@@ -513,6 +520,7 @@ int rebuild(gpma_handle *g, const gkey *pk, const gval *pv, uint64_t np, bool ch
         TRY(replace_arrays(g, T, c, old));
     }
     g->C = T;
+    g->spl_age = SPL_EVERY;  // the rows moved: new splitters before the next sort
     if (check_bounds) g->resizes++;
     else g->bulk++;
     launch_merge_redistribute(g->tmpk.as<gkey>(), g->tmpv.as<gval>(), c, pk, pv, np, g->keys.as<gkey>(),
@@ -608,6 +616,7 @@ int ensure_msd(gpma_handle *g, uint64_t n, const MsdPlan &pl) {
     TRY(g->msd_rk.ensure(n * 4, t));
     TRY(g->msd_mat.ensure(2ull * pl.ntiles * pl.NB * 4, t));
     TRY(g->msd_bstart.ensure(((uint64_t)pl.NB + 1) * 4, t));
+    TRY(g->msd_spl.ensure((uint64_t)SPL_TABLE * 4, t));
     void *before = g->msd_status.p;
     TRY(g->msd_status.ensure((uint64_t)pl.nscan * 8, t));
     if (g->msd_status.p != before) fill_u64(g->msd_status.as<uint64_t>(), 0, g->msd_status.bytes / 8, g->st);
@@ -630,10 +639,18 @@ uint32_t enqueue_update(gpma_handle *g, const gkey *src_k, const gval *src_v, ui
     if (pl) {
         MsdWork w{g->msd_kp.as<uint64_t>(), g->msd_rk.as<uint32_t>(), g->sk1.as<uint64_t>(), g->msd_mat.as<uint32_t>(),
                   g->msd_bstart.as<uint32_t>(), g->msd_status.as<unsigned long long>()};
+        // the row splitters: refreshed every SPL_EVERY updates, after a
+        // rebuild, and at the start of every captured step
+        const bool refresh = pl->by_row && (g->capturing || g->spl_age >= SPL_EVERY);
+        if (refresh) {
+            msd_splitters(g->row_offset.as<uint32_t>(), g->N, g->C, g->msd_spl.as<uint32_t>(), st);
+            g->spl_age = 0;
+        }
+        if (pl->by_row) g->spl_age++;
         msd_sort(src_k, src_v, g->sk0.as<gkey>(), g->sv0.as<gval>(), *pl, w, n_ub, d_n, g->N, guards_ok,
-                 g->row_offset.as<uint32_t>(), g->C, ctr + C_INVALID, next_epoch(g), ctr + C_MSD_TICKET,
-                 ctr + C_SORTERR, ctr + C_MSD_MAXB, st);
-        phase_end(g, P_SORT, 4);
+                 g->msd_spl.as<uint32_t>(), ctr + C_INVALID, next_epoch(g), ctr + C_MSD_TICKET, ctr + C_SORTERR,
+                 ctr + C_MSD_MAXB, st);
+        phase_end(g, P_SORT, refresh ? 5 : 4);
     } else {
         // digits: bits of N-1 plus one (so KEY_NONE sorts after every valid key)
         const uint32_t nb = std::max<uint32_t>(1, fls32(g->N ? g->N - 1 : 0)) + 1;
@@ -1130,7 +1147,7 @@ std::vector<uint64_t> step_signature(const gpma_handle *g, uint32_t cls, uint32_
                          &g->in_k, &g->in_v,
                          &g->sk0, &g->sv0, &g->sk1, &g->win, &g->pk0, &g->pv0, &g->pw0, &g->pd0, &g->pk1, &g->pv1,
                          &g->pw1, &g->pd1, &g->ccstat, &g->msd_kp, &g->msd_rk, &g->msd_mat, &g->msd_bstart,
-                         &g->msd_status, &g->ms_seen, &g->ms_fr0, &g->ms_fr1, &g->ms_nx, &g->ms_lv16, &g->ms_q0,
+                         &g->msd_status, &g->msd_spl, &g->ms_seen, &g->ms_fr0, &g->ms_fr1, &g->ms_nx, &g->ms_lv16, &g->ms_q0,
                          &g->ms_q1, &g->ms_qb0, &g->ms_qb1, &g->ms_touched, &g->ms_cnt, &g->ms_cntb, &g->ms_tcnt,
                          &g->ms_partial, &g->ms_starts, &g->ms_reach, &g->ms_ev, &g->ms_evn};
     for (const Buf *b : bufs) {
@@ -1575,7 +1592,7 @@ int gpma_destroy(gpma_t g) {
     if (g->st) hipStreamSynchronize(g->st);
     Buf *all[] = {&g->keys, &g->values, &g->seps, &g->row_offset, &g->in_k, &g->in_v, &g->sk0, &g->sv0,
                   &g->sk1, &g->sv1, &g->win, &g->blkcnt, &g->blkoff, &g->sortwork, &g->msd_kp, &g->msd_rk,
-                  &g->msd_mat, &g->msd_bstart, &g->msd_status, &g->ccstat, &g->pk0, &g->pv0,
+                  &g->msd_mat, &g->msd_bstart, &g->msd_status, &g->msd_spl, &g->ccstat, &g->pk0, &g->pv0,
                   &g->pw0, &g->pd0, &g->pk1, &g->pv1, &g->pw1, &g->pd1, &g->tmpk, &g->tmpv, &g->chunkcnt,
                   &g->chunkoff, &g->lev, &g->fr0, &g->fr1, &g->fb0, &g->fb1, &g->bcnt, &g->bcntb,
                   &g->ms_seen, &g->ms_fr0, &g->ms_fr1, &g->ms_nx, &g->ms_lv16, &g->ms_q0, &g->ms_q1, &g->ms_qb0,

@@ -61,14 +61,20 @@ struct MsdWork {
     uint32_t *bstart;           // NB + 1 bucket starts
     unsigned long long *status; // nscan look-back words (zeroed once)
 };
-// Sorted (ok, ov) from (ik, iv); same contract as stage_and_sort. epoch:
+// Row splitters of the row buckets: SPL_TABLE u32 entries from row_offset
+// (N + 1 entries) for an array of C slots (k_msd_splitters).
+constexpr uint32_t SPL_BITS = 12;
+constexpr uint32_t SPL_TABLE = 1u << SPL_BITS;
+void msd_splitters(const uint32_t *row_offset, uint32_t N, uint64_t C, uint32_t *spl, hipStream_t st);
+// Sorted (ok, ov) from (ik, iv); same contract as stage_and_sort. spl: the
+// splitter table (by_row plans). epoch:
 // non-zero tag distinct from earlier calls on the same status words;
 // *ticket zero before the call; *maxb raised to the largest bucket above 1024
 // items. dn (optional): the batch size is read from *dn on the device, and
 // n and the plan are an upper bound of it (launch shapes fixed in advance).
 void msd_sort(const gkey *ik, const gval *iv, gkey *ok, gval *ov, const MsdPlan &pl, const MsdWork &w, uint64_t n,
-              const uint32_t *dn, uint32_t N, bool guards_ok, const uint32_t *row_offset, uint64_t C,
-              uint32_t *invalid, uint32_t epoch, uint32_t *ticket, uint32_t *err, uint32_t *maxb, hipStream_t st);
+              const uint32_t *dn, uint32_t N, bool guards_ok, const uint32_t *spl, uint32_t *invalid, uint32_t epoch,
+              uint32_t *ticket, uint32_t *err, uint32_t *maxb, hipStream_t st);
 
 // ---- update.hip
 constexpr uint32_t CMP_BLOCK = 256;  // items per compaction block

@@ -37,6 +37,7 @@ constexpr uint32_t MWI = WAVE * MIPT;        // items per wave
 constexpr uint32_t MTILE = MT * MIPT;        // items per tile
 constexpr uint32_t MAX_D = 12;               // bucket bits
 constexpr uint32_t MAX_NB = 1u << MAX_D;
+static_assert(SPL_BITS == MAX_D, "the splitter table serves every bucket width up to MAX_D");
 constexpr uint32_t TINY = 128;               // buckets ranked by plain comparison in LDS
 constexpr uint32_t SMALL = 1024;             // buckets sorted in LDS (runs of 64, then a merge rank)
 constexpr uint32_t RUN = WAVE;               // items per register-sorted run
@@ -68,21 +69,34 @@ __device__ __forceinline__ void unpack_key(uint64_t p, const PackInfo &pi, gkey 
 
 // Tiles of MT * IPT items (IPT items per lane): the plan picks IPT from the
 // batch size so that small batches still spread over every CU. The bucket of
-// an item is the top D bits of its packed key, or, with ro (row buckets), its
-// row's slot position in the array scaled to D bits (ro[src] >> rs, or << -rs),
-// which follows the key order as well and stays balanced when the sources are
-// skewed (R-MAT). rk[i] = bucket << 16 | rank inside (tile, bucket).
+// an item is the top D bits of its packed key, or, with spl (row buckets), the
+// last bucket whose splitter row is <= the item's row: the splitters cut the
+// rows where their slot positions cross multiples of C / 2^D (k_msd_splitters),
+// so the buckets follow the key order and stay balanced when the sources are
+// skewed (R-MAT). The table sits in LDS: no per-item global read.
+// rk[i] = bucket << 16 | rank inside (tile, bucket).
 template <uint32_t IPT>
 __global__ __launch_bounds__(MT) void k_msd_rank(const gkey *__restrict__ ik, const gval *__restrict__ iv, uint64_t n,
                                                  const uint32_t *__restrict__ dn, PackInfo pi, uint32_t bshift,
-                                                 uint32_t D, const uint32_t *__restrict__ ro, int32_t rs,
+                                                 uint32_t D, const uint32_t *__restrict__ spl,
                                                  uint64_t *__restrict__ kp, uint32_t *__restrict__ rk,
                                                  uint16_t *__restrict__ counts, uint32_t *__restrict__ invalid) {
     constexpr uint32_t TILE = MT * IPT, WI = WAVE * IPT;
     __shared__ uint16_t cnt[MW][MAX_NB];
+    __shared__ uint32_t s_spl[MAX_NB];
     if (dn) n = *dn;
     if ((uint64_t)blockIdx.x * TILE >= n) return;  // tiles past the batch (grid sized for a capacity)
     const uint32_t NB = 1u << D;
+    if (spl) {
+        // splitters 1..NB-1 (every (2^(12 - D))-th of the 12-bit table) in
+        // breadth-first tree order: splitter b sits at depth D - 1 - ctz(b),
+        // so the lanes of one search round read neighbouring words (no bank
+        // conflicts), where a sorted table would put them in one bank
+        for (uint32_t b = 1 + threadIdx.x; b < NB; b += MT) {
+            const uint32_t t = (uint32_t)__builtin_ctz(b);
+            s_spl[(1u << (D - 1 - t)) + (b >> (t + 1))] = spl[b << (SPL_BITS - D)];
+        }
+    }
     {  // zero the first NB counters of every wave's row (NB >= 2)
         const uint32_t hs = D - 1;  // log2 of the u32 words in use per row
         for (uint32_t i = threadIdx.x; i < (MW << hs); i += MT)
@@ -109,19 +123,30 @@ __global__ __launch_bounds__(MT) void k_msd_rank(const gkey *__restrict__ ik, co
         b[j] = NB;  // no item
         if (idx < n) {
             p[j] = pack_key(k[j], v[j], pi, bad);
-            if (!ro) {
-                b[j] = (uint32_t)(p[j] >> bshift) & (NB - 1);
-            } else if (bad) {
-                b[j] = NB - 1;  // packs as all ones: after every valid key
-            } else {
-                const uint32_t pos = ro[(uint32_t)(k[j] >> 32)];
-                b[j] = min(NB - 1, rs >= 0 ? pos >> rs : pos << -rs);
-            }
+            if (!spl) b[j] = (uint32_t)(p[j] >> bshift) & (NB - 1);
+            else if (bad) b[j] = NB - 1;  // packs as all ones: after every valid key
         }
         nbad += (uint32_t)__popcll(ballot(bad));
     }
     if (lane == 0 && nbad) atomicAdd(invalid, nbad);
-    __syncthreads();  // counters zeroed
+    __syncthreads();  // counters zeroed, splitters staged
+    if (spl) {
+        // the bucket = the number of splitters 1..NB-1 <= src (splitter 0
+        // is row 0): D rounds down the tree, the IPT searches interleaved
+        uint32_t x[IPT], src[IPT];
+#pragma unroll
+        for (uint32_t j = 0; j < IPT; j++) {
+            x[j] = 1;
+            src[j] = (uint32_t)(k[j] >> 32);
+        }
+        for (uint32_t d = 0; d < D; d++) {
+#pragma unroll
+            for (uint32_t j = 0; j < IPT; j++) x[j] = 2 * x[j] + (s_spl[x[j]] <= src[j] ? 1u : 0u);
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < IPT; j++)
+            if (b[j] == NB && wbase + (uint64_t)j * WAVE + lane < n) b[j] = x[j] - NB;
+    }
     uint32_t r[IPT];
 #pragma unroll
     for (uint32_t j = 0; j < IPT; j++) r[j] = wave_rank<MAX_D>(b[j], b[j] < NB, cnt[wid]);
@@ -145,6 +170,28 @@ __global__ __launch_bounds__(MT) void k_msd_rank(const gkey *__restrict__ ik, co
             rk[idx] = (b[j] << 16) | (cnt[wid][b[j]] + r[j]);
         }
     }
+}
+
+// Row splitters at SPL_BITS resolution: spl[c] = the first row r whose slot
+// position ro[r] scaled to SPL_BITS bits (ro[r] >> (cs - SPL_BITS), cs =
+// log2 C) is >= c, or N when no row reaches c. One thread per row r in
+// [0, N]: it writes the c in (scale(ro[r - 1]), scale(ro[r])] (row N stands
+// for every c past the last row), so each entry is written exactly once.
+// The bucket of src at D bits is then the last b with spl[b << (SPL_BITS - D)]
+// <= src, which is ro[src] scaled to D bits. Any non-decreasing table sorts
+// correctly; a stale one only balances the buckets less well.
+__global__ void k_msd_splitters(const uint32_t *__restrict__ ro, uint32_t N, int32_t cs, uint32_t *__restrict__ spl) {
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > N) return;
+    constexpr uint32_t TOP = (1u << SPL_BITS) - 1;
+    auto scale = [&](uint32_t pos) {
+        const uint64_t c = cs >= (int32_t)SPL_BITS ? (uint64_t)pos >> (cs - (int32_t)SPL_BITS)
+                                                   : (uint64_t)pos << ((int32_t)SPL_BITS - cs);
+        return (uint32_t)(c < TOP ? c : TOP);
+    };
+    const uint32_t hi = r < N ? scale(ro[r]) : TOP;
+    const uint32_t lo = r > 0 ? scale(ro[r - 1]) + 1 : 0;
+    for (uint32_t c = lo; c <= hi; c++) spl[c] = (uint32_t)r;
 }
 
 // offs[t][b] = items in buckets < b + items of bucket b in tiles < t;
@@ -465,19 +512,24 @@ MsdPlan msd_plan(uint64_t n, uint32_t N, uint32_t bucket_bits, bool by_row, uint
     return pl;
 }
 
-void msd_sort(const gkey *ik, const gval *iv, gkey *ok, gval *ov, const MsdPlan &pl, const MsdWork &w, uint64_t n,
-              const uint32_t *dn, uint32_t N, bool guards_ok, const uint32_t *row_offset, uint64_t C,
-              uint32_t *invalid, uint32_t epoch, uint32_t *ticket, uint32_t *err, uint32_t *maxb, hipStream_t st) {
-    if (n == 0) return;
-    const uint32_t *ro = pl.by_row ? row_offset : nullptr;
+void msd_splitters(const uint32_t *row_offset, uint32_t N, uint64_t C, uint32_t *spl, hipStream_t st) {
     int32_t cs = 0;
     while ((1ull << (cs + 1)) <= C) cs++;  // C is a power of two
-    const int32_t rs = cs - (int32_t)pl.D;
+    const uint64_t rows = (uint64_t)N + 1;
+    hipLaunchKernelGGL(k_msd_splitters, dim3((uint32_t)((rows + 255) / 256)), dim3(256), 0, st, row_offset, N, cs,
+                       spl);
+}
+
+void msd_sort(const gkey *ik, const gval *iv, gkey *ok, gval *ov, const MsdPlan &pl, const MsdWork &w, uint64_t n,
+              const uint32_t *dn, uint32_t N, bool guards_ok, const uint32_t *spl, uint32_t *invalid, uint32_t epoch,
+              uint32_t *ticket, uint32_t *err, uint32_t *maxb, hipStream_t st) {
+    if (n == 0) return;
+    const uint32_t *sp = pl.by_row ? spl : nullptr;
     const PackInfo pi{N, pl.bs, guards_ok ? 1u : 0u};
     uint16_t *counts = reinterpret_cast<uint16_t *>(w.mat);                        // [ntiles][NB] u16
     uint32_t *offs = w.mat + ((uint64_t)pl.ntiles * pl.NB + 1) / 2;             // [ntiles][NB] u32
 #define GPMA_MSD_TILE(IPT)                                                                                    \
-    hipLaunchKernelGGL(k_msd_rank<IPT>, dim3(pl.ntiles), dim3(MT), 0, st, ik, iv, n, dn, pi, pl.bshift, pl.D, ro, rs, \
+    hipLaunchKernelGGL(k_msd_rank<IPT>, dim3(pl.ntiles), dim3(MT), 0, st, ik, iv, n, dn, pi, pl.bshift, pl.D, sp,     \
                        w.kp, w.rk, counts, invalid);                                                              \
     hipLaunchKernelGGL(k_msd_scan, dim3(pl.nscan), dim3(SCAN_T), 0, st, counts, pl.ntiles, MT * IPT, pl.NB, n, dn, \
                        offs, w.bstart, w.status, epoch, ticket, err);                                              \
