@@ -148,10 +148,10 @@ constexpr uint32_t C_LVL_SLOTS = C_COUNT - C_LVL;
 constexpr uint64_t MSD_MAX_N = 4ull << 20;
 constexpr uint32_t MSD_BIG_BUCKET = 16384;
 constexpr uint32_t MSD_COOL_BATCHES = 64;
-// Updates between refreshes of the row splitters of the bucket sort. Between
-// refreshes the guards move only inside the windows the updates rebalance,
-// so a stale table shifts a bucket boundary by less than a window; captured
-// steps refresh the table at their start.
+// Updates (or captured steps) between refreshes of the row splitters of the
+// bucket sort. Between refreshes the guards move only inside the windows the
+// updates rebalance, so a stale table shifts a bucket boundary by little; any
+// table sorts correctly, a stale one only balances the buckets less well.
 constexpr uint32_t SPL_EVERY = 8;
 // Levels 1..spec_levels (default SPEC_LEVELS) run without a host round trip;
 // the host reads the counters once after them and drives any higher level
@@ -639,14 +639,15 @@ uint32_t enqueue_update(gpma_handle *g, const gkey *src_k, const gval *src_v, ui
     if (pl) {
         MsdWork w{g->msd_kp.as<uint64_t>(), g->msd_rk.as<uint32_t>(), g->sk1.as<uint64_t>(), g->msd_mat.as<uint32_t>(),
                   g->msd_bstart.as<uint32_t>(), g->msd_status.as<unsigned long long>()};
-        // the row splitters: refreshed every SPL_EVERY updates, after a
-        // rebuild, and at the start of every captured step
-        const bool refresh = pl->by_row && (g->capturing || g->spl_age >= SPL_EVERY);
+        // the row splitters: refreshed every SPL_EVERY updates and after a
+        // rebuild (a captured step's graph holds none: run_step refreshes
+        // them before a replay on the same schedule)
+        const bool refresh = pl->by_row && !g->capturing && g->spl_age >= SPL_EVERY;
         if (refresh) {
             msd_splitters(g->row_offset.as<uint32_t>(), g->N, g->C, g->msd_spl.as<uint32_t>(), st);
             g->spl_age = 0;
         }
-        if (pl->by_row) g->spl_age++;
+        if (pl->by_row && !g->capturing) g->spl_age++;
         msd_sort(src_k, src_v, g->sk0.as<gkey>(), g->sv0.as<gval>(), *pl, w, n_ub, d_n, g->N, guards_ok,
                  g->msd_spl.as<uint32_t>(), ctr + C_INVALID, next_epoch(g), ctr + C_MSD_TICKET, ctr + C_SORTERR,
                  ctr + C_MSD_MAXB, st);
@@ -1300,6 +1301,12 @@ int run_step(gpma_handle *g, const uint64_t *keys, const uint8_t *values, uint64
     *(volatile uint32_t *)g->sig = 0;
     std::atomic_thread_fence(std::memory_order_seq_cst);
     g->ms_wide = false;  // the replay's traversal stages u16 levels
+    // the bucket sort's row splitters, every SPL_EVERY steps, ahead of the graph
+    if (g->spl_age >= SPL_EVERY) {
+        msd_splitters(g->row_offset.as<uint32_t>(), g->N, g->C, g->msd_spl.as<uint32_t>(), st);
+        g->spl_age = 0;
+    }
+    g->spl_age++;
     HIP_TRY(hipGraphLaunch(sg->exec, st));
     g->sst.replays++;
     TRY(wait_signal(g, SIG_STEP));
