@@ -174,24 +174,42 @@ __global__ __launch_bounds__(MT) void k_msd_rank(const gkey *__restrict__ ik, co
 
 // Row splitters at SPL_BITS resolution: spl[c] = the first row r whose slot
 // position ro[r] scaled to SPL_BITS bits (ro[r] >> (cs - SPL_BITS), cs =
-// log2 C) is >= c, or N when no row reaches c. One thread per row r in
-// [0, N]: it writes the c in (scale(ro[r - 1]), scale(ro[r])] (row N stands
-// for every c past the last row), so each entry is written exactly once.
-// The bucket of src at D bits is then the last b with spl[b << (SPL_BITS - D)]
-// <= src, which is ro[src] scaled to D bits. Any non-decreasing table sorts
-// correctly; a stale one only balances the buckets less well.
+// log2 C) is >= c, or N when no row reaches c. Row r writes the c in
+// (scale(ro[r - 1]), scale(ro[r])] (row N stands for every c past the last
+// row), so each entry is written exactly once; a thread takes 4 rows with one
+// 16-byte load. The bucket of src at D bits is then the last b with
+// spl[b << (SPL_BITS - D)] <= src, which is ro[src] scaled to D bits. Any
+// non-decreasing table sorts correctly; a stale one only balances the buckets
+// less well.
 __global__ void k_msd_splitters(const uint32_t *__restrict__ ro, uint32_t N, int32_t cs, uint32_t *__restrict__ spl) {
-    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r > N) return;
+    const uint64_t r0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    if (r0 > N) return;
     constexpr uint32_t TOP = (1u << SPL_BITS) - 1;
     auto scale = [&](uint32_t pos) {
         const uint64_t c = cs >= (int32_t)SPL_BITS ? (uint64_t)pos >> (cs - (int32_t)SPL_BITS)
                                                    : (uint64_t)pos << ((int32_t)SPL_BITS - cs);
         return (uint32_t)(c < TOP ? c : TOP);
     };
-    const uint32_t hi = r < N ? scale(ro[r]) : TOP;
-    const uint32_t lo = r > 0 ? scale(ro[r - 1]) + 1 : 0;
-    for (uint32_t c = lo; c <= hi; c++) spl[c] = (uint32_t)r;
+    uint32_t p[4];
+    if (r0 + 4 <= N) {  // ro is 256-byte aligned: rows 4t.. sit in one 16-byte word
+        const uint4 x = *reinterpret_cast<const uint4 *>(ro + r0);
+        p[0] = x.x;
+        p[1] = x.y;
+        p[2] = x.z;
+        p[3] = x.w;
+    } else {
+#pragma unroll
+        for (uint32_t j = 0; j < 4; j++) p[j] = r0 + j < N ? ro[r0 + j] : 0u;
+    }
+    uint32_t lo = r0 > 0 ? scale(ro[r0 - 1]) + 1 : 0;
+#pragma unroll
+    for (uint32_t j = 0; j < 4; j++) {
+        const uint64_t r = r0 + j;
+        if (r > N) break;
+        const uint32_t hi = r < N ? scale(p[j]) : TOP;
+        for (uint32_t c = lo; c <= hi; c++) spl[c] = (uint32_t)r;
+        lo = hi + 1;
+    }
 }
 
 // offs[t][b] = items in buckets < b + items of bucket b in tiles < t;
@@ -515,8 +533,8 @@ MsdPlan msd_plan(uint64_t n, uint32_t N, uint32_t bucket_bits, bool by_row, uint
 void msd_splitters(const uint32_t *row_offset, uint32_t N, uint64_t C, uint32_t *spl, hipStream_t st) {
     int32_t cs = 0;
     while ((1ull << (cs + 1)) <= C) cs++;  // C is a power of two
-    const uint64_t rows = (uint64_t)N + 1;
-    hipLaunchKernelGGL(k_msd_splitters, dim3((uint32_t)((rows + 255) / 256)), dim3(256), 0, st, row_offset, N, cs,
+    const uint64_t threads = ((uint64_t)N + 1 + 3) / 4;  // 4 rows each, rows 0..N
+    hipLaunchKernelGGL(k_msd_splitters, dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, st, row_offset, N, cs,
                        spl);
 }
 
