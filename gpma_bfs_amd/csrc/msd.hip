@@ -9,8 +9,10 @@
 //             one u64 carries key and value; a guard's dst (0xFFFFFFFF)
 //             packs as row_num and an out-of-range key as all ones, which
 //             keeps the order of the unpacked keys.
-//   rank      (k_msd_rank) each tile (1024..4096 items) ranks its items by bucket =
-//             the top D bits of the packed key (wave ballots), writes the
+//   rank      (k_msd_rank) each tile (1024..4096 items) ranks its items by bucket
+//             (wave ballots) = the row's slot position scaled to D bits, read
+//             from a table of row splitters in LDS (k_msd_splitters), or the
+//             top D bits of the packed key for the guard insertion; writes the
 //             packed keys, their rank inside (tile, bucket) and the tile's
 //             bucket counts;
 //   scan      (k_msd_scan) exclusive scan of the counts in (bucket, tile)
